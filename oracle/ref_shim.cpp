@@ -1,0 +1,246 @@
+// ref_shim.cpp -- C entry points into the reference TIPL headers.
+//
+// TEST INFRASTRUCTURE ONLY.  Built by oracle/Makefile against the reference
+// headers where they lie (-I <reference root>) into oracle/_ref/libtiplref.so.
+// It contains no reference code: every function below only instantiates a
+// reference template on plain buffers so that tests can check the C oracle
+// (oracle/oracle.c) against the real implementation, and bench.py can time the
+// reference's CPU path as its baseline.
+#include <algorithm>
+#include <atomic>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "reg/cdm.hpp"
+#include "reg/linear.hpp"
+
+namespace {
+
+tipl::shape<3> shp(const int* d) { return tipl::shape<3>(d[0], d[1], d[2]); }
+
+tipl::transformation_matrix<float, 3> mat(const float* T)
+{
+    tipl::transformation_matrix<float, 3> M;
+    std::copy_n(T, 12, M.begin());
+    return M;
+}
+
+using field_t = tipl::image<3, tipl::vector<3>>;
+
+field_t to_field(const float* f, const int* d)
+{
+    field_t F(shp(d));
+    std::memcpy(&F[0][0], f, sizeof(float) * 3 * F.size());
+    return F;
+}
+
+void from_field(const field_t& F, float* f)
+{
+    std::memcpy(f, &F[0][0], sizeof(float) * 3 * F.size());
+}
+
+}  // namespace
+
+extern "C" {
+
+void tref_set_threads(int n) { tipl::max_thread_count = std::max(1, n); }
+int tref_get_threads(void) { return tipl::max_thread_count; }
+
+void tref_affine_matrix(const float* p, const int* fdim, const float* fvs, const int* tdim,
+                        const float* tvs, float* T)
+{
+    tipl::affine_param<float, 3> arg(p);
+    tipl::transformation_matrix<float, 3> M(arg, shp(fdim), tipl::vector<3>(fvs), shp(tdim),
+                                            tipl::vector<3>(tvs));
+    std::copy_n(M.begin(), 12, T);
+}
+
+int tref_matrix_inverse(float* T)
+{
+    auto M = mat(T);
+    bool ok = M.inverse();
+    std::copy_n(M.begin(), 12, T);
+    return ok ? 1 : 0;
+}
+
+void tref_resample_u8(const uint8_t* from, const int* fdim, uint8_t* to, const int* tdim,
+                      const float* T)
+{
+    tipl::const_pointer_image<3, uint8_t> F(from, shp(fdim));
+    tipl::image<3, uint8_t> out(shp(tdim));
+    tipl::resample(F, out, mat(T));
+    std::memcpy(to, out.data(), out.size());
+}
+
+uint8_t tref_sample_u8(const uint8_t* I, const int* dim, float x, float y, float z)
+{
+    tipl::const_pointer_image<3, uint8_t> F(I, shp(dim));
+    return F[tipl::vector<3>(x, y, z)];
+}
+
+double tref_corr_cost(const uint8_t* from, const int* fdim, const uint8_t* to, const int* tdim,
+                      const float* T)
+{
+    tipl::const_pointer_image<3, uint8_t> F(from, shp(fdim)), G(to, shp(tdim));
+    return tipl::reg::correlation()(F, G, mat(T));
+}
+
+void tref_mi_hist(const uint8_t* from, const int* fdim, const uint8_t* to, const int* tdim,
+                  const float* T, int32_t* hist)
+{
+    tipl::const_pointer_image<3, uint8_t> F(from, shp(fdim)), G(to, shp(tdim));
+    auto h = tipl::reg::get_mutual_info(G, F, mat(T));
+    std::memcpy(hist, h.data(), sizeof(int32_t) * h.size());
+}
+
+double tref_mi_cost(const uint8_t* from, const int* fdim, const uint8_t* to, const int* tdim,
+                    const float* T)
+{
+    tipl::const_pointer_image<3, uint8_t> F(from, shp(fdim)), G(to, shp(tdim));
+    tipl::reg::mutual_information<3> mi;
+    return mi(F, G, mat(T));
+}
+
+// K cost evaluations the way the reference optimisers issue them: one
+// shared cost functor, tipl::par_for over the candidates
+// (numerical/optimization.hpp:21-28, 82-87, 356-404), so each evaluation runs
+// on one thread (nested par_for is serial, mt.hpp:126-129).
+void tref_cost_batch(int kind, const uint8_t* from, const int* fdim, const uint8_t* to,
+                     const int* tdim, const float* T, int K, double* costs)
+{
+    tipl::const_pointer_image<3, uint8_t> F(from, shp(fdim)), G(to, shp(tdim));
+    if (kind == 1) {
+        tipl::reg::mutual_information<3> mi;
+        tipl::par_for(K, [&](int k) { costs[k] = mi(F, G, mat(T + 12 * k)); });
+    } else {
+        tipl::reg::correlation corr;
+        tipl::par_for(K, [&](int k) { costs[k] = corr(F, G, mat(T + 12 * k)); });
+    }
+}
+
+int tref_normalize_f32(const float* in, size_t n, const int* dim, uint8_t* out, float upper)
+{
+    tipl::const_pointer_image<3, float> I(in, shp(dim));
+    tipl::image<3, uint8_t> O;
+    tipl::normalize_upper_lower2(I, O, upper);
+    if (O.empty()) return 0;
+    std::memcpy(out, O.data(), n);
+    return 1;
+}
+
+void tref_downsample_u8(const uint8_t* in, const int* dim, uint8_t* out, int* odim)
+{
+    tipl::const_pointer_image<3, uint8_t> I(in, shp(dim));
+    tipl::image<3, uint8_t> O;
+    tipl::downsample_with_padding(I, O);
+    odim[0] = O.width();
+    odim[1] = O.height();
+    odim[2] = O.depth();
+    std::memcpy(out, O.data(), O.size());
+}
+
+void tref_upsample_f3(const float* in, const int* idim, float* out, const int* odim)
+{
+    field_t F = to_field(in, idim);
+    tipl::upsample_with_padding(F, shp(odim));
+    from_field(F, out);
+}
+
+void tref_compose_displacement(const uint8_t* Is, const int* dim, const float* d, uint8_t* Js)
+{
+    tipl::const_pointer_image<3, uint8_t> I(Is, shp(dim));
+    field_t D = to_field(d, dim);
+    auto J = tipl::compose_displacement(I, D);
+    std::memcpy(Js, J.data(), J.size());
+}
+
+float tref_cdm_gradient(const uint8_t* Js, const uint8_t* It, const int* dim, char type,
+                        float* new_d)
+{
+    tipl::image<3, uint8_t> J(Js, shp(dim));
+    tipl::const_pointer_image<3, uint8_t> T(It, shp(dim));
+    field_t D;
+    float c = tipl::reg::cdm_get_gradient(J, T, D, (unsigned char)type);
+    from_field(D, new_d);
+    return c;
+}
+
+void tref_cdm_poisson(float* new_d, const int* dim)
+{
+    field_t D = to_field(new_d, dim);
+    bool terminated = false;
+    tipl::reg::cdm_solve_poisson(D, terminated);
+    from_field(D, new_d);
+}
+
+float tref_cdm_max_length(const float* d, const int* dim)
+{
+    field_t D = to_field(d, dim);
+    return tipl::reg::cdm_max_displacement_length(D);
+}
+
+void tref_accumulate_displacement(const float* cur, float* nd, const int* dim)
+{
+    field_t C = to_field(cur, dim), N = to_field(nd, dim);
+    tipl::accumulate_displacement(C, N);
+    from_field(N, nd);
+}
+
+void tref_cdm_smooth(const float* d, float* dd, const int* dim, float smoothing)
+{
+    field_t D = to_field(d, dim), DD = to_field(dd, dim);
+    tipl::reg::cdm_smooth(D, DD, smoothing);
+    from_field(DD, dd);
+}
+
+int tref_cdm_improved(float* cost, int* n)
+{
+    std::vector<float> c(cost, cost + *n);
+    bool r = tipl::reg::cdm_improved(c);
+    std::copy(c.begin(), c.end(), cost);
+    *n = int(c.size());
+    return r ? 1 : 0;
+}
+
+int tref_cdm(const uint8_t* const* It, const uint8_t* const* Is, int nmod, const int* dim,
+             float resolution, float speed, float smoothing, int min_dimension,
+             char gradient_type, float* field)
+{
+    std::vector<tipl::const_pointer_image<3, uint8_t>> vt, vs;
+    for (int i = 0; i < nmod; ++i) {
+        vt.emplace_back(It[i], shp(dim));
+        vs.emplace_back(Is[i], shp(dim));
+    }
+    tipl::reg::cdm_param p;
+    p.resolution = resolution;
+    p.speed = speed;
+    p.smoothing = smoothing;
+    p.min_dimension = (unsigned int)min_dimension;
+    p.gradient_type = std::string(size_t(nmod), gradient_type);
+    field_t D;
+    bool terminated = false;
+    bool ok = tipl::reg::cdm(vt, vs, D, terminated, p);
+    if (ok) from_field(D, field);
+    return ok ? 1 : 0;
+}
+
+// tipl::reg::linear on the CPU; arg (12 floats) is in/out.
+float tref_linear(const uint8_t* from, const int* fdim, const float* fvs, const uint8_t* to,
+                  const int* tdim, const float* tvs, int cost_type, int reg_type, float* arg)
+{
+    std::vector<tipl::const_pointer_image<3, uint8_t>> f{tipl::const_pointer_image<3, uint8_t>(from, shp(fdim))};
+    std::vector<tipl::const_pointer_image<3, uint8_t>> t{tipl::const_pointer_image<3, uint8_t>(to, shp(tdim))};
+    tipl::affine_param<float, 3> a(arg);
+    tipl::reg::linear_reg_param param;
+    param.cost_type = cost_type ? tipl::reg::mutual_info : tipl::reg::corr;
+    param.reg_type = tipl::reg::reg_type(reg_type);
+    param.cuda = false;
+    bool terminated = false;
+    float r = tipl::reg::linear<void>(f, tipl::vector<3>(fvs), t, tipl::vector<3>(tvs), a, param,
+                                      terminated);
+    std::copy_n(a.begin(), 12, arg);
+    return r;
+}
+
+}  // extern "C"
