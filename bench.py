@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X registration hot path.
+
+One step = one batched affine cost evaluation of 64 candidate transforms
+(one optimiser line-search grid) for the 182x218x182 mutual-information
+pair (64x64-bin joint histograms), inputs resident in HBM.  The JSON line
+also carries the 256-bin (reference-semantics) and correlation rates, the
+single-candidate rate, CDM iterations/s at 182x218x182, the roofline of
+the dominant kernel and the CPU reference timed on this host.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "affine cost GVox/s; CDM iter/s @182x218x182; served regs/s + p99 latency"
+HBM_PEAK_GBS = 8000.0
+CANDIDATES = 64
+
+
+def log(msg: str):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_candidates(rng, K, shape_from, shape_to):
+    import tipl_amd
+    from tipl_amd import phantoms
+    P = phantoms.candidate_params(rng, K)
+    Ts = np.stack([tipl_amd.affine_matrix(p, shape_from, (1, 1, 1), shape_to, (1, 1, 1)) for p in P])
+    return P, Ts
+
+
+def time_steps(ctx, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    ctx.synchronize()
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(F, M, Ts, seconds_target=12.0):
+    """The reference's CPU MI cost (oracle/_ref, 256 bins) on a bounded
+    sample of the same candidate set, candidates spread over host threads
+    as the reference optimisers do (each evaluation on one thread)."""
+    from oracle.bindings import load_oracle, load_reference
+    threads = max(1, min(16, os.cpu_count() or 1))
+    ref = load_reference()
+    if ref is not None:
+        ref.set_threads(threads)
+        kind = "reference"
+        t0 = time.perf_counter()
+        ref.cost_batch("mi", F, M, Ts[:threads])
+        one = time.perf_counter() - t0
+        rounds = max(1, int(seconds_target / max(one, 1e-3)))
+        K = min(len(Ts), threads * rounds)
+        t0 = time.perf_counter()
+        ref.cost_batch("mi", F, M, Ts[:K])
+        dt = time.perf_counter() - t0
+    else:
+        orc = load_oracle()
+        kind, threads = "port", 1
+        t0 = time.perf_counter()
+        K = 0
+        while time.perf_counter() - t0 < seconds_target and K < len(Ts):
+            orc.mi_cost(F, M, Ts[K])
+            K += 1
+        dt = time.perf_counter() - t0
+    evals = K / dt
+    return {"value": round(F.size * evals / 1e9, 4), "unit": "GVox/s", "cores": threads,
+            "kind": kind, "evals_per_s": round(evals, 3),
+            "sample": f"{K} MI (256-bin) evaluations of the 182x218x182 pair, "
+                      f"{threads} threads, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cdm-iters", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (each pair or "
+              f"candidate set fits one MI355X; sharding is described in DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import tipl_amd
+    from tipl_amd import phantoms
+    log("generating the 182x218x182 tissue pair")
+    fixed, moving, truth = phantoms.tissue_pair(1000)
+    ctx = tipl_amd.Context(0)
+    vf = ctx.volume(fixed, 255.99)   # quantised on the device (normalize_upper_lower2)
+    vm = ctx.volume(moving, 255.99)
+    shape = fixed.shape
+    N = int(np.prod(shape))
+    rng = np.random.default_rng(1000)
+    P, Ts = make_candidates(rng, CANDIDATES, shape, shape)
+
+    res = {}
+    # --- headline: K=64 MI candidates per launch, 64x64 bins ------------------
+    ctx.set_kernel_timing(True)
+    dt = time_steps(ctx, lambda: ctx.cost_batch(vf, vm, Ts, "mi", 64), args.steps, args.warmup)
+    kms, kn = ctx.kernel_time()
+    ctx.set_kernel_timing(False)
+    # kernel_time includes warmup launches; use the per-launch average
+    k_avg_ms = kms / kn
+    value = N * CANDIDATES * args.steps / dt / 1e9
+    ms_per_step = dt / args.steps * 1e3
+    log(f"mi64 K=64: {value:.1f} GVox/s, {ms_per_step:.3f} ms/step, kernel {k_avg_ms:.3f} ms")
+
+    # --- the other affine configurations (same candidate set) -------------
+    def rate(cost, bins, K):
+        T = Ts[:K]
+        n = max(5, args.steps // (2 if K > 1 else 1))
+        t = time_steps(ctx, lambda: ctx.cost_batch(vf, vm, T, cost, bins), n, 2)
+        return N * K * n / t / 1e9, t / n * 1e3
+
+    res["mi256_k64_gvox_s"], res["mi256_k64_ms"] = rate("mi", 256, 64)
+    res["corr_k64_gvox_s"], res["corr_k64_ms"] = rate("corr", 256, 64)
+    res["mi64_k1_gvox_s"], res["mi64_k1_ms"] = rate("mi", 64, 1)
+    res["mi256_k1_gvox_s"], res["mi256_k1_ms"] = rate("mi", 256, 1)
+    res["corr_k1_gvox_s"], res["corr_k1_ms"] = rate("corr", 256, 1)
+    for k, v in res.items():
+        log(f"{k} = {v:.3f}")
+
+    # --- CDM iterations/s at one 182x218x182 level -------------------------
+    log("CDM single level at 182x218x182")
+    vt = ctx.volume(fixed, 255.99)
+    vs = ctx.volume(phantoms.warp_field(fixed, phantoms.rbf_field(shape, 30, 6.0, 2000)), 255.99)
+    ctx.cdm_level_iterations(vt, vs, 2)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.cdm_level_iterations(vt, vs, args.cdm_iters)
+    ctx.synchronize()
+    cdm_dt = time.perf_counter() - t0
+    cdm_ips = args.cdm_iters / cdm_dt
+    log(f"CDM {cdm_ips:.1f} iter/s")
+
+    # --- roofline of the dominant kernel (MI 64-bin joint histogram) --------
+    alg_bytes = 2.0 * N * CANDIDATES            # 1 B from + 1 B to footprint per voxel-candidate
+    achieved = alg_bytes / (k_avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "mi_hist_kernel<64>", "kernel_ms": round(k_avg_ms, 4)}
+
+    cpu = None
+    if not args.no_cpu:
+        log("CPU reference baseline")
+        F8 = vf.read()
+        M8 = vm.read()
+        cpu = cpu_baseline(F8, M8, Ts)
+        log(f"cpu {cpu}")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "GVox/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (seeded 4-class tissue phantom, monotone remap + random affine)",
+        "config": {"workload": "affine MI cost, 64 candidates per launch, 64x64-bin joint "
+                               "histogram, 182x218x182 uint8", "shape": list(shape),
+                   "candidates_per_step": CANDIDATES, "bins": 64},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "cdm_iter_per_s": round(cdm_ips, 3),
+        "cdm_shape": list(shape),
+        **{k: round(v, 4) for k, v in res.items()},
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,219 @@
+/*
+ * tiplhip.h -- C ABI of the MI355X (gfx950) registration hot path.
+ *
+ * This is the drop-in boundary under TIPL's registration entry points.  The
+ * reference plugs its GPU path in through two C++-mangled explicit template
+ * instantiations compiled in a CUDA translation unit:
+ *
+ *   linear_reg_imp<out,mr,uint8,3>(std::true_type, shared_ptr<linear_reg_runner>,
+ *                                  cost_type, bool&)
+ *       declared reg/linear.hpp:606-609, instantiated reg/linear_cu.hpp:7-16;
+ *       its cost functors are correlation_cuda (reg/linear.hpp:40-78) and
+ *       mutual_information_cuda (reg/linear.hpp:207-252);
+ *   cdm_cuda<out,uint8,dim>(It, Is, d, terminated, cdm_param)
+ *       declared reg/cdm.hpp:525-531, instantiated reg/cdm_cu.hpp:25-36.
+ *
+ * This library replaces both with plain C entry points (no C++ types, no
+ * exceptions across the boundary).  The header-only C++ mirror in
+ * include/tipl/ calls them from tipl::reg::linear and tipl::reg::cdm_common.
+ *
+ * Conventions
+ *   - Volumes are x-fastest uint8 arrays, dim = {w, h, d}
+ *     (tipl::image<3,unsigned char>, utility/basic_image.hpp:136-317).
+ *   - A transform is 12 floats: sr[9] (row-major 3x3) then shift[3]
+ *     (tipl::transformation_matrix<float,3>, numerical/transformation.hpp:589-604);
+ *     it maps a voxel of the `from` grid to a position in the `to` volume.
+ *   - An affine parameter vector is 12 floats: translocation[3], rotation[3],
+ *     scaling[3], affine[3] (tipl::affine_param<float,3>,
+ *     numerical/transformation.hpp:421-437).
+ *   - Displacement fields are packed float3 per voxel
+ *     (tipl::image<3,tipl::vector<3>>).
+ *   - Every function returns TIPLHIP_OK (0) or an error code; the message of
+ *     the last error on the calling thread is tiplhip_last_error().
+ *   - A context owns one HIP stream on one device.  Calls on one context are
+ *     serialised by the context (the reference's cost functors are re-entered
+ *     from several threads, reg/linear.hpp:431-436); use one context per
+ *     thread for concurrency.
+ */
+#ifndef TIPLHIP_H
+#define TIPLHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TIPLHIP_OK 0
+#define TIPLHIP_ERR_INVALID 1     /* bad argument */
+#define TIPLHIP_ERR_DEVICE 2      /* HIP runtime error */
+#define TIPLHIP_ERR_MEMORY 3      /* device allocation failed */
+#define TIPLHIP_ERR_NO_DEVICE 4   /* no usable GPU */
+#define TIPLHIP_ERR_TERMINATED 5  /* the caller's terminate flag was raised */
+#define TIPLHIP_ERR_FAILED 6      /* the algorithm returned false (e.g. zero cost) */
+
+#define TIPLHIP_U8 0
+#define TIPLHIP_I16 1
+#define TIPLHIP_F32 2
+
+#define TIPLHIP_COST_CORR 0       /* tipl::reg::corr */
+#define TIPLHIP_COST_MI 1         /* tipl::reg::mutual_info */
+
+typedef struct tiplhip_ctx tiplhip_ctx;
+typedef struct tiplhip_volume tiplhip_volume;
+
+/* ---- errors, devices, contexts ------------------------------------------ */
+
+/* Message of the last failed call on this thread ("" if none).
+ * Replaces the std::runtime_error(cudaGetErrorName(..)) throws of cu.hpp:288-322. */
+const char* tiplhip_last_error(void);
+
+/* Number of visible HIP devices (tipl::has_gpu, def.hpp:7-13). */
+int tiplhip_device_count(int* count);
+
+/* A context: one device, one stream, reusable scratch memory.  Replaces the
+ * per-functor device state of correlation_cuda / mutual_information_cuda
+ * (reg/linear.hpp:43-44, 211-213) and the legacy default stream. */
+int tiplhip_ctx_create(int device, tiplhip_ctx** out);
+int tiplhip_ctx_destroy(tiplhip_ctx* ctx);
+int tiplhip_ctx_synchronize(tiplhip_ctx* ctx);
+
+/* Device time accounting: when enabled, the affine cost entry points record
+ * HIP events around their main kernel on the context stream and accumulate
+ * the elapsed time.  name is "affine_cost" (the dominant kernel). */
+int tiplhip_ctx_set_kernel_timing(tiplhip_ctx* ctx, int enable);
+int tiplhip_ctx_kernel_time(tiplhip_ctx* ctx, double* total_ms, int64_t* launches);
+
+/* Stream-level timer: events recorded on the context stream. */
+int tiplhip_timer_start(tiplhip_ctx* ctx);
+int tiplhip_timer_stop(tiplhip_ctx* ctx, float* ms);
+
+/* ---- volumes ------------------------------------------------------------- */
+
+/* Upload a volume and keep it resident on the device as uint8.
+ * dtype TIPLHIP_U8: copied as is (reg/linear.hpp:58-62, 222-229).
+ * dtype TIPLHIP_I16 / TIPLHIP_F32: quantised on the device with
+ * normalize_upper_lower2(in, out, upper_limit) (numerical/numerical.hpp:909-927,
+ * 1085-1108); the reference uses upper_limit 255.99 for correlation
+ * (reg/linear.hpp:69-70) and 255.99 for MI (mi_his_bandwidth - 0.01,
+ * reg/linear.hpp:195-196).  A constant volume is left all zero. */
+int tiplhip_volume_create(tiplhip_ctx* ctx, const void* host, int dtype, const int dim[3],
+                          float upper_limit, tiplhip_volume** out);
+/* Upload from a device pointer (no host copy). */
+int tiplhip_volume_create_device(tiplhip_ctx* ctx, const void* device_ptr, int dtype,
+                                 const int dim[3], float upper_limit, tiplhip_volume** out);
+int tiplhip_volume_destroy(tiplhip_volume* vol);
+int tiplhip_volume_dims(const tiplhip_volume* vol, int dim[3]);
+int tiplhip_volume_read(tiplhip_ctx* ctx, const tiplhip_volume* vol, uint8_t* host);
+/* Exact value statistics kept with the volume: sum, sum of squares and the
+ * 256-bin value histogram (histogram(), numerical/basic_op.hpp:1293-1311). */
+int tiplhip_volume_stats(const tiplhip_volume* vol, int64_t* sum, int64_t* sum_sq,
+                         uint32_t hist256[256]);
+
+/* downsample_with_padding (numerical/resampling.hpp:575-613): a new volume of
+ * shape (dim+1)/2. */
+int tiplhip_volume_downsample(tiplhip_ctx* ctx, const tiplhip_volume* in, tiplhip_volume** out);
+
+/* ---- affine resampling and costs ---------------------------------------- */
+
+/* tipl::resample(from, to, trans) (numerical/resampling.hpp:1239-1299):
+ * out (host, out_dim voxels) receives src(T * p) by trilinear interpolation
+ * truncated to uint8 (0 outside).  crop_shortcut = 1 reproduces the CPU
+ * reference's exact crop when T is an integer shift (:1244-1262);
+ * 0 reproduces the reference's device resample (no shortcut). */
+int tiplhip_resample(tiplhip_ctx* ctx, const tiplhip_volume* src, const int out_dim[3],
+                     const float T[12], int crop_shortcut, uint8_t* out);
+
+/* Exact correlation sums for K transforms: for candidate k,
+ * sums[5k..5k+4] = {Sx, Sxx, Sy, Syy, Sxy} with x = from and
+ * y = resample(to, from grid, T_k) (tipl::correlation on uint8,
+ * numerical/statistics.hpp:186-322, 491-625).  No role swap is applied. */
+int tiplhip_corr_sums(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_volume* to,
+                      const float* T, int K, int64_t* sums);
+
+/* Joint histogram of get_mutual_info (reg/linear.hpp:138-169): for
+ * candidate k, hist[k*bins*bins + (from << b) + to(T_k * p)] over the from
+ * grid.  bins = 256 is the reference's (mi_band_width 8, reg/linear.hpp:84-85);
+ * bins = 64 bins both values by >> 2. */
+int tiplhip_mi_hist(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_volume* to,
+                    const float* T, int K, int bins, int32_t* hist);
+
+/* K cost evaluations in one batched launch -- the optimiser's candidate sets
+ * (numerical/optimization.hpp:16-28, 59-96, 272-512).
+ * cost = TIPLHIP_COST_CORR: reg::correlation()(from, to, T) (reg/linear.hpp:23-37),
+ *        1 - float(r)^2, with the from/to role swap of :28-33 and the
+ *        CPU resample's integer-shift crop;
+ * cost = TIPLHIP_COST_MI: mutual_information<3>()(from, to, T) (reg/linear.hpp:173-203),
+ *        the negated FP64 entropy mean.
+ * T is host memory (K x 12 floats); costs is host memory (K doubles). */
+int tiplhip_cost_batch(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_volume* to,
+                       const float* T, int K, int cost, int bins, double* costs);
+
+/* The same with the K transforms computed on the host from affine
+ * parameters (transformation_matrix(arg, from, from_vs, to, to_vs),
+ * numerical/transformation.hpp:612-666). */
+int tiplhip_cost_batch_params(tiplhip_ctx* ctx, const tiplhip_volume* from, const float from_vs[3],
+                              const tiplhip_volume* to, const float to_vs[3], const float* params,
+                              int K, int cost, int bins, double* costs);
+
+/* ---- host geometry helpers (same float arithmetic as the reference) ---- */
+
+/* transformation_matrix(affine_param, from, from_vs, to, to_vs). */
+int tiplhip_affine_matrix(const float param[12], const int from_dim[3], const float from_vs[3],
+                          const int to_dim[3], const float to_vs[3], float T[12]);
+/* transformation_matrix::inverse (numerical/transformation.hpp:820-832);
+ * TIPLHIP_ERR_INVALID when singular. */
+int tiplhip_matrix_inverse(float T[12]);
+
+/* ---- CDM ------------------------------------------------------------------ */
+
+/* cdm_param (reg/cdm.hpp:36-44) plus the iteration cap that cdm() hard-codes
+ * as 200 (reg/cdm.hpp:418). */
+typedef struct {
+    float resolution;       /* 2.0 */
+    float speed;            /* 0.3 */
+    float smoothing;        /* 0.05 */
+    int min_dimension;      /* 16 */
+    int max_iterations;     /* 200 */
+    char gradient_type[8];  /* per modality: 'r' (default) or 'd' */
+} tiplhip_cdm_param;
+
+void tiplhip_cdm_param_default(tiplhip_cdm_param* p);
+
+/* tipl::reg::cdm / cdm_cuda (reg/cdm.hpp:373-532) for n modality pairs of
+ * equal shape.  field (host) receives vox(dim) float3.  terminated may be
+ * NULL; it is polled once per iteration (reg/cdm.hpp:418).  Returns
+ * TIPLHIP_ERR_FAILED where the reference returns false. */
+int tiplhip_cdm(tiplhip_ctx* ctx, const tiplhip_volume* const* It, const tiplhip_volume* const* Is,
+                int n, const tiplhip_cdm_param* param, float* field,
+                const volatile int* terminated, int* iterations);
+
+/* One resolution level of the CDM loop with no pyramid: runs exactly
+ * `iterations` loop iterations (no early exit) from a zero field, for
+ * timing.  field may be NULL. */
+int tiplhip_cdm_level_iterations(tiplhip_ctx* ctx, const tiplhip_volume* It,
+                                 const tiplhip_volume* Is, const tiplhip_cdm_param* param,
+                                 int iterations, float* field);
+
+/* Single CDM steps on host buffers (upload, one kernel, download), for
+ * parity checks of each kernel against the reference routine it restates. */
+int tiplhip_compose_displacement(tiplhip_ctx* ctx, const uint8_t* Is, const int dim[3],
+                                 const float* d, uint8_t* Js);            /* numerical/dif.hpp:90-107 */
+int tiplhip_cdm_gradient(tiplhip_ctx* ctx, const uint8_t* Js, const uint8_t* It,
+                         const int dim[3], char type, float* new_d,
+                         float* cost);                                   /* reg/cdm.hpp:46-106 */
+int tiplhip_cdm_poisson(tiplhip_ctx* ctx, float* new_d, const int dim[3], int sweeps,
+                        float* max_length);                              /* reg/cdm.hpp:158-296 */
+int tiplhip_accumulate_displacement(tiplhip_ctx* ctx, const float* cur, float* new_d,
+                                    const int dim[3], float scale);      /* numerical/dif.hpp:166-186 */
+int tiplhip_cdm_smooth(tiplhip_ctx* ctx, const float* d, float* dd, const int dim[3],
+                       float smoothing);                                 /* reg/cdm.hpp:302-342 */
+int tiplhip_upsample_field(tiplhip_ctx* ctx, const float* in, const int in_dim[3], float* out,
+                           const int out_dim[3]);                        /* numerical/resampling.hpp:849-858 */
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TIPLHIP_H */

@@ -1,0 +1,128 @@
+"""GPU parity of the CDM path: every step kernel bit-exact against the oracle
+on the same seeded inputs, and whole multi-level cdm() runs against the
+oracle's field (bar: <= 0.01 voxel RMS; in practice bit-exact)."""
+import numpy as np
+import pytest
+
+from conftest import smooth_volume
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(20, 24, 28), (13, 17, 11), (1, 9, 33)]
+
+
+def _field(shape, seed, scale=2.0, zero_frac=0.2):
+    from scipy.ndimage import gaussian_filter
+    r = np.random.default_rng(seed)
+    d = gaussian_filter(r.standard_normal(shape + (3,)), (2, 2, 2, 0)).astype(np.float32) * scale
+    d[r.random(shape) < zero_frac] = 0.0
+    return d
+
+
+def _bits_equal(a, b):
+    return np.array_equal(np.asarray(a, np.float32).view(np.uint32),
+                          np.asarray(b, np.float32).view(np.uint32))
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_compose_bit_exact(ctx, oracle, shape):
+    Is = smooth_volume(shape, 1, background=False)
+    d = _field(shape, 2, 4.0)
+    assert np.array_equal(ctx.compose(Is, d), oracle.compose(Is, d))
+
+
+@pytest.mark.parametrize("kind", ["r", "d"])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_gradient_bit_exact(ctx, oracle, shape, kind):
+    Js = smooth_volume(shape, 3)
+    It = smooth_volume(shape, 4)
+    nd, c = ctx.gradient(Js, It, kind)
+    nd_o, c_o = oracle.gradient(Js, It, kind)
+    assert _bits_equal(nd, nd_o)
+    assert abs(c - c_o) <= 1e-6 * max(abs(c_o), 1e-30)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_poisson_bit_exact(ctx, oracle, shape):
+    nd = _field(shape, 5, 1.0, 0.0)
+    out, ml = ctx.poisson(nd, 12)
+    want = oracle.poisson(nd, 12)
+    assert _bits_equal(out, want)
+    assert ml == oracle.max_length(want)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_accumulate_bit_exact(ctx, oracle, shape):
+    cur = _field(shape, 6, 2.0, 0.1)
+    nd = _field(shape, 7, 1.5, 0.3)
+    k = np.float32(0.3) / np.float32(1.7)
+    got = ctx.accumulate(cur, nd, float(k))
+    want = oracle.accumulate(cur, (nd * k).astype(np.float32))
+    assert _bits_equal(got, want)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("s", [0.8, 0.05, 0.0])
+def test_smooth_bit_exact(ctx, oracle, shape, s):
+    d = _field(shape, 8, 1.0, 0.0)
+    dd = _field(shape, 9, 1.0, 0.0)
+    assert _bits_equal(ctx.smooth(d, dd, s), oracle.smooth(d, dd, s))
+
+
+@pytest.mark.parametrize("ishape,oshape", [((10, 12, 14), (20, 24, 28)), ((7, 9, 6), (13, 17, 11))])
+def test_upsample_bit_exact(ctx, oracle, ishape, oshape):
+    d = _field(ishape, 10, 1.0, 0.0)
+    assert _bits_equal(ctx.upsample(d, oshape), oracle.upsample(d * np.float32(2.0), oshape))
+
+
+def _pair(shape, seed):
+    from scipy.ndimage import shift
+    It = smooth_volume(shape, seed, sigma=2.5)
+    Is = shift(It.astype(np.float32), (1.0, -1.5, 2.0), order=1).astype(np.uint8)
+    return It, Is
+
+
+@pytest.mark.parametrize("shape", [(40, 48, 40), (37, 45, 41), (24, 20, 18)])
+def test_cdm_matches_oracle(ctx, oracle, shape):
+    It, Is = _pair(shape, 11)
+    ok_o, f_o, it_o = oracle.cdm(It, Is)
+    rc, f, iters = ctx.cdm(ctx.volume(It), ctx.volume(Is))
+    assert ok_o and rc == 0
+    rms = float(np.sqrt(((f - f_o) ** 2).sum(-1).mean()))
+    assert rms <= 0.01, rms
+    assert iters == it_o
+
+
+def test_cdm_d_gradient_and_two_modalities(ctx, oracle):
+    shape = (36, 40, 32)
+    It, Is = _pair(shape, 12)
+    It2, Is2 = _pair(shape, 13)
+    for kind in ("d", "r"):
+        ok_o, f_o, _ = oracle.cdm([It, It2], [Is, Is2], gradient_type=kind)
+        p = ctx.cdm_param(gradient_type=kind)
+        rc, f, _ = ctx.cdm([ctx.volume(It), ctx.volume(It2)], [ctx.volume(Is), ctx.volume(Is2)], p)
+        assert ok_o and rc == 0
+        assert float(np.sqrt(((f - f_o) ** 2).sum(-1).mean())) <= 0.01
+
+
+def test_cdm_zero_cost_returns_failed(ctx, oracle):
+    import tipl_amd
+    z = np.zeros((20, 20, 20), np.uint8)
+    ok_o, _, _ = oracle.cdm(z, z)
+    rc, _, _ = ctx.cdm(ctx.volume(z), ctx.volume(z))
+    assert not ok_o and rc == tipl_amd.ERR_FAILED
+
+
+def test_cdm_terminated_flag(ctx):
+    import ctypes
+    import tipl_amd
+    It, Is = _pair((32, 32, 32), 14)
+    flag = ctypes.c_int(1)
+    rc, field, iters = ctx.cdm(ctx.volume(It), ctx.volume(Is), terminated=flag)
+    assert rc == tipl_amd.ERR_TERMINATED and iters == 0 and not field.any()
+
+
+def test_cdm_level_iterations_runs(ctx):
+    It, Is = _pair((30, 34, 26), 15)
+    f = ctx.cdm_level_iterations(ctx.volume(It), ctx.volume(Is), 5, want_field=True)
+    assert np.isfinite(f).all()

@@ -1,0 +1,362 @@
+"""tipl_amd -- MI355X-native registration hot path of TIPL.
+
+The product is libtiplhip.so (HIP kernels for gfx950 behind the C ABI in
+include/tiplhip.h) and the header-only C++ mirror of TIPL's entry points in
+include/tipl/.  This module is a thin ctypes binding of the C ABI for Python
+callers, tests and bench.py.  There is no CPU fallback: without the built
+library, or without a GPU, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtiplhip.so")
+
+OK, ERR_INVALID, ERR_DEVICE, ERR_MEMORY, ERR_NO_DEVICE, ERR_TERMINATED, ERR_FAILED = range(7)
+U8, I16, F32 = 0, 1, 2
+COST_CORR, COST_MI = 0, 1
+
+# symbols declared in include/tiplhip.h
+EXPORTS = (
+    "tiplhip_last_error", "tiplhip_device_count", "tiplhip_ctx_create", "tiplhip_ctx_destroy",
+    "tiplhip_ctx_synchronize", "tiplhip_ctx_set_kernel_timing", "tiplhip_ctx_kernel_time",
+    "tiplhip_timer_start", "tiplhip_timer_stop", "tiplhip_volume_create",
+    "tiplhip_volume_create_device", "tiplhip_volume_destroy", "tiplhip_volume_dims",
+    "tiplhip_volume_read", "tiplhip_volume_stats", "tiplhip_volume_downsample",
+    "tiplhip_resample", "tiplhip_corr_sums", "tiplhip_mi_hist", "tiplhip_cost_batch",
+    "tiplhip_cost_batch_params", "tiplhip_affine_matrix", "tiplhip_matrix_inverse",
+    "tiplhip_cdm_param_default", "tiplhip_cdm", "tiplhip_cdm_level_iterations",
+    "tiplhip_compose_displacement", "tiplhip_cdm_gradient", "tiplhip_cdm_poisson",
+    "tiplhip_accumulate_displacement", "tiplhip_cdm_smooth", "tiplhip_upsample_field",
+)
+
+
+class TiplHipError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"tiplhip error {code}: {msg}")
+        self.code = code
+
+
+class CdmParam(C.Structure):
+    """tiplhip_cdm_param (cdm_param, reg/cdm.hpp:36-44, plus the iteration cap)."""
+    _fields_ = [("resolution", C.c_float), ("speed", C.c_float), ("smoothing", C.c_float),
+                ("min_dimension", C.c_int), ("max_iterations", C.c_int),
+                ("gradient_type", C.c_char * 8)]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Loads libtiplhip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"{path} is missing: build it with `python -m tipl_amd.build` (there is no CPU fallback)")
+    lib = C.CDLL(path)
+    vp, ip = C.c_void_p, C.c_int
+    P = C.POINTER
+    sig = {
+        "tiplhip_last_error": (C.c_char_p, []),
+        "tiplhip_device_count": (ip, [P(ip)]),
+        "tiplhip_ctx_create": (ip, [ip, P(vp)]),
+        "tiplhip_ctx_destroy": (ip, [vp]),
+        "tiplhip_ctx_synchronize": (ip, [vp]),
+        "tiplhip_ctx_set_kernel_timing": (ip, [vp, ip]),
+        "tiplhip_ctx_kernel_time": (ip, [vp, P(C.c_double), P(C.c_int64)]),
+        "tiplhip_timer_start": (ip, [vp]),
+        "tiplhip_timer_stop": (ip, [vp, P(C.c_float)]),
+        "tiplhip_volume_create": (ip, [vp, vp, ip, P(ip), C.c_float, P(vp)]),
+        "tiplhip_volume_create_device": (ip, [vp, vp, ip, P(ip), C.c_float, P(vp)]),
+        "tiplhip_volume_destroy": (ip, [vp]),
+        "tiplhip_volume_dims": (ip, [vp, P(ip)]),
+        "tiplhip_volume_read": (ip, [vp, vp, vp]),
+        "tiplhip_volume_stats": (ip, [vp, P(C.c_int64), P(C.c_int64), vp]),
+        "tiplhip_volume_downsample": (ip, [vp, vp, P(vp)]),
+        "tiplhip_resample": (ip, [vp, vp, P(ip), vp, ip, vp]),
+        "tiplhip_corr_sums": (ip, [vp, vp, vp, vp, ip, vp]),
+        "tiplhip_mi_hist": (ip, [vp, vp, vp, vp, ip, ip, vp]),
+        "tiplhip_cost_batch": (ip, [vp, vp, vp, vp, ip, ip, ip, vp]),
+        "tiplhip_cost_batch_params": (ip, [vp, vp, vp, vp, vp, vp, ip, ip, ip, vp]),
+        "tiplhip_affine_matrix": (ip, [vp, P(ip), vp, P(ip), vp, vp]),
+        "tiplhip_matrix_inverse": (ip, [vp]),
+        "tiplhip_cdm_param_default": (None, [P(CdmParam)]),
+        "tiplhip_cdm": (ip, [vp, vp, vp, ip, P(CdmParam), vp, vp, P(ip)]),
+        "tiplhip_cdm_level_iterations": (ip, [vp, vp, vp, P(CdmParam), ip, vp]),
+        "tiplhip_compose_displacement": (ip, [vp, vp, P(ip), vp, vp]),
+        "tiplhip_cdm_gradient": (ip, [vp, vp, vp, P(ip), C.c_char, vp, P(C.c_float)]),
+        "tiplhip_cdm_poisson": (ip, [vp, vp, P(ip), ip, P(C.c_float)]),
+        "tiplhip_accumulate_displacement": (ip, [vp, vp, vp, P(ip), C.c_float]),
+        "tiplhip_cdm_smooth": (ip, [vp, vp, vp, P(ip), C.c_float]),
+        "tiplhip_upsample_field": (ip, [vp, vp, P(ip), vp, P(ip)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc: int):
+    if rc != OK:
+        msg = load_library().tiplhip_last_error().decode(errors="replace")
+        raise TiplHipError(rc, msg)
+
+
+def _dims(shape) -> "C.Array":
+    """x-fastest {w,h,d} from a C-ordered (z, y, x) numpy shape."""
+    z, y, x = shape
+    return (C.c_int * 3)(x, y, z)
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    _check(load_library().tiplhip_device_count(C.byref(n)))
+    return n.value
+
+
+def affine_matrix(param, from_shape, from_vs, to_shape, to_vs) -> np.ndarray:
+    """transformation_matrix(affine_param, from, from_vs, to, to_vs) as 12 floats."""
+    lib = load_library()
+    p = np.ascontiguousarray(param, np.float32)
+    fv = np.ascontiguousarray(from_vs, np.float32)
+    tv = np.ascontiguousarray(to_vs, np.float32)
+    T = np.zeros(12, np.float32)
+    _check(lib.tiplhip_affine_matrix(_ptr(p), _dims(from_shape), _ptr(fv), _dims(to_shape),
+                                     _ptr(tv), _ptr(T)))
+    return T
+
+
+def matrix_inverse(T) -> np.ndarray:
+    T = np.array(T, np.float32)
+    _check(load_library().tiplhip_matrix_inverse(_ptr(T)))
+    return T
+
+
+class Volume:
+    """A device-resident uint8 volume (quantised on upload when not uint8)."""
+
+    def __init__(self, ctx: "Context", handle: int, shape):
+        self.ctx = ctx
+        self.handle = handle
+        self.shape = tuple(shape)
+
+    def read(self) -> np.ndarray:
+        out = np.empty(self.shape, np.uint8)
+        _check(load_library().tiplhip_volume_read(self.ctx.handle, self.handle, _ptr(out)))
+        return out
+
+    def stats(self):
+        s, s2 = C.c_int64(0), C.c_int64(0)
+        h = np.zeros(256, np.uint32)
+        _check(load_library().tiplhip_volume_stats(self.handle, C.byref(s), C.byref(s2), _ptr(h)))
+        return s.value, s2.value, h
+
+    def downsample(self) -> "Volume":
+        out = C.c_void_p()
+        _check(load_library().tiplhip_volume_downsample(self.ctx.handle, self.handle, C.byref(out)))
+        z, y, x = self.shape
+        return Volume(self.ctx, out.value, ((z + 1) // 2, (y + 1) // 2, (x + 1) // 2))
+
+    def free(self):
+        if self.handle:
+            _check(load_library().tiplhip_volume_destroy(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            if self.handle and self.ctx.handle:
+                load_library().tiplhip_volume_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class Context:
+    """A device context: one GPU, one HIP stream, reusable scratch memory."""
+
+    def __init__(self, device: int = 0):
+        lib = load_library()
+        h = C.c_void_p()
+        _check(lib.tiplhip_ctx_create(device, C.byref(h)))
+        self.handle = h.value
+        self.device = device
+
+    def close(self):
+        if self.handle:
+            _check(load_library().tiplhip_ctx_destroy(self.handle))
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def synchronize(self):
+        _check(load_library().tiplhip_ctx_synchronize(self.handle))
+
+    # -- timing ------------------------------------------------------------
+    def set_kernel_timing(self, enable: bool):
+        _check(load_library().tiplhip_ctx_set_kernel_timing(self.handle, int(enable)))
+
+    def kernel_time(self):
+        ms, n = C.c_double(0), C.c_int64(0)
+        _check(load_library().tiplhip_ctx_kernel_time(self.handle, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def timer_start(self):
+        _check(load_library().tiplhip_timer_start(self.handle))
+
+    def timer_stop(self) -> float:
+        ms = C.c_float(0)
+        _check(load_library().tiplhip_timer_stop(self.handle, C.byref(ms)))
+        return ms.value
+
+    # -- volumes -----------------------------------------------------------
+    def volume(self, data: np.ndarray, upper_limit: float = 255.99) -> Volume:
+        if data.dtype == np.uint8:
+            dt = U8
+        elif data.dtype == np.int16:
+            dt = I16
+        else:
+            data = data.astype(np.float32, copy=False)
+            dt = F32
+        data = np.ascontiguousarray(data)
+        h = C.c_void_p()
+        _check(load_library().tiplhip_volume_create(self.handle, _ptr(data), dt, _dims(data.shape),
+                                                    C.c_float(upper_limit), C.byref(h)))
+        return Volume(self, h.value, data.shape)
+
+    # -- affine --------------------------------------------------------------
+    def resample(self, src: Volume, out_shape, T, crop: bool = True) -> np.ndarray:
+        out = np.empty(tuple(out_shape), np.uint8)
+        Tm = np.ascontiguousarray(T, np.float32)
+        _check(load_library().tiplhip_resample(self.handle, src.handle, _dims(out_shape), _ptr(Tm),
+                                               int(crop), _ptr(out)))
+        return out
+
+    def corr_sums(self, frm: Volume, to: Volume, Ts) -> np.ndarray:
+        Ts = np.ascontiguousarray(Ts, np.float32).reshape(-1, 12)
+        out = np.zeros((Ts.shape[0], 5), np.int64)
+        _check(load_library().tiplhip_corr_sums(self.handle, frm.handle, to.handle, _ptr(Ts),
+                                                Ts.shape[0], _ptr(out)))
+        return out
+
+    def mi_hist(self, frm: Volume, to: Volume, Ts, bins: int = 256) -> np.ndarray:
+        Ts = np.ascontiguousarray(Ts, np.float32).reshape(-1, 12)
+        out = np.zeros((Ts.shape[0], bins, bins), np.int32)
+        _check(load_library().tiplhip_mi_hist(self.handle, frm.handle, to.handle, _ptr(Ts),
+                                              Ts.shape[0], bins, _ptr(out)))
+        return out
+
+    def cost_batch(self, frm: Volume, to: Volume, Ts, cost: str = "mi", bins: int = 256) -> np.ndarray:
+        Ts = np.ascontiguousarray(Ts, np.float32).reshape(-1, 12)
+        out = np.zeros(Ts.shape[0], np.float64)
+        _check(load_library().tiplhip_cost_batch(self.handle, frm.handle, to.handle, _ptr(Ts),
+                                                 Ts.shape[0], COST_MI if cost == "mi" else COST_CORR,
+                                                 bins, _ptr(out)))
+        return out
+
+    def cost_batch_params(self, frm: Volume, from_vs, to: Volume, to_vs, params, cost="mi",
+                          bins=256) -> np.ndarray:
+        P = np.ascontiguousarray(params, np.float32).reshape(-1, 12)
+        fv = np.ascontiguousarray(from_vs, np.float32)
+        tv = np.ascontiguousarray(to_vs, np.float32)
+        out = np.zeros(P.shape[0], np.float64)
+        _check(load_library().tiplhip_cost_batch_params(
+            self.handle, frm.handle, _ptr(fv), to.handle, _ptr(tv), _ptr(P), P.shape[0],
+            COST_MI if cost == "mi" else COST_CORR, bins, _ptr(out)))
+        return out
+
+    # -- CDM -----------------------------------------------------------------
+    @staticmethod
+    def cdm_param(**kw) -> CdmParam:
+        p = CdmParam()
+        load_library().tiplhip_cdm_param_default(C.byref(p))
+        gt = kw.pop("gradient_type", None)
+        for k, v in kw.items():
+            setattr(p, k, v)
+        if gt is not None:
+            p.gradient_type = (gt * 8)[:8].encode() if len(gt) == 1 else gt.encode()[:8].ljust(8, b"r")
+        return p
+
+    def cdm(self, It: Sequence[Volume], Is: Sequence[Volume], param: Optional[CdmParam] = None,
+            terminated: Optional[C.c_int] = None):
+        if isinstance(It, Volume):
+            It, Is = [It], [Is]
+        n = len(It)
+        at = (C.c_void_p * n)(*[v.handle for v in It])
+        as_ = (C.c_void_p * n)(*[v.handle for v in Is])
+        field = np.zeros(It[0].shape + (3,), np.float32)
+        iters = C.c_int(0)
+        p = param if param is not None else self.cdm_param()
+        rc = load_library().tiplhip_cdm(self.handle, at, as_, n, C.byref(p), _ptr(field),
+                                        C.byref(terminated) if terminated is not None else None,
+                                        C.byref(iters))
+        return rc, field, iters.value
+
+    def cdm_level_iterations(self, It: Volume, Is: Volume, iterations: int,
+                             param: Optional[CdmParam] = None, want_field: bool = False):
+        p = param if param is not None else self.cdm_param()
+        field = np.zeros(It.shape + (3,), np.float32) if want_field else None
+        _check(load_library().tiplhip_cdm_level_iterations(
+            self.handle, It.handle, Is.handle, C.byref(p), iterations,
+            _ptr(field) if want_field else None))
+        return field
+
+    def compose(self, Is: np.ndarray, d: np.ndarray) -> np.ndarray:
+        Is = np.ascontiguousarray(Is, np.uint8)
+        d = np.ascontiguousarray(d, np.float32)
+        out = np.empty_like(Is)
+        _check(load_library().tiplhip_compose_displacement(self.handle, _ptr(Is), _dims(Is.shape),
+                                                           _ptr(d), _ptr(out)))
+        return out
+
+    def gradient(self, Js: np.ndarray, It: np.ndarray, kind: str = "r"):
+        Js = np.ascontiguousarray(Js, np.uint8)
+        It = np.ascontiguousarray(It, np.uint8)
+        nd = np.zeros(Js.shape + (3,), np.float32)
+        c = C.c_float(0)
+        _check(load_library().tiplhip_cdm_gradient(self.handle, _ptr(Js), _ptr(It), _dims(Js.shape),
+                                                   kind.encode(), _ptr(nd), C.byref(c)))
+        return nd, c.value
+
+    def poisson(self, nd: np.ndarray, sweeps: int = 12):
+        nd = np.array(nd, np.float32, order="C")
+        ml = C.c_float(0)
+        _check(load_library().tiplhip_cdm_poisson(self.handle, _ptr(nd), _dims(nd.shape[:3]), sweeps,
+                                                  C.byref(ml)))
+        return nd, ml.value
+
+    def accumulate(self, cur: np.ndarray, nd: np.ndarray, scale: float = 1.0) -> np.ndarray:
+        cur = np.ascontiguousarray(cur, np.float32)
+        nd = np.array(nd, np.float32, order="C")
+        _check(load_library().tiplhip_accumulate_displacement(self.handle, _ptr(cur), _ptr(nd),
+                                                              _dims(cur.shape[:3]), C.c_float(scale)))
+        return nd
+
+    def smooth(self, d: np.ndarray, dd: np.ndarray, smoothing: float) -> np.ndarray:
+        d = np.ascontiguousarray(d, np.float32)
+        dd = np.array(dd, np.float32, order="C")
+        _check(load_library().tiplhip_cdm_smooth(self.handle, _ptr(d), _ptr(dd), _dims(d.shape[:3]),
+                                                 C.c_float(smoothing)))
+        return dd
+
+    def upsample(self, field: np.ndarray, out_shape) -> np.ndarray:
+        field = np.ascontiguousarray(field, np.float32)
+        out = np.zeros(tuple(out_shape) + (3,), np.float32)
+        _check(load_library().tiplhip_upsample_field(self.handle, _ptr(field), _dims(field.shape[:3]),
+                                                     _ptr(out), _dims(out_shape)))
+        return out

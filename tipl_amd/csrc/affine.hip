@@ -1,0 +1,631 @@
+// affine.hip -- affine resampling, correlation sums and the MI joint
+// histogram for K candidate transforms per launch (gfx950).
+//
+// Restates, MI355X-first:
+//   tipl::resample            numerical/resampling.hpp:1239-1299
+//   reg::correlation(_cuda)   reg/linear.hpp:23-78  + tipl::correlation
+//                             numerical/statistics.hpp:186-625
+//   reg::get_mutual_info      reg/linear.hpp:138-169
+//   reg::get_mutual_info_mean reg/linear.hpp:87-134
+//
+// Work decomposition.  The `from` grid is cut into row segments of 64
+// voxels (one wave, one row, x contiguous), so y and z are wave-uniform and
+// every load of `from` is one coalesced 64-byte access.  A launch covers K
+// candidates x C chunks of segments; workgroups are remapped so that each
+// XCD works through a contiguous slab of the volume for all candidates.
+// Integer results (sums, histograms) are accumulated with integer atomics
+// only, so they are exact and independent of scheduling.
+#include "device.hpp"
+#include "internal.hpp"
+
+#include <algorithm>
+#include <vector>
+
+namespace tiplhip {
+namespace {
+
+struct BatchGeo {
+    int fw, fh, fd;             // from grid
+    uint32_t nseg;              // segments per row: ceil(fw / 64)
+    uint32_t items;             // fh * fd * nseg
+    uint32_t items_per_block;
+    uint32_t K;
+    uint32_t total_blocks;      // K * ceil(items / items_per_block)
+    uint32_t from_bytes, to_bytes;
+    VolGeo to;
+};
+
+VolGeo make_geo(const int* dim)
+{
+    VolGeo g;
+    g.w = dim[0];
+    g.h = dim[1];
+    g.d = dim[2];
+    g.wh = uint32_t(dim[0]) * uint32_t(dim[1]);
+    g.wm1 = float(dim[0] - 1);
+    g.hm1 = float(dim[1] - 1);
+    g.dm1 = float(dim[2] - 1);
+    return g;
+}
+
+BatchGeo make_batch(const tiplhip_volume* from, const tiplhip_volume* to, int K,
+                    uint32_t max_items_per_block, uint32_t target_blocks)
+{
+    BatchGeo b;
+    b.fw = from->dim[0];
+    b.fh = from->dim[1];
+    b.fd = from->dim[2];
+    b.nseg = uint32_t((b.fw + kWave - 1) / kWave);
+    b.items = b.nseg * uint32_t(b.fh) * uint32_t(b.fd);
+    uint64_t want = (uint64_t(b.items) * uint64_t(K) + target_blocks - 1) / target_blocks;
+    uint32_t ipb = uint32_t(std::min<uint64_t>(want, max_items_per_block));
+    ipb = std::max<uint32_t>(ipb, 16u);
+    ipb = std::min<uint32_t>(ipb, max_items_per_block);
+    b.items_per_block = ipb;
+    b.K = uint32_t(K);
+    uint32_t chunks = (b.items + ipb - 1) / ipb;
+    b.total_blocks = chunks * uint32_t(K);
+    b.from_bytes = uint32_t(from->n);
+    b.to_bytes = uint32_t(to->n);
+    b.to = make_geo(to->dim);
+    return b;
+}
+
+// Position of voxel (x, y, z) of the from grid in the to volume:
+// vector_transformation(.., rotation, shift, vdim<3>)
+// (numerical/transformation.hpp:58-73): ((x*r0 + y*r1) + z*r2) + s0.
+struct Pos {
+    float x, y, z;
+};
+__device__ inline Pos apply(const float* T, float fx, float fy, float fz)
+{
+    Pos p;
+    p.x = fx * T[0] + fy * T[1] + fz * T[2] + T[9];
+    p.y = fx * T[3] + fy * T[4] + fz * T[5] + T[10];
+    p.z = fx * T[6] + fy * T[7] + fz * T[8] + T[11];
+    return p;
+}
+
+// ---------------------------------------------------------------------------
+// Correlation: per candidate the exact sums Sy, Syy, Sxy (Sx, Sxx of `from`
+// do not depend on the transform and are kept with the volume).
+// crop[k] = {flag, sx, sy, sz}: flag 1 reproduces the CPU resample's exact
+// crop for an integer-shift transform (numerical/resampling.hpp:1244-1262).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void corr_sums_kernel(const uint8_t* __restrict__ from,
+                                                        const uint8_t* __restrict__ to,
+                                                        const float* __restrict__ Ts,
+                                                        const int4* __restrict__ crop, BatchGeo g,
+                                                        unsigned long long* __restrict__ sums)
+{
+    const uint32_t L = xcd_remap(blockIdx.x, g.total_blocks);
+    const uint32_t cand = L % g.K;
+    const uint32_t chunk = L / g.K;
+    const float* T = Ts + 12 * cand;
+    const int4 cr = crop[cand];
+    const __amdgpu_buffer_rsrc_t rf = make_rsrc(from, g.from_bytes);
+    const __amdgpu_buffer_rsrc_t rt = make_rsrc(to, g.to_bytes);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t nwaves = blockDim.x >> 6;
+    const uint32_t it0 = chunk * g.items_per_block;
+    const uint32_t it1 = min(it0 + g.items_per_block, g.items);
+    uint32_t sy = 0, syy = 0, sxy = 0;
+    for (uint32_t it = it0 + wave; it < it1; it += nwaves) {
+        const uint32_t row = it / g.nseg;
+        const uint32_t seg = it - row * g.nseg;
+        const uint32_t y = row % uint32_t(g.fh);
+        const uint32_t z = row / uint32_t(g.fh);
+        const uint32_t x = seg * 64u + lane;
+        if (x < uint32_t(g.fw)) {
+            const uint32_t xv = load_u8(rf, row * uint32_t(g.fw) + x, 0);
+            uint32_t yv;
+            if (cr.x) {
+                int sx = int(x) + cr.y, s_y = int(y) + cr.z, sz = int(z) + cr.w;
+                bool in = sx >= 0 && sx < g.to.w && s_y >= 0 && s_y < g.to.h && sz >= 0 &&
+                          sz < g.to.d;
+                yv = in ? load_u8(rt, uint32_t(sz) * g.to.wh + uint32_t(s_y) * uint32_t(g.to.w) +
+                                          uint32_t(sx), 0)
+                        : 0u;
+            } else {
+                Pos p = apply(T, float(x), float(y), float(z));
+                yv = sample_u8(rt, g.to, p.x, p.y, p.z);
+            }
+            sy += yv;
+            syy += yv * yv;
+            sxy += xv * yv;
+        }
+    }
+    __shared__ unsigned long long red[3][4];
+    unsigned long long a = wave_sum_u64(sy), b = wave_sum_u64(syy), c = wave_sum_u64(sxy);
+    if (lane == 0) {
+        red[0][wave] = a;
+        red[1][wave] = b;
+        red[2][wave] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        unsigned long long s = 0;
+        for (uint32_t w = 0; w < nwaves; ++w) s += red[threadIdx.x][w];
+        atomicAdd(&sums[cand * 4 + threadIdx.x], s);
+    }
+}
+
+// tipl::correlation from exact sums (numerical/statistics.hpp:555-586,
+// 407-443) in FP64, then `float c; 1.0f - c*c` (reg/linear.hpp:34-35).
+__global__ void corr_finalize_kernel(const unsigned long long* __restrict__ sums, long long sx,
+                                     long long sxx, double n, int K, double* __restrict__ costs,
+                                     long long* __restrict__ sums_out)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    long long sy = (long long)sums[k * 4 + 0];
+    long long syy = (long long)sums[k * 4 + 1];
+    long long sxy = (long long)sums[k * 4 + 2];
+    double mean_x = double(sx) / n;
+    double mean_y = double(sy) / n;
+    double var_x = double(sxx) / n - mean_x * mean_x;
+    double var_y = double(syy) / n - mean_y * mean_y;
+    double sd1 = var_x > 0.0 ? sqrt(var_x) : 0.0;
+    double sd2 = var_y > 0.0 ? sqrt(var_y) : 0.0;
+    double r = 0.0;
+    if (!(sd1 == 0.0 || sd2 == 0.0)) {
+        double cov = double(sxy) / n - mean_x * mean_y;
+        r = cov / sd1 / sd2;
+    }
+    float c = float(r);
+    costs[k] = double(1.0f - c * c);
+    if (sums_out) {
+        sums_out[k * 5 + 0] = sx;
+        sums_out[k * 5 + 1] = sxx;
+        sums_out[k * 5 + 2] = sy;
+        sums_out[k * 5 + 3] = syy;
+        sums_out[k * 5 + 4] = sxy;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Mutual information: joint histogram in LDS.
+//   BINS = 256: 65536 bins as packed uint16 pairs in 32768 words (128 KB),
+//               so a workgroup may add at most 65535 voxels (items_per_block
+//               <= 1023 segments of 64);
+//   BINS = 64 : 4096 uint32 words (16 KB).
+// Adds are wave-aggregated: the lanes whose bin equals the first active
+// lane's bin (the background bin, mostly) are counted with one ballot and
+// one LDS atomic; the other lanes add individually.  At the end the
+// non-zero words are flushed into the candidate's global int32 histogram.
+// ---------------------------------------------------------------------------
+template <int BINS>
+struct MiTraits;
+template <>
+struct MiTraits<256> {
+    static constexpr int kWords = 32768;
+    static constexpr int kBand = 8;
+    static constexpr int kShift = 0;
+    static constexpr int kThreads = 1024;
+    __device__ static uint32_t word(uint32_t bin) { return bin >> 1; }
+    __device__ static uint32_t inc(uint32_t bin) { return (bin & 1u) ? 0x10000u : 1u; }
+};
+template <>
+struct MiTraits<64> {
+    static constexpr int kWords = 4096;
+    static constexpr int kBand = 6;
+    static constexpr int kShift = 2;
+    static constexpr int kThreads = 256;
+    __device__ static uint32_t word(uint32_t bin) { return bin; }
+    __device__ static uint32_t inc(uint32_t) { return 1u; }
+};
+
+template <int BINS>
+__global__ __launch_bounds__(MiTraits<BINS>::kThreads) void mi_hist_kernel(
+    const uint8_t* __restrict__ from, const uint8_t* __restrict__ to, const float* __restrict__ Ts,
+    BatchGeo g, int* __restrict__ hist)
+{
+    using Tr = MiTraits<BINS>;
+    __shared__ uint32_t lds[Tr::kWords];
+    for (uint32_t i = threadIdx.x; i < uint32_t(Tr::kWords); i += blockDim.x) lds[i] = 0u;
+    __syncthreads();
+
+    const uint32_t L = xcd_remap(blockIdx.x, g.total_blocks);
+    const uint32_t cand = L % g.K;
+    const uint32_t chunk = L / g.K;
+    const float* T = Ts + 12 * cand;
+    const __amdgpu_buffer_rsrc_t rf = make_rsrc(from, g.from_bytes);
+    const __amdgpu_buffer_rsrc_t rt = make_rsrc(to, g.to_bytes);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t nwaves = blockDim.x >> 6;
+    const uint32_t it0 = chunk * g.items_per_block;
+    const uint32_t it1 = min(it0 + g.items_per_block, g.items);
+    for (uint32_t it = it0 + wave; it < it1; it += nwaves) {
+        const uint32_t row = it / g.nseg;
+        const uint32_t seg = it - row * g.nseg;
+        const uint32_t y = row % uint32_t(g.fh);
+        const uint32_t z = row / uint32_t(g.fh);
+        const uint32_t x = seg * 64u + lane;
+        if (x < uint32_t(g.fw)) {
+            const uint32_t fv = load_u8(rf, row * uint32_t(g.fw) + x, 0);
+            Pos p = apply(T, float(x), float(y), float(z));
+            const uint32_t tv = sample_u8(rt, g.to, p.x, p.y, p.z);
+            const uint32_t bin = ((fv >> Tr::kShift) << Tr::kBand) | (tv >> Tr::kShift);
+            const uint32_t b0 = __builtin_amdgcn_readfirstlane(bin);
+            const bool same = bin == b0;
+            const unsigned long long m = __ballot(same);
+            if (same) {
+                const uint32_t leader = uint32_t(__ffsll((long long)m) - 1);
+                if (lane == leader)
+                    atomicAdd(&lds[Tr::word(b0)], Tr::inc(b0) * uint32_t(__popcll(m)));
+            } else {
+                atomicAdd(&lds[Tr::word(bin)], Tr::inc(bin));
+            }
+        }
+    }
+    __syncthreads();
+    int* h = hist + size_t(cand) * BINS * BINS;
+    for (uint32_t i = threadIdx.x; i < uint32_t(Tr::kWords); i += blockDim.x) {
+        const uint32_t v = lds[i];
+        if (!v) continue;
+        if constexpr (BINS == 256) {
+            if (v & 0xffffu) atomicAdd(&h[2 * i], int(v & 0xffffu));
+            if (v >> 16) atomicAdd(&h[2 * i + 1], int(v >> 16));
+        } else {
+            atomicAdd(&h[i], int(v));
+        }
+    }
+}
+
+// get_mutual_info_mean (reg/linear.hpp:87-104): to_hist = column sums,
+// sum of mu*log(mu/from_h[row]/to_h[col]) in FP64 over non-zero bins,
+// divided by float(BINS*BINS) and negated (reg/linear.hpp:201).  One
+// workgroup per candidate; the FP64 sum is a fixed-shape tree, so it is
+// deterministic (the reference's sum is sequential in row-major order; the
+// two agree to rounding).
+template <int BINS>
+__global__ __launch_bounds__(256) void mi_entropy_kernel(const int* __restrict__ hist,
+                                                         const uint32_t* __restrict__ from_hist,
+                                                         double* __restrict__ costs)
+{
+    const int* h = hist + size_t(blockIdx.x) * BINS * BINS;
+    __shared__ uint32_t to_h[BINS];
+    __shared__ double part[4];
+    for (int c = threadIdx.x; c < BINS; c += blockDim.x) {
+        uint32_t s = 0;
+        for (int r = 0; r < BINS; ++r) s += uint32_t(h[r * BINS + c]);
+        to_h[c] = s;
+    }
+    __syncthreads();
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < BINS * BINS; i += blockDim.x) {
+        const int mu = h[i];
+        if (mu == 0) continue;
+        const double m = double(mu);
+        acc += m * log(m / double(from_hist[i / BINS]) / double(to_h[i % BINS]));
+    }
+    acc = wave_sum_f64(acc);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = (part[0] + part[1]) + (part[2] + part[3]);
+        costs[blockIdx.x] = -(s / double(float(BINS * BINS)));
+    }
+}
+
+struct TransformArg {
+    float t[12];
+};
+
+// tipl::resample for one transform into a device uint8 grid of shape
+// (ow, oh, od): out(p) = src(T p), or the exact integer-shift crop when
+// crop.x is set.
+__global__ __launch_bounds__(256) void resample_kernel(const uint8_t* __restrict__ src,
+                                                       uint32_t src_bytes, VolGeo sg, int ow,
+                                                       int oh, int od, TransformArg T, int4 crop,
+                                                       uint8_t* __restrict__ out)
+{
+    const __amdgpu_buffer_rsrc_t rs = make_rsrc(src, src_bytes);
+    const size_t n = size_t(ow) * oh * od;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += size_t(gridDim.x) * blockDim.x) {
+        const uint32_t x = uint32_t(i % uint32_t(ow));
+        const uint32_t r = uint32_t(i / uint32_t(ow));
+        const uint32_t y = r % uint32_t(oh);
+        const uint32_t z = r / uint32_t(oh);
+        uint32_t v;
+        if (crop.x) {
+            int sx = int(x) + crop.y, s_y = int(y) + crop.z, sz = int(z) + crop.w;
+            bool in = sx >= 0 && sx < sg.w && s_y >= 0 && s_y < sg.h && sz >= 0 && sz < sg.d;
+            v = in ? load_u8(rs, uint32_t(sz) * sg.wh + uint32_t(s_y) * uint32_t(sg.w) + uint32_t(sx), 0)
+                   : 0u;
+        } else {
+            Pos p = apply(T.t, float(x), float(y), float(z));
+            v = sample_u8(rs, sg, p.x, p.y, p.z);
+        }
+        out[i] = uint8_t(v);
+    }
+}
+
+__global__ void fill_u64_kernel(unsigned long long* p, size_t n)
+{
+    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (; i < n; i += stride) p[i] = 0ull;
+}
+
+__global__ void fill_u32x4_kernel(uint4* p, size_t n)
+{
+    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (; i < n; i += stride) p[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+constexpr int kMaxK = 4096;
+
+void check_volume(const tiplhip_ctx* ctx, const tiplhip_volume* v, const char* what)
+{
+    TIPLHIP_REQUIRE(v != nullptr, std::string(what) + " is null");
+    TIPLHIP_REQUIRE(v->ctx == ctx, std::string(what) + " belongs to another context");
+    TIPLHIP_REQUIRE(v->n > 0, std::string(what) + " is empty");
+    TIPLHIP_REQUIRE(v->n < (size_t(1) << 31), std::string(what) + " exceeds 2^31 voxels");
+}
+
+void timing_begin(tiplhip_ctx* ctx)
+{
+    if (ctx->kernel_timing) TIPLHIP_CHECK(hipEventRecord(ctx->k0, ctx->stream));
+}
+
+void timing_end(tiplhip_ctx* ctx)
+{
+    if (!ctx->kernel_timing) return;
+    TIPLHIP_CHECK(hipEventRecord(ctx->k1, ctx->stream));
+    TIPLHIP_CHECK(hipEventSynchronize(ctx->k1));
+    float ms = 0.0f;
+    TIPLHIP_CHECK(hipEventElapsedTime(&ms, ctx->k0, ctx->k1));
+    ctx->kernel_ms += ms;
+    ctx->kernel_launches += 1;
+}
+
+void zero_u64(tiplhip_ctx* ctx, void* p, size_t n)
+{
+    if (!n) return;
+    int blocks = int(std::min<size_t>((n + 255) / 256, 1024));
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(blocks), dim3(256), 0, ctx->stream,
+                       static_cast<unsigned long long*>(p), n);
+    TIPLHIP_CHECK(hipGetLastError());
+}
+
+void zero_bytes16(tiplhip_ctx* ctx, void* p, size_t bytes)
+{
+    size_t n = bytes / 16;
+    if (!n) return;
+    int blocks = int(std::min<size_t>((n + 255) / 256, 4096));
+    hipLaunchKernelGGL(fill_u32x4_kernel, dim3(blocks), dim3(256), 0, ctx->stream,
+                       static_cast<uint4*>(p), n);
+    TIPLHIP_CHECK(hipGetLastError());
+}
+
+// Uploads K transforms (and per-candidate crop info) to the context scratch.
+void upload_transforms(tiplhip_ctx* ctx, const float* T, int K, const int4* crop)
+{
+    size_t tbytes = sizeof(float) * 12 * size_t(K);
+    size_t cbytes = sizeof(int4) * size_t(K);
+    ctx->h_T.reserve(tbytes + cbytes);
+    ctx->d_T.reserve(tbytes);
+    ctx->d_aux.reserve(cbytes);
+    std::memcpy(ctx->h_T.p, T, tbytes);
+    TIPLHIP_CHECK(hipMemcpyAsync(ctx->d_T.p, ctx->h_T.p, tbytes, hipMemcpyHostToDevice, ctx->stream));
+    if (crop) {
+        std::memcpy(static_cast<char*>(ctx->h_T.p) + tbytes, crop, cbytes);
+        TIPLHIP_CHECK(hipMemcpyAsync(ctx->d_aux.p, static_cast<char*>(ctx->h_T.p) + tbytes, cbytes,
+                                     hipMemcpyHostToDevice, ctx->stream));
+    }
+}
+
+// Launches the correlation sums for K transforms already in ctx->d_T / d_aux;
+// leaves Sy, Syy, Sxy in ctx->d_sums (K x 4 u64).
+void launch_corr_sums(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_volume* to, int K)
+{
+    ctx->d_sums.reserve(sizeof(unsigned long long) * 4 * size_t(K));
+    zero_u64(ctx, ctx->d_sums.p, 4 * size_t(K));
+    BatchGeo g = make_batch(from, to, K, 4096u, uint32_t(ctx->cu_count) * 8u);
+    timing_begin(ctx);
+    hipLaunchKernelGGL(corr_sums_kernel, dim3(g.total_blocks), dim3(256), 0, ctx->stream, from->d,
+                       to->d, ctx->d_T.as<float>(), ctx->d_aux.as<int4>(), g,
+                       ctx->d_sums.as<unsigned long long>());
+    TIPLHIP_CHECK(hipGetLastError());
+    timing_end(ctx);
+}
+
+// Launches the joint histograms for K transforms in ctx->d_T into ctx->d_hist.
+void launch_mi_hist(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_volume* to, int K,
+                    int bins)
+{
+    size_t hb = sizeof(int) * size_t(bins) * bins * size_t(K);
+    ctx->d_hist.reserve(hb);
+    zero_bytes16(ctx, ctx->d_hist.p, hb);
+    timing_begin(ctx);
+    if (bins == 256) {
+        // 1023 segments of <= 64 voxels keep every uint16 LDS counter below 65536.
+        BatchGeo g = make_batch(from, to, K, 1023u, uint32_t(ctx->cu_count) * 4u);
+        hipLaunchKernelGGL(mi_hist_kernel<256>, dim3(g.total_blocks), dim3(1024), 0, ctx->stream,
+                           from->d, to->d, ctx->d_T.as<float>(), g, ctx->d_hist.as<int>());
+    } else {
+        BatchGeo g = make_batch(from, to, K, 4096u, uint32_t(ctx->cu_count) * 8u);
+        hipLaunchKernelGGL(mi_hist_kernel<64>, dim3(g.total_blocks), dim3(256), 0, ctx->stream,
+                           from->d, to->d, ctx->d_T.as<float>(), g, ctx->d_hist.as<int>());
+    }
+    TIPLHIP_CHECK(hipGetLastError());
+    timing_end(ctx);
+}
+
+void launch_mi_entropy(tiplhip_ctx* ctx, const tiplhip_volume* from, int K, int bins)
+{
+    ctx->d_costs.reserve(sizeof(double) * size_t(K));
+    if (bins == 256)
+        hipLaunchKernelGGL(mi_entropy_kernel<256>, dim3(K), dim3(256), 0, ctx->stream,
+                           ctx->d_hist.as<int>(), from->d_hist, ctx->d_costs.as<double>());
+    else
+        hipLaunchKernelGGL(mi_entropy_kernel<64>, dim3(K), dim3(256), 0, ctx->stream,
+                           ctx->d_hist.as<int>(), from->d_hist + 256, ctx->d_costs.as<double>());
+    TIPLHIP_CHECK(hipGetLastError());
+}
+
+void read_back(tiplhip_ctx* ctx, const void* dev, void* host, size_t bytes)
+{
+    ctx->h_out.reserve(bytes);
+    TIPLHIP_CHECK(hipMemcpyAsync(ctx->h_out.p, dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
+    std::memcpy(host, ctx->h_out.p, bytes);
+}
+
+void cost_batch(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_volume* to,
+                const float* T, int K, int cost, int bins, double* costs)
+{
+    check_volume(ctx, from, "from");
+    check_volume(ctx, to, "to");
+    TIPLHIP_REQUIRE(T != nullptr && costs != nullptr, "null transform or cost buffer");
+    TIPLHIP_REQUIRE(K >= 1 && K <= kMaxK, "K must be in [1, 4096]");
+    TIPLHIP_REQUIRE(cost == TIPLHIP_COST_CORR || cost == TIPLHIP_COST_MI, "unknown cost type");
+    DeviceGuard dg(ctx->device);
+    if (cost == TIPLHIP_COST_CORR) {
+        // reg::correlation swaps the roles when `from` is larger
+        // (reg/linear.hpp:28-33); a singular inverse leaves T unchanged, as
+        // transformation_matrix::inverse does.
+        std::vector<float> Tk(T, T + 12 * size_t(K));
+        if (from->n > to->n) {
+            std::swap(from, to);
+            for (int k = 0; k < K; ++k) (void)matrix_inverse(&Tk[12 * size_t(k)]);
+        }
+        std::vector<int4> crop(static_cast<size_t>(K));
+        for (int k = 0; k < K; ++k) {
+            int pos[3];
+            bool c = is_integer_shift(&Tk[12 * size_t(k)], pos);
+            crop[size_t(k)] = make_int4(c ? 1 : 0, pos[0], pos[1], pos[2]);
+        }
+        upload_transforms(ctx, Tk.data(), K, crop.data());
+        launch_corr_sums(ctx, from, to, K);
+        ctx->d_costs.reserve(sizeof(double) * size_t(K));
+        hipLaunchKernelGGL(corr_finalize_kernel, dim3((K + 63) / 64), dim3(64), 0, ctx->stream,
+                           ctx->d_sums.as<unsigned long long>(), (long long)from->sum,
+                           (long long)from->sum_sq, double(from->n), K,
+                           ctx->d_costs.as<double>(), (long long*)nullptr);
+        TIPLHIP_CHECK(hipGetLastError());
+    } else {
+        TIPLHIP_REQUIRE(bins == 256 || bins == 64, "bins must be 256 or 64");
+        upload_transforms(ctx, T, K, nullptr);
+        launch_mi_hist(ctx, from, to, K, bins);
+        launch_mi_entropy(ctx, from, K, bins);
+    }
+    read_back(ctx, ctx->d_costs.p, costs, sizeof(double) * size_t(K));
+}
+
+}  // namespace
+}  // namespace tiplhip
+
+using namespace tiplhip;
+
+extern "C" {
+
+int tiplhip_resample(tiplhip_ctx* ctx, const tiplhip_volume* src, const int out_dim[3],
+                     const float T[12], int crop_shortcut, uint8_t* out)
+{
+    return guarded([&] {
+        TIPLHIP_REQUIRE(ctx != nullptr, "null context");
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        check_volume(ctx, src, "src");
+        TIPLHIP_REQUIRE(out_dim && out_dim[0] > 0 && out_dim[1] > 0 && out_dim[2] > 0,
+                        "bad output shape");
+        TIPLHIP_REQUIRE(T && out, "null transform or output");
+        DeviceGuard dg(ctx->device);
+        size_t n = vox(out_dim);
+        ctx->d_tmp.reserve(n);
+        TransformArg ta;
+        std::memcpy(ta.t, T, sizeof(ta.t));
+        int pos[3] = {0, 0, 0};
+        bool c = crop_shortcut && is_integer_shift(T, pos);
+        int4 crop = make_int4(c ? 1 : 0, pos[0], pos[1], pos[2]);
+        int blocks = int(std::min<size_t>((n + 255) / 256, 65536));
+        hipLaunchKernelGGL(resample_kernel, dim3(blocks), dim3(256), 0, ctx->stream, src->d,
+                           uint32_t(src->n), make_geo(src->dim), out_dim[0], out_dim[1],
+                           out_dim[2], ta, crop, ctx->d_tmp.as<uint8_t>());
+        TIPLHIP_CHECK(hipGetLastError());
+        read_back(ctx, ctx->d_tmp.p, out, n);
+    });
+}
+
+int tiplhip_corr_sums(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_volume* to,
+                      const float* T, int K, int64_t* sums)
+{
+    return guarded([&] {
+        TIPLHIP_REQUIRE(ctx != nullptr, "null context");
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        check_volume(ctx, from, "from");
+        check_volume(ctx, to, "to");
+        TIPLHIP_REQUIRE(T && sums, "null transform or output");
+        TIPLHIP_REQUIRE(K >= 1 && K <= kMaxK, "K must be in [1, 4096]");
+        DeviceGuard dg(ctx->device);
+        std::vector<int4> crop(size_t(K), make_int4(0, 0, 0, 0));
+        upload_transforms(ctx, T, K, crop.data());
+        launch_corr_sums(ctx, from, to, K);
+        ctx->d_costs.reserve(sizeof(double) * size_t(K));
+        ctx->d_tmp2.reserve(sizeof(long long) * 5 * size_t(K));
+        hipLaunchKernelGGL(corr_finalize_kernel, dim3((K + 63) / 64), dim3(64), 0, ctx->stream,
+                           ctx->d_sums.as<unsigned long long>(), (long long)from->sum,
+                           (long long)from->sum_sq, double(from->n), K,
+                           ctx->d_costs.as<double>(), ctx->d_tmp2.as<long long>());
+        TIPLHIP_CHECK(hipGetLastError());
+        read_back(ctx, ctx->d_tmp2.p, sums, sizeof(int64_t) * 5 * size_t(K));
+    });
+}
+
+int tiplhip_mi_hist(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_volume* to,
+                    const float* T, int K, int bins, int32_t* hist)
+{
+    return guarded([&] {
+        TIPLHIP_REQUIRE(ctx != nullptr, "null context");
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        check_volume(ctx, from, "from");
+        check_volume(ctx, to, "to");
+        TIPLHIP_REQUIRE(T && hist, "null transform or output");
+        TIPLHIP_REQUIRE(K >= 1 && K <= kMaxK, "K must be in [1, 4096]");
+        TIPLHIP_REQUIRE(bins == 256 || bins == 64, "bins must be 256 or 64");
+        DeviceGuard dg(ctx->device);
+        upload_transforms(ctx, T, K, nullptr);
+        launch_mi_hist(ctx, from, to, K, bins);
+        read_back(ctx, ctx->d_hist.p, hist, sizeof(int) * size_t(bins) * bins * size_t(K));
+    });
+}
+
+int tiplhip_cost_batch(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_volume* to,
+                       const float* T, int K, int cost, int bins, double* costs)
+{
+    return guarded([&] {
+        TIPLHIP_REQUIRE(ctx != nullptr, "null context");
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        cost_batch(ctx, from, to, T, K, cost, bins, costs);
+    });
+}
+
+int tiplhip_cost_batch_params(tiplhip_ctx* ctx, const tiplhip_volume* from, const float from_vs[3],
+                              const tiplhip_volume* to, const float to_vs[3], const float* params,
+                              int K, int cost, int bins, double* costs)
+{
+    return guarded([&] {
+        TIPLHIP_REQUIRE(ctx != nullptr, "null context");
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        check_volume(ctx, from, "from");
+        check_volume(ctx, to, "to");
+        TIPLHIP_REQUIRE(params && from_vs && to_vs, "null parameters");
+        TIPLHIP_REQUIRE(K >= 1 && K <= kMaxK, "K must be in [1, 4096]");
+        std::vector<float> T(12 * size_t(K));
+        for (int k = 0; k < K; ++k)
+            affine_matrix(params + 12 * size_t(k), from->dim, from_vs, to->dim, to_vs,
+                          &T[12 * size_t(k)]);
+        cost_batch(ctx, from, to, T.data(), K, cost, bins, costs);
+    });
+}
+
+}  // extern "C"

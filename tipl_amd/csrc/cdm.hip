@@ -1,0 +1,884 @@
+// cdm.hip -- the CDM nonlinear registration loop on gfx950.
+//
+// Restates, MI355X-first:
+//   compose_displacement        numerical/dif.hpp:90-115, 260-285
+//   cdm_get_gradient (r, d)     reg/cdm.hpp:46-152 with get_window_at_width<2>
+//                               (numerical/window.hpp:92-110), linear_regression
+//                               (numerical/statistics.hpp:694-713), gradient_at
+//                               (numerical/numerical.hpp:102-109)
+//   cdm_solve_poisson           reg/cdm.hpp:158-254
+//   cdm_max_displacement_length reg/cdm.hpp:260-296
+//   accumulate_displacement     numerical/dif.hpp:166-186, 311-329
+//   cdm_smooth                  reg/cdm.hpp:302-367
+//   upsample_with_padding       numerical/resampling.hpp:849-891
+//   cdm / cdm_cuda              reg/cdm.hpp:373-532
+//
+// Fields are packed float3 per voxel (the reference's image<3,vector<3>>).
+// Kernels are launched on a 3-D grid of 64 x 4 thread tiles (x = one wave),
+// so no index division is needed.  The per-iteration control (cost history,
+// cdm_improved, smoothing schedule) runs on the host from one float cost
+// read back per iteration; everything else stays on the device.
+#include "device.hpp"
+#include "internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace tiplhip {
+namespace {
+
+struct FieldGeo {
+    int w, h, d;
+    uint32_t wh;
+    uint32_t n;
+    float wm1, hm1, dm1;
+};
+
+FieldGeo field_geo(const int* dim)
+{
+    FieldGeo g;
+    g.w = dim[0];
+    g.h = dim[1];
+    g.d = dim[2];
+    g.wh = uint32_t(dim[0]) * uint32_t(dim[1]);
+    g.n = g.wh * uint32_t(dim[2]);
+    g.wm1 = float(dim[0] - 1);
+    g.hm1 = float(dim[1] - 1);
+    g.dm1 = float(dim[2] - 1);
+    return g;
+}
+
+constexpr int kTX = 64, kTY = 4;
+
+dim3 tile_grid(const FieldGeo& g)
+{
+    return dim3(uint32_t((g.w + kTX - 1) / kTX), uint32_t((g.h + kTY - 1) / kTY), uint32_t(g.d));
+}
+
+struct Vox {
+    int x, y, z;
+    uint32_t p;
+    bool ok;
+};
+
+__device__ inline Vox this_voxel(const FieldGeo& g)
+{
+    Vox v;
+    v.x = int(blockIdx.x) * kTX + int(threadIdx.x);
+    v.y = int(blockIdx.y) * kTY + int(threadIdx.y);
+    v.z = int(blockIdx.z);
+    v.ok = v.x < g.w && v.y < g.h;
+    v.p = uint32_t(v.z) * g.wh + uint32_t(v.y) * uint32_t(g.w) + uint32_t(v.x);
+    return v;
+}
+
+__device__ inline float3 ld3(const float* f, uint32_t p)
+{
+    return make_float3(f[3 * size_t(p)], f[3 * size_t(p) + 1], f[3 * size_t(p) + 2]);
+}
+
+__device__ inline void st3(float* f, uint32_t p, float3 v)
+{
+    f[3 * size_t(p)] = v.x;
+    f[3 * size_t(p) + 1] = v.y;
+    f[3 * size_t(p) + 2] = v.z;
+}
+
+// Trilinear location as interpolator::linear<3>::get_location
+// (numerical/interpolation.hpp:337-383); false when outside.
+struct Tri {
+    uint32_t i0;
+    int ix, iy, iz;
+    float w[8];
+};
+
+__device__ inline bool locate(const FieldGeo& g, float x, float y, float z, Tri& t)
+{
+    if (!(x >= 0.0f && y >= 0.0f && z >= 0.0f && x < g.wm1 && y < g.hm1 && z < g.dm1)) return false;
+    const uint32_t ix = uint32_t(x), iy = uint32_t(y), iz = uint32_t(z);
+    const float p0 = x - float(ix), p1 = y - float(iy), p2 = z - float(iz);
+    const float n0 = 1.0f - p0, n1 = 1.0f - p1, n2 = 1.0f - p2;
+    t.i0 = iz * g.wh + iy * uint32_t(g.w) + ix;
+    t.ix = int(ix);
+    t.iy = int(iy);
+    t.iz = int(iz);
+    const float n0n1 = n0 * n1, p0n1 = p0 * n1, n0p1 = n0 * p1, p0p1 = p0 * p1;
+    t.w[0] = n0n1 * n2;
+    t.w[1] = p0n1 * n2;
+    t.w[2] = n0p1 * n2;
+    t.w[3] = p0p1 * n2;
+    t.w[4] = n0n1 * p2;
+    t.w[5] = p0n1 * p2;
+    t.w[6] = n0p1 * p2;
+    t.w[7] = p0p1 * p2;
+    return true;
+}
+
+__device__ inline uint32_t corner(const FieldGeo& g, uint32_t i0, int k)
+{
+    return i0 + uint32_t(k & 1) + ((k & 2) ? uint32_t(g.w) : 0u) + ((k & 4) ? g.wh : 0u);
+}
+
+// ---------------------------------------------------------------------------
+// compose_displacement: Js(x) = Is(x + d(x)); exact copy where d(x) == 0.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void compose_kernel(const uint8_t* __restrict__ Is,
+                                                      const float* __restrict__ d, FieldGeo g,
+                                                      uint8_t* __restrict__ Js)
+{
+    const Vox v = this_voxel(g);
+    if (!v.ok) return;
+    const float3 dv = ld3(d, v.p);
+    if (dv.x == 0.0f && dv.y == 0.0f && dv.z == 0.0f) {
+        Js[v.p] = Is[v.p];
+        return;
+    }
+    const __amdgpu_buffer_rsrc_t r = make_rsrc(Is, g.n);
+    VolGeo vg{g.w, g.h, g.d, g.wh, g.wm1, g.hm1, g.dm1};
+    Js[v.p] = uint8_t(sample_u8(r, vg, float(v.x) + dv.x, float(v.y) + dv.y, float(v.z) + dv.z));
+}
+
+// ---------------------------------------------------------------------------
+// cdm_get_gradient.  'r': local linear regression of Js onto It over the
+// border-clipped 5x5x5 window (exact integer sums, then FP64 as in
+// linear_regression), gradient scaled by r2*(a*Js+b-It) where a > 0.
+// 'd': SSD gradient.  accumulate = 0 writes nd = 0 + g (a fresh field);
+// accumulate = 1 adds a further modality: nd = nd + (0 + g).
+// Per-block FP64 partial sums of the cost map go to `partials`.
+// ---------------------------------------------------------------------------
+template <char TYPE>
+__global__ __launch_bounds__(256) void gradient_kernel(const uint8_t* __restrict__ Js,
+                                                       const uint8_t* __restrict__ It, FieldGeo g,
+                                                       int accumulate, float* __restrict__ nd,
+                                                       double* __restrict__ partials)
+{
+    const Vox v = this_voxel(g);
+    float3 gr = make_float3(0.0f, 0.0f, 0.0f);
+    float cost = 0.0f;
+    if (v.ok) {
+        const uint32_t js = Js[v.p], it = It[v.p];
+        bool active;
+        float s = 0.0f;
+        if (TYPE == 'd') {
+            const int dif = int(js) - int(it);
+            s = float(dif);
+            cost = float(dif * dif);
+            active = true;
+        } else {
+            active = false;
+            if (js != 0u && it != 0u) {
+                const int x0 = max(v.x - 2, 0), x1 = min(v.x + 2, g.w - 1);
+                const int y0 = max(v.y - 2, 0), y1 = min(v.y + 2, g.h - 1);
+                const int z0 = max(v.z - 2, 0), z1 = min(v.z + 2, g.d - 1);
+                const __amdgpu_buffer_rsrc_t rj = make_rsrc(Js, g.n);
+                const __amdgpu_buffer_rsrc_t ri = make_rsrc(It, g.n);
+                uint32_t sx = 0, sxx = 0, sy = 0, syy = 0, sxy = 0;
+                for (int zz = z0; zz <= z1; ++zz)
+                    for (int yy = y0; yy <= y1; ++yy) {
+                        const uint32_t row = uint32_t(zz) * g.wh + uint32_t(yy) * uint32_t(g.w);
+                        for (int xx = x0; xx <= x1; ++xx) {
+                            const uint32_t a = load_u8(rj, row + uint32_t(xx), 0);
+                            const uint32_t b = load_u8(ri, row + uint32_t(xx), 0);
+                            sx += a;
+                            sxx += a * a;
+                            sy += b;
+                            syy += b * b;
+                            sxy += a * b;
+                        }
+                    }
+                const double dn = double((x1 - x0 + 1) * (y1 - y0 + 1) * (z1 - z0 + 1));
+                const double mean_x = double(sx) / dn;
+                const double mean_y = double(sy) / dn;
+                const double x_var = double(sxx) / dn - mean_x * mean_x;
+                const double y_var = double(syy) / dn - mean_y * mean_y;
+                if (!(x_var == 0.0 || y_var == 0.0)) {
+                    const double cov = double(sxy) / dn - mean_x * mean_y;
+                    const float a = float(cov / x_var);
+                    if (a > 0.0f) {
+                        const float b = float(mean_y - double(a) * mean_x);
+                        const float r2 = float(cov * cov / x_var / y_var);
+                        s = float(js) * a;
+                        s = s + b;
+                        s = s - float(it);
+                        s = r2 * s;
+                        cost = -r2;
+                        active = true;
+                    }
+                }
+            }
+        }
+        if (active) {
+            // gradient_at: central differences, 0 on the first/last slab.
+            const uint32_t w = uint32_t(g.w);
+            const float gx = (v.x + 1 < g.w && v.x) ? float(int(Js[v.p + 1]) - int(Js[v.p - 1])) : 0.0f;
+            const float gy = (v.y + 1 < g.h && v.y) ? float(int(Js[v.p + w]) - int(Js[v.p - w])) : 0.0f;
+            const float gz = (v.z + 1 < g.d && v.z) ? float(int(Js[v.p + g.wh]) - int(Js[v.p - g.wh])) : 0.0f;
+            gr.x = 0.0f + gx * s;
+            gr.y = 0.0f + gy * s;
+            gr.z = 0.0f + gz * s;
+        }
+        if (accumulate) {
+            float3 o = ld3(nd, v.p);
+            o.x = o.x + gr.x;
+            o.y = o.y + gr.y;
+            o.z = o.z + gr.z;
+            st3(nd, v.p, o);
+        } else {
+            st3(nd, v.p, gr);
+        }
+    }
+    // deterministic per-block partial sum of the cost map
+    __shared__ double part[4];
+    double c = wave_sum_f64(double(cost));
+    const uint32_t tid = threadIdx.y * blockDim.x + threadIdx.x;
+    if ((tid & 63u) == 0) part[tid >> 6] = c;
+    __syncthreads();
+    if (tid == 0) {
+        const uint32_t b = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        partials[b] = (part[0] + part[1]) + (part[2] + part[3]);
+    }
+}
+
+// Sums the block partials in a fixed order and stores float(sum / n).
+__global__ __launch_bounds__(256) void cost_reduce_kernel(const double* __restrict__ partials,
+                                                          uint32_t count, double n,
+                                                          float* __restrict__ out)
+{
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) acc += partials[i];
+    __shared__ double part[4];
+    acc = wave_sum_f64(acc);
+    if ((threadIdx.x & 63u) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) out[0] = float(((part[0] + part[1]) + (part[2] + part[3])) / n);
+}
+
+// ---------------------------------------------------------------------------
+// cdm_solve_poisson: one Jacobi sweep, 7-point stencil with neighbours taken
+// by flat index and dropped at the array ends.  FIRST: the previous iterate
+// is nd * (-1/6), formed on the fly.  MAXLEN: also the max of
+// float(sqrt(double(|v|^2))) (cdm_max_displacement_length) as float bits.
+// ---------------------------------------------------------------------------
+constexpr float kInvD2 = -0.5f / 3.0f;
+
+template <bool FIRST, bool MAXLEN>
+__global__ __launch_bounds__(256) void poisson_kernel(const float* __restrict__ nd,
+                                                      const float* __restrict__ a, FieldGeo g,
+                                                      float* __restrict__ b,
+                                                      unsigned int* __restrict__ maxlen)
+{
+    const Vox v = this_voxel(g);
+    float len = 0.0f;
+    if (v.ok) {
+        auto A = [&](uint32_t q) -> float3 {
+            if (FIRST) {
+                float3 t = ld3(nd, q);
+                return make_float3(t.x * kInvD2, t.y * kInvD2, t.z * kInvD2);
+            }
+            return ld3(a, q);
+        };
+        const uint32_t p = v.p, n = g.n, w = uint32_t(g.w), wh = g.wh;
+        float3 r = ld3(nd, p);
+        auto sub = [&](float3 t) {
+            r.x = r.x - t.x;
+            r.y = r.y - t.y;
+            r.z = r.z - t.z;
+        };
+        if (p >= 1u) sub(A(p - 1));
+        if (p < n - 1u) sub(A(p + 1));
+        if (p >= w) sub(A(p - w));
+        if (p < n - w) sub(A(p + w));
+        if (p >= wh) sub(A(p - wh));
+        if (p < n - wh) sub(A(p + wh));
+        r.x = r.x * kInvD2;
+        r.y = r.y * kInvD2;
+        r.z = r.z * kInvD2;
+        st3(b, p, r);
+        if (MAXLEN) {
+            float s2 = r.x * r.x + r.y * r.y;
+            s2 = s2 + r.z * r.z;
+            len = float(sqrt(double(s2)));
+        }
+    }
+    if (MAXLEN) {
+        __shared__ float part[4];
+        const uint32_t tid = threadIdx.y * blockDim.x + threadIdx.x;
+        float m = wave_max_f32(len);
+        if ((tid & 63u) == 0) part[tid >> 6] = m;
+        __syncthreads();
+        if (tid == 0) {
+            m = fmaxf(fmaxf(part[0], part[1]), fmaxf(part[2], part[3]));
+            atomicMax(maxlen, __float_as_uint(m));  // lengths are >= 0: bit order = value order
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// new_d *= k, then accumulate_displacement(cur, new_d) in place:
+// new(x) = m(x + new(x)) - x with m(y) = cur(y) + y by trilinear
+// interpolation; cur(x) where new(x) == 0 or the sample is outside.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void accumulate_kernel(const float* __restrict__ cur,
+                                                         float* __restrict__ nd, float k,
+                                                         FieldGeo g)
+{
+    const Vox v = this_voxel(g);
+    if (!v.ok) return;
+    float3 d = ld3(nd, v.p);
+    d.x = d.x * k;
+    d.y = d.y * k;
+    d.z = d.z * k;
+    if (d.x != 0.0f || d.y != 0.0f || d.z != 0.0f) {
+        Tri t;
+        if (locate(g, d.x + float(v.x), d.y + float(v.y), d.z + float(v.z), t)) {
+            const int ix = t.ix, iy = t.iy, iz = t.iz;
+            float3 acc = make_float3(0.0f, 0.0f, 0.0f);
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const float3 m = ld3(cur, corner(g, t.i0, c));
+                const float mx = m.x + float(ix + (c & 1));
+                const float my = m.y + float(iy + ((c >> 1) & 1));
+                const float mz = m.z + float(iz + ((c >> 2) & 1));
+                if (c == 0) {
+                    acc.x = mx * t.w[0];
+                    acc.y = my * t.w[0];
+                    acc.z = mz * t.w[0];
+                } else {
+                    acc.x = acc.x + mx * t.w[c];
+                    acc.y = acc.y + my * t.w[c];
+                    acc.z = acc.z + mz * t.w[c];
+                }
+            }
+            st3(nd, v.p,
+                make_float3(acc.x - float(v.x), acc.y - float(v.y), acc.z - float(v.z)));
+            return;
+        }
+    }
+    st3(nd, v.p, ld3(cur, v.p));
+}
+
+// cdm_smooth: dd = d*(1-s) + (s/6)*(neighbours +1, -1, +w, -w, +wh, -wh by
+// flat index, dropped at the array ends).
+__global__ __launch_bounds__(256) void smooth_kernel(const float* __restrict__ d,
+                                                     float* __restrict__ dd, float w_6, float w_1,
+                                                     FieldGeo g)
+{
+    const Vox v = this_voxel(g);
+    if (!v.ok) return;
+    const uint32_t p = v.p, n = g.n, w = uint32_t(g.w), wh = g.wh;
+    float3 s = make_float3(0.0f, 0.0f, 0.0f);
+    auto add = [&](uint32_t q) {
+        const float3 t = ld3(d, q);
+        s.x = s.x + t.x;
+        s.y = s.y + t.y;
+        s.z = s.z + t.z;
+    };
+    if (p + 1u < n) add(p + 1u);
+    if (p >= 1u) add(p - 1u);
+    if (p + w < n) add(p + w);
+    if (p >= w) add(p - w);
+    if (p + wh < n) add(p + wh);
+    if (p >= wh) add(p - wh);
+    const float3 c = ld3(d, p);
+    st3(dd, p, make_float3(c.x * w_1 + s.x * w_6, c.y * w_1 + s.y * w_6, c.z * w_1 + s.z * w_6));
+}
+
+// best_d *= 2 then upsample_with_padding<linear>: out(p) = in2(p * 0.5),
+// 0 outside.
+__global__ __launch_bounds__(256) void upsample_kernel(const float* __restrict__ in, FieldGeo gi,
+                                                       FieldGeo go, float* __restrict__ out)
+{
+    const Vox v = this_voxel(go);
+    if (!v.ok) return;
+    float x = float(v.x), y = float(v.y), z = float(v.z);
+    x = x * 0.5f;
+    y = y * 0.5f;
+    z = z * 0.5f;
+    Tri t;
+    float3 acc = make_float3(0.0f, 0.0f, 0.0f);
+    if (locate(gi, x, y, z, t)) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            float3 m = ld3(in, corner(gi, t.i0, c));
+            m.x = m.x * 2.0f;
+            m.y = m.y * 2.0f;
+            m.z = m.z * 2.0f;
+            if (c == 0) {
+                acc.x = m.x * t.w[0];
+                acc.y = m.y * t.w[0];
+                acc.z = m.z * t.w[0];
+            } else {
+                acc.x = acc.x + m.x * t.w[c];
+                acc.y = acc.y + m.y * t.w[c];
+                acc.z = acc.z + m.z * t.w[c];
+            }
+        }
+    }
+    st3(out, v.p, acc);
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+struct DevMem {
+    void* p = nullptr;
+    DevMem() = default;
+    explicit DevMem(size_t bytes)
+    {
+        if (bytes) TIPLHIP_CHECK(hipMalloc(&p, bytes));
+    }
+    DevMem(const DevMem&) = delete;
+    DevMem& operator=(const DevMem&) = delete;
+    DevMem(DevMem&& o) noexcept : p(o.p) { o.p = nullptr; }
+    ~DevMem()
+    {
+        if (p) (void)hipFree(p);
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+void zero(tiplhip_ctx* ctx, void* p, size_t bytes)
+{
+    TIPLHIP_CHECK(hipMemsetAsync(p, 0, bytes, ctx->stream));
+}
+
+void copy_d2d(tiplhip_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+    TIPLHIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+}
+
+uint32_t block_count(const FieldGeo& g)
+{
+    dim3 t = tile_grid(g);
+    return t.x * t.y * t.z;
+}
+
+void launch_compose(tiplhip_ctx* ctx, const uint8_t* Is, const float* d, const FieldGeo& g,
+                    uint8_t* Js)
+{
+    hipLaunchKernelGGL(compose_kernel, tile_grid(g), dim3(kTX, kTY), 0, ctx->stream, Is, d, g, Js);
+    TIPLHIP_CHECK(hipGetLastError());
+}
+
+void launch_gradient(tiplhip_ctx* ctx, const uint8_t* Js, const uint8_t* It, const FieldGeo& g,
+                     char type, int accumulate, float* nd, double* partials, float* cost_out)
+{
+    if (type == 'd')
+        hipLaunchKernelGGL(gradient_kernel<'d'>, tile_grid(g), dim3(kTX, kTY), 0, ctx->stream, Js,
+                           It, g, accumulate, nd, partials);
+    else
+        hipLaunchKernelGGL(gradient_kernel<'r'>, tile_grid(g), dim3(kTX, kTY), 0, ctx->stream, Js,
+                           It, g, accumulate, nd, partials);
+    TIPLHIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(cost_reduce_kernel, dim3(1), dim3(256), 0, ctx->stream, partials,
+                       block_count(g), double(g.n), cost_out);
+    TIPLHIP_CHECK(hipGetLastError());
+}
+
+// `sweeps` Jacobi sweeps of nd; returns the buffer holding the result (a or b).
+// With maxlen != nullptr the last sweep also writes the max length bits.
+float* launch_poisson(tiplhip_ctx* ctx, const float* nd, float* a, float* b, const FieldGeo& g,
+                      int sweeps, unsigned int* maxlen)
+{
+    float* src = a;
+    float* dst = b;
+    for (int it = 0; it < sweeps; ++it) {
+        const bool first = it == 0, last = it + 1 == sweeps && maxlen;
+        if (first && last)
+            hipLaunchKernelGGL((poisson_kernel<true, true>), tile_grid(g), dim3(kTX, kTY), 0,
+                               ctx->stream, nd, src, g, dst, maxlen);
+        else if (first)
+            hipLaunchKernelGGL((poisson_kernel<true, false>), tile_grid(g), dim3(kTX, kTY), 0,
+                               ctx->stream, nd, src, g, dst, maxlen);
+        else if (last)
+            hipLaunchKernelGGL((poisson_kernel<false, true>), tile_grid(g), dim3(kTX, kTY), 0,
+                               ctx->stream, nd, src, g, dst, maxlen);
+        else
+            hipLaunchKernelGGL((poisson_kernel<false, false>), tile_grid(g), dim3(kTX, kTY), 0,
+                               ctx->stream, nd, src, g, dst, maxlen);
+        TIPLHIP_CHECK(hipGetLastError());
+        std::swap(src, dst);
+    }
+    return src;
+}
+
+void launch_accumulate(tiplhip_ctx* ctx, const float* cur, float* nd, float k, const FieldGeo& g)
+{
+    hipLaunchKernelGGL(accumulate_kernel, tile_grid(g), dim3(kTX, kTY), 0, ctx->stream, cur, nd, k, g);
+    TIPLHIP_CHECK(hipGetLastError());
+}
+
+void launch_smooth(tiplhip_ctx* ctx, const float* d, float* dd, float smoothing, const FieldGeo& g)
+{
+    const float w_6 = smoothing / float(2.0f * 3.0f);
+    const float w_1 = 1.0f - smoothing;
+    hipLaunchKernelGGL(smooth_kernel, tile_grid(g), dim3(kTX, kTY), 0, ctx->stream, d, dd, w_6, w_1, g);
+    TIPLHIP_CHECK(hipGetLastError());
+}
+
+void launch_upsample(tiplhip_ctx* ctx, const float* in, const FieldGeo& gi, const FieldGeo& go,
+                     float* out)
+{
+    hipLaunchKernelGGL(upsample_kernel, tile_grid(go), dim3(kTX, kTY), 0, ctx->stream, in, gi, go, out);
+    TIPLHIP_CHECK(hipGetLastError());
+}
+
+// cdm_improved (reg/cdm.hpp:19-34) with linear_regression over (index, cost)
+// (numerical/statistics.hpp:694-713): false when the cost trend rises;
+// drops the oldest entry once more than 7 are held.
+bool cdm_improved(std::vector<float>& cost)
+{
+    const size_t sz = cost.size();
+    if (sz > 5) {
+        int64_t sx = 0, sxx = 0;
+        double sy = 0.0, syy = 0.0, sxy = 0.0;
+        for (size_t i = 0; i < sz; ++i) {
+            sx += int64_t(i);
+            sxx += int64_t(i) * int64_t(i);
+            sy += double(cost[i]);
+            syy += double(cost[i]) * double(cost[i]);
+            sxy += double(float(int(i)) * cost[i]);
+        }
+        const double dn = double(sz);
+        const double mean_x = double(sx) / dn, mean_y = sy / dn;
+        const double x_var = double(sxx) / dn - mean_x * mean_x;
+        const double y_var = syy / dn - mean_y * mean_y;
+        float a = 0.0f;
+        if (!(x_var == 0.0 || y_var == 0.0)) a = float((sxy / dn - mean_x * mean_y) / x_var);
+        if (a > 0.0f) return false;
+        if (sz > 7) cost.erase(cost.begin());
+    }
+    return true;
+}
+
+struct CdmRun {
+    tiplhip_ctx* ctx;
+    tiplhip_cdm_param param;
+    int nmod;
+    const volatile int* terminated;
+    int iterations = 0;
+    bool no_early_exit = false;  // timing mode: run max_iterations exactly
+    bool stopped = false;        // terminated by the caller
+};
+
+bool is_terminated(const CdmRun& r) { return r.terminated && *r.terminated; }
+
+// One cdm() call at shape dim.  best_d is a zeroed device field of vox(dim).
+bool cdm_level(CdmRun& run, const std::vector<const uint8_t*>& It,
+               const std::vector<const uint8_t*>& Is, const int* dim, float* best_d,
+               float resolution)
+{
+    tiplhip_ctx* ctx = run.ctx;
+    const FieldGeo g = field_geo(dim);
+    const size_t n = g.n, fb = sizeof(float) * 3 * n;
+    if (!run.no_early_exit && std::min({dim[0], dim[1], dim[2]}) > run.param.min_dimension) {
+        const int rd[3] = {(dim[0] + 1) / 2, (dim[1] + 1) / 2, (dim[2] + 1) / 2};
+        const size_t rn = vox(rd);
+        DevMem rbuf(2 * size_t(run.nmod) * rn);
+        std::vector<const uint8_t*> rIt, rIs;
+        for (int i = 0; i < run.nmod; ++i) {
+            uint8_t* t = rbuf.as<uint8_t>() + size_t(2 * i) * rn;
+            uint8_t* s = rbuf.as<uint8_t>() + size_t(2 * i + 1) * rn;
+            launch_downsample_u8(ctx, It[size_t(i)], dim, t);
+            launch_downsample_u8(ctx, Is[size_t(i)], dim, s);
+            rIt.push_back(t);
+            rIs.push_back(s);
+        }
+        DevMem rbest(sizeof(float) * 3 * rn);
+        zero(ctx, rbest.p, sizeof(float) * 3 * rn);
+        if (!cdm_level(run, rIt, rIs, rd, rbest.as<float>(), resolution / 2.0f)) return false;
+        launch_upsample(ctx, rbest.as<float>(), field_geo(rd), g, best_d);
+        TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));  // rbuf / rbest are freed next
+        if (resolution > 1.0f) return true;
+    }
+    if (run.stopped) return true;
+
+    DevMem cur(fb), nd(fb), pa(fb), pb(fb), Js(n);
+    const uint32_t nblocks = block_count(g);
+    DevMem partials(sizeof(double) * nblocks), scal(sizeof(float) * 16 + sizeof(unsigned int));
+    float* d_cost = scal.as<float>();
+    unsigned int* d_maxlen = reinterpret_cast<unsigned int*>(scal.as<float>() + 16);
+    copy_d2d(ctx, cur.p, best_d, fb);
+
+    float theta = 0.0f;
+    std::vector<float> cost;
+    float best_cost = 3.402823466e+38f;
+    float cur_smoothing = 0.8f;
+    std::vector<float> mod_cost(size_t(run.nmod));
+    for (int index = 0; index < run.param.max_iterations; ++index) {
+        if (is_terminated(run)) {
+            run.stopped = true;
+            break;
+        }
+        ++run.iterations;
+        for (int i = 0; i < run.nmod; ++i) {
+            launch_compose(ctx, Is[size_t(i)], cur.as<float>(), g, Js.as<uint8_t>());
+            const char type = run.param.gradient_type[std::min(i, 7)] == 'd' ? 'd' : 'r';
+            launch_gradient(ctx, Js.as<uint8_t>(), It[size_t(i)], g, type, i > 0 ? 1 : 0,
+                            nd.as<float>(), partials.as<double>(), d_cost + i);
+        }
+        TIPLHIP_CHECK(hipMemcpyAsync(mod_cost.data(), d_cost, sizeof(float) * size_t(run.nmod),
+                                     hipMemcpyDeviceToHost, ctx->stream));
+        TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
+        float sum_cost = 0.0f;
+        for (int i = 0; i < run.nmod; ++i) {
+            sum_cost += mod_cost[size_t(i)];
+            if (sum_cost == 0.0f && !run.no_early_exit) return false;
+        }
+        cost.push_back(sum_cost / float(run.nmod));
+        if (cost.back() < best_cost) {
+            best_cost = cost.back();
+            if (index) copy_d2d(ctx, best_d, cur.p, fb);
+        }
+        if (!cdm_improved(cost) && !run.no_early_exit) {
+            if (cur_smoothing <= run.param.smoothing) break;
+            copy_d2d(ctx, cur.p, best_d, fb);
+            cur_smoothing *= 0.5f;
+            cost.clear();
+        }
+        const bool need_theta = theta == 0.0f;
+        if (need_theta) zero(ctx, d_maxlen, sizeof(unsigned int));
+        float* solved = launch_poisson(ctx, nd.as<float>(), pa.as<float>(), pb.as<float>(), g, 12,
+                                       need_theta ? d_maxlen : nullptr);
+        if (need_theta) {
+            unsigned int bits = 0;
+            TIPLHIP_CHECK(hipMemcpyAsync(&bits, d_maxlen, sizeof(bits), hipMemcpyDeviceToHost,
+                                         ctx->stream));
+            TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
+            std::memcpy(&theta, &bits, sizeof(theta));
+        }
+        if (theta == 0.0f) break;
+        const float k = run.param.speed / theta;
+        launch_accumulate(ctx, cur.as<float>(), solved, k, g);
+        if (cur_smoothing != 0.0f) launch_smooth(ctx, solved, cur.as<float>(), cur_smoothing, g);
+    }
+    TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return true;
+}
+
+std::vector<const uint8_t*> volume_ptrs(tiplhip_ctx* ctx, const tiplhip_volume* const* v, int n,
+                                        const int* dim)
+{
+    std::vector<const uint8_t*> out;
+    for (int i = 0; i < n; ++i) {
+        TIPLHIP_REQUIRE(v[i] != nullptr, "null volume");
+        TIPLHIP_REQUIRE(v[i]->ctx == ctx, "volume belongs to another context");
+        TIPLHIP_REQUIRE(std::equal(dim, dim + 3, v[i]->dim), "all CDM volumes must share one shape");
+        out.push_back(v[i]->d);
+    }
+    return out;
+}
+
+void check_dim(const int* dim)
+{
+    TIPLHIP_REQUIRE(dim && dim[0] > 0 && dim[1] > 0 && dim[2] > 0, "dims must be positive");
+    TIPLHIP_REQUIRE(vox(dim) < (size_t(1) << 31), "volume exceeds 2^31 voxels");
+}
+
+}  // namespace
+}  // namespace tiplhip
+
+using namespace tiplhip;
+
+extern "C" {
+
+void tiplhip_cdm_param_default(tiplhip_cdm_param* p)
+{
+    if (!p) return;
+    p->resolution = 2.0f;
+    p->speed = 0.3f;
+    p->smoothing = 0.05f;
+    p->min_dimension = 16;
+    p->max_iterations = 200;
+    for (char& c : p->gradient_type) c = 'r';
+}
+
+int tiplhip_cdm(tiplhip_ctx* ctx, const tiplhip_volume* const* It, const tiplhip_volume* const* Is,
+                int n, const tiplhip_cdm_param* param, float* field, const volatile int* terminated,
+                int* iterations)
+{
+    return guarded([&] {
+        TIPLHIP_REQUIRE(ctx && It && Is && field, "null argument");
+        TIPLHIP_REQUIRE(n >= 1 && n <= 8, "between 1 and 8 modality pairs");
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        DeviceGuard dg(ctx->device);
+        const int* dim = It[0] ? It[0]->dim : nullptr;
+        TIPLHIP_REQUIRE(dim != nullptr, "null volume");
+        auto vt = volume_ptrs(ctx, It, n, dim);
+        auto vs = volume_ptrs(ctx, Is, n, dim);
+        CdmRun run{ctx, {}, n, terminated};
+        if (param)
+            run.param = *param;
+        else
+            tiplhip_cdm_param_default(&run.param);
+        TIPLHIP_REQUIRE(run.param.max_iterations >= 0, "negative iteration cap");
+        const size_t fb = sizeof(float) * 3 * vox(dim);
+        DevMem best(fb);
+        zero(ctx, best.p, fb);
+        const bool ok = cdm_level(run, vt, vs, dim, best.as<float>(), run.param.resolution);
+        if (iterations) *iterations = run.iterations;
+        if (!ok) throw Error(TIPLHIP_ERR_FAILED, "cdm returned false (zero cost)");
+        TIPLHIP_CHECK(hipMemcpyAsync(field, best.p, fb, hipMemcpyDeviceToHost, ctx->stream));
+        TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
+        // As in the reference, a terminated run still returns its best field.
+        if (run.stopped) throw Error(TIPLHIP_ERR_TERMINATED, "terminated by the caller");
+    });
+}
+
+int tiplhip_cdm_level_iterations(tiplhip_ctx* ctx, const tiplhip_volume* It,
+                                 const tiplhip_volume* Is, const tiplhip_cdm_param* param,
+                                 int iterations, float* field)
+{
+    return guarded([&] {
+        TIPLHIP_REQUIRE(ctx && It && Is, "null argument");
+        TIPLHIP_REQUIRE(iterations >= 0, "negative iteration count");
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        DeviceGuard dg(ctx->device);
+        const int* dim = It->dim;
+        auto vt = volume_ptrs(ctx, &It, 1, dim);
+        auto vs = volume_ptrs(ctx, &Is, 1, dim);
+        CdmRun run{ctx, {}, 1, nullptr};
+        if (param)
+            run.param = *param;
+        else
+            tiplhip_cdm_param_default(&run.param);
+        run.param.max_iterations = iterations;
+        run.no_early_exit = true;
+        const size_t fb = sizeof(float) * 3 * vox(dim);
+        DevMem best(fb);
+        zero(ctx, best.p, fb);
+        (void)cdm_level(run, vt, vs, dim, best.as<float>(), 1.0f);
+        if (field) {
+            TIPLHIP_CHECK(hipMemcpyAsync(field, best.p, fb, hipMemcpyDeviceToHost, ctx->stream));
+            TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
+        }
+    });
+}
+
+// ---- single steps on host buffers ----------------------------------------
+
+int tiplhip_compose_displacement(tiplhip_ctx* ctx, const uint8_t* Is, const int dim[3],
+                                 const float* d, uint8_t* Js)
+{
+    return guarded([&] {
+        TIPLHIP_REQUIRE(ctx && Is && d && Js, "null argument");
+        check_dim(dim);
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        DeviceGuard dg(ctx->device);
+        const FieldGeo g = field_geo(dim);
+        DevMem dI(g.n), dD(sizeof(float) * 3 * g.n), dJ(g.n);
+        TIPLHIP_CHECK(hipMemcpyAsync(dI.p, Is, g.n, hipMemcpyHostToDevice, ctx->stream));
+        TIPLHIP_CHECK(hipMemcpyAsync(dD.p, d, sizeof(float) * 3 * g.n, hipMemcpyHostToDevice, ctx->stream));
+        launch_compose(ctx, dI.as<uint8_t>(), dD.as<float>(), g, dJ.as<uint8_t>());
+        TIPLHIP_CHECK(hipMemcpyAsync(Js, dJ.p, g.n, hipMemcpyDeviceToHost, ctx->stream));
+        TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int tiplhip_cdm_gradient(tiplhip_ctx* ctx, const uint8_t* Js, const uint8_t* It, const int dim[3],
+                         char type, float* new_d, float* cost)
+{
+    return guarded([&] {
+        TIPLHIP_REQUIRE(ctx && Js && It && new_d, "null argument");
+        TIPLHIP_REQUIRE(type == 'r' || type == 'd', "gradient type must be 'r' or 'd'");
+        check_dim(dim);
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        DeviceGuard dg(ctx->device);
+        const FieldGeo g = field_geo(dim);
+        DevMem dJ(g.n), dI(g.n), dN(sizeof(float) * 3 * g.n);
+        DevMem part(sizeof(double) * block_count(g)), dc(sizeof(float));
+        TIPLHIP_CHECK(hipMemcpyAsync(dJ.p, Js, g.n, hipMemcpyHostToDevice, ctx->stream));
+        TIPLHIP_CHECK(hipMemcpyAsync(dI.p, It, g.n, hipMemcpyHostToDevice, ctx->stream));
+        launch_gradient(ctx, dJ.as<uint8_t>(), dI.as<uint8_t>(), g, type, 0, dN.as<float>(),
+                        part.as<double>(), dc.as<float>());
+        TIPLHIP_CHECK(hipMemcpyAsync(new_d, dN.p, sizeof(float) * 3 * g.n, hipMemcpyDeviceToHost, ctx->stream));
+        float c = 0.0f;
+        TIPLHIP_CHECK(hipMemcpyAsync(&c, dc.p, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
+        if (cost) *cost = c;
+    });
+}
+
+int tiplhip_cdm_poisson(tiplhip_ctx* ctx, float* new_d, const int dim[3], int sweeps,
+                        float* max_length)
+{
+    return guarded([&] {
+        TIPLHIP_REQUIRE(ctx && new_d, "null argument");
+        TIPLHIP_REQUIRE(sweeps >= 1 && sweeps <= 1000, "sweeps must be in [1, 1000]");
+        check_dim(dim);
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        DeviceGuard dg(ctx->device);
+        const FieldGeo g = field_geo(dim);
+        const size_t fb = sizeof(float) * 3 * g.n;
+        DevMem dN(fb), a(fb), b(fb), ml(sizeof(unsigned int));
+        TIPLHIP_CHECK(hipMemcpyAsync(dN.p, new_d, fb, hipMemcpyHostToDevice, ctx->stream));
+        zero(ctx, ml.p, sizeof(unsigned int));
+        float* out = launch_poisson(ctx, dN.as<float>(), a.as<float>(), b.as<float>(), g, sweeps,
+                                    ml.as<unsigned int>());
+        TIPLHIP_CHECK(hipMemcpyAsync(new_d, out, fb, hipMemcpyDeviceToHost, ctx->stream));
+        unsigned int bits = 0;
+        TIPLHIP_CHECK(hipMemcpyAsync(&bits, ml.p, sizeof(bits), hipMemcpyDeviceToHost, ctx->stream));
+        TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
+        if (max_length) std::memcpy(max_length, &bits, sizeof(float));
+    });
+}
+
+int tiplhip_accumulate_displacement(tiplhip_ctx* ctx, const float* cur, float* new_d,
+                                    const int dim[3], float scale)
+{
+    return guarded([&] {
+        TIPLHIP_REQUIRE(ctx && cur && new_d, "null argument");
+        check_dim(dim);
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        DeviceGuard dg(ctx->device);
+        const FieldGeo g = field_geo(dim);
+        const size_t fb = sizeof(float) * 3 * g.n;
+        DevMem dC(fb), dN(fb);
+        TIPLHIP_CHECK(hipMemcpyAsync(dC.p, cur, fb, hipMemcpyHostToDevice, ctx->stream));
+        TIPLHIP_CHECK(hipMemcpyAsync(dN.p, new_d, fb, hipMemcpyHostToDevice, ctx->stream));
+        launch_accumulate(ctx, dC.as<float>(), dN.as<float>(), scale, g);
+        TIPLHIP_CHECK(hipMemcpyAsync(new_d, dN.p, fb, hipMemcpyDeviceToHost, ctx->stream));
+        TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int tiplhip_cdm_smooth(tiplhip_ctx* ctx, const float* d, float* dd, const int dim[3], float smoothing)
+{
+    return guarded([&] {
+        TIPLHIP_REQUIRE(ctx && d && dd, "null argument");
+        check_dim(dim);
+        if (smoothing == 0.0f) return;  // cdm_smooth leaves dd untouched
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        DeviceGuard dg(ctx->device);
+        const FieldGeo g = field_geo(dim);
+        const size_t fb = sizeof(float) * 3 * g.n;
+        DevMem dD(fb), dO(fb);
+        TIPLHIP_CHECK(hipMemcpyAsync(dD.p, d, fb, hipMemcpyHostToDevice, ctx->stream));
+        launch_smooth(ctx, dD.as<float>(), dO.as<float>(), smoothing, g);
+        TIPLHIP_CHECK(hipMemcpyAsync(dd, dO.p, fb, hipMemcpyDeviceToHost, ctx->stream));
+        TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int tiplhip_upsample_field(tiplhip_ctx* ctx, const float* in, const int in_dim[3], float* out,
+                           const int out_dim[3])
+{
+    return guarded([&] {
+        TIPLHIP_REQUIRE(ctx && in && out, "null argument");
+        check_dim(in_dim);
+        check_dim(out_dim);
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        DeviceGuard dg(ctx->device);
+        const FieldGeo gi = field_geo(in_dim), go = field_geo(out_dim);
+        DevMem dI(sizeof(float) * 3 * gi.n), dO(sizeof(float) * 3 * go.n);
+        TIPLHIP_CHECK(hipMemcpyAsync(dI.p, in, sizeof(float) * 3 * gi.n, hipMemcpyHostToDevice, ctx->stream));
+        launch_upsample(ctx, dI.as<float>(), gi, go, dO.as<float>());
+        TIPLHIP_CHECK(hipMemcpyAsync(out, dO.p, sizeof(float) * 3 * go.n, hipMemcpyDeviceToHost, ctx->stream));
+        TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,109 @@
+// device.hpp -- device-side building blocks shared by the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#pragma clang fp contract(off)
+
+namespace tiplhip {
+
+constexpr int kWave = 64;
+
+// Raw buffer resource over `bytes` bytes: out-of-range loads return 0 and
+// never fault.  Word 3 = 0x00020000 is the gfx9-family (incl. gfx950) raw
+// buffer format.
+__device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void* p, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, int(bytes), 0x00020000);
+}
+
+__device__ inline uint32_t load_u8(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    return __builtin_amdgcn_raw_buffer_load_b8(r, int(voff), int(soff), 0);
+}
+
+// Geometry of a uint8 volume sampled by trilinear interpolation.
+struct VolGeo {
+    int w, h, d;
+    uint32_t wh;       // w*h
+    float wm1, hm1, dm1;  // float(w-1) etc.
+};
+
+// interpolator::linear<3> + weighted_sum + interp_type<uint8>::assign
+// (numerical/interpolation.hpp:25-73, 337-401) for a uint8 volume held in a
+// raw buffer.  The sample is inside iff 0 <= x < w-1 (and the same for y, z):
+// for x >= 0 that is exactly the reference's `size_t(x)+1 < w` test, and a
+// NaN coordinate is outside as on the x86 reference.  All 8 taps share one
+// vector offset; the +1 / +w / +wh neighbours come from scalar and immediate
+// offsets.  Every float operation is in the reference's order and contract
+// is off, so the truncated result is bit-exact.
+__device__ inline uint32_t sample_u8(__amdgpu_buffer_rsrc_t r, const VolGeo& g, float x, float y,
+                                     float z)
+{
+    if (!(x >= 0.0f && y >= 0.0f && z >= 0.0f && x < g.wm1 && y < g.hm1 && z < g.dm1))
+        return 0u;
+    uint32_t ix = uint32_t(x), iy = uint32_t(y), iz = uint32_t(z);
+    float p0 = x - float(ix);
+    float p1 = y - float(iy);
+    float p2 = z - float(iz);
+    float n0 = 1.0f - p0;
+    float n1 = 1.0f - p1;
+    float n2 = 1.0f - p2;
+    uint32_t base = iz * g.wh + iy * uint32_t(g.w) + ix;
+    uint32_t w = uint32_t(g.w);
+    uint32_t v0 = load_u8(r, base, 0);
+    uint32_t v1 = load_u8(r, base + 1, 0);
+    uint32_t v2 = load_u8(r, base, w);
+    uint32_t v3 = load_u8(r, base + 1, w);
+    uint32_t v4 = load_u8(r, base, g.wh);
+    uint32_t v5 = load_u8(r, base + 1, g.wh);
+    uint32_t v6 = load_u8(r, base, g.wh + w);
+    uint32_t v7 = load_u8(r, base + 1, g.wh + w);
+    float n0n1 = n0 * n1;
+    float p0n1 = p0 * n1;
+    float n0p1 = n0 * p1;
+    float p0p1 = p0 * p1;
+    float acc = float(v0) * (n0n1 * n2);
+    acc = acc + float(v1) * (p0n1 * n2);
+    acc = acc + float(v2) * (n0p1 * n2);
+    acc = acc + float(v3) * (p0p1 * n2);
+    acc = acc + float(v4) * (n0n1 * p2);
+    acc = acc + float(v5) * (p0n1 * p2);
+    acc = acc + float(v6) * (n0p1 * p2);
+    acc = acc + float(v7) * (p0p1 * p2);
+    // v <= 0 -> 0, else truncate; the conversion already maps (-inf,1) to 0.
+    return acc <= 0.0f ? 0u : uint32_t(acc);
+}
+
+// Hardware dispatches workgroup b of a 1-D grid to XCD b % 8.  This remap
+// gives each XCD a contiguous range of logical workgroups so that the
+// volume region a range touches stays in that XCD's L2.  Bijective on
+// [0, total).
+__device__ inline uint32_t xcd_remap(uint32_t b, uint32_t total)
+{
+    uint32_t q = total >> 3, r = total & 7u;
+    uint32_t xcd = b & 7u, slot = b >> 3;
+    return xcd * q + (xcd < r ? xcd : r) + slot;
+}
+
+__device__ inline uint64_t wave_sum_u64(uint64_t v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+__device__ inline double wave_sum_f64(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+__device__ inline float wave_max_f32(float v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+
+}  // namespace tiplhip

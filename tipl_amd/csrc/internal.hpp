@@ -1,0 +1,186 @@
+// internal.hpp -- shared host/device definitions of libtiplhip (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/tiplhip.h"
+
+// Every FP32 operation on the bit-exact paths must be a separately rounded
+// IEEE operation in source order (the reference's uint8 truncation after
+// trilinear interpolation depends on it).  The build also passes
+// -ffp-contract=off; the pragma keeps that true for anyone compiling these
+// sources by hand.
+#pragma clang fp contract(off)
+
+namespace tiplhip {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string& msg);
+void clear_last_error();
+
+#define TIPLHIP_CHECK(expr)                                                                      \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            throw ::tiplhip::Error(e_ == hipErrorOutOfMemory ? TIPLHIP_ERR_MEMORY                \
+                                                             : TIPLHIP_ERR_DEVICE,               \
+                                   std::string(#expr) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+
+#define TIPLHIP_REQUIRE(cond, msg)                                                               \
+    do {                                                                                         \
+        if (!(cond)) throw ::tiplhip::Error(TIPLHIP_ERR_INVALID, (msg));                         \
+    } while (0)
+
+// Runs f, converting exceptions into status codes and the thread's last error.
+template <class F>
+int guarded(F&& f)
+{
+    try {
+        clear_last_error();
+        f();
+        return TIPLHIP_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return TIPLHIP_ERR_MEMORY;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return TIPLHIP_ERR_INVALID;
+    } catch (...) {
+        set_last_error("unknown error");
+        return TIPLHIP_ERR_INVALID;
+    }
+}
+
+// Device allocation that only grows.
+struct DeviceBuffer {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    ~DeviceBuffer() { release(); }
+    void reserve(size_t n)
+    {
+        if (n <= bytes) return;
+        release();
+        TIPLHIP_CHECK(hipMalloc(&p, n));
+        bytes = n;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+// Pinned host allocation that only grows.
+struct PinnedBuffer {
+    void* p = nullptr;
+    size_t bytes = 0;
+    PinnedBuffer() = default;
+    PinnedBuffer(const PinnedBuffer&) = delete;
+    PinnedBuffer& operator=(const PinnedBuffer&) = delete;
+    ~PinnedBuffer() { release(); }
+    void reserve(size_t n)
+    {
+        if (n <= bytes) return;
+        release();
+        TIPLHIP_CHECK(hipHostMalloc(&p, n, hipHostMallocDefault));
+        bytes = n;
+    }
+    void release()
+    {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+// Makes ctx's device current for the lifetime of the guard.
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        TIPLHIP_CHECK(hipGetDevice(&prev));
+        if (prev != dev) TIPLHIP_CHECK(hipSetDevice(dev));
+    }
+    ~DeviceGuard()
+    {
+        int cur = -1;
+        if (hipGetDevice(&cur) == hipSuccess && prev >= 0 && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+inline size_t vox(const int* d) { return size_t(d[0]) * size_t(d[1]) * size_t(d[2]); }
+
+// Host restatements of the reference's float geometry (same operation order).
+void affine_matrix(const float* p, const int* fdim, const float* fvs, const int* tdim,
+                   const float* tvs, float* T);
+bool matrix_inverse(float* T);
+bool is_integer_shift(const float* T, int* from_pos);
+
+}  // namespace tiplhip
+
+struct tiplhip_ctx {
+    int device = 0;
+    int cu_count = 256;
+    hipStream_t stream = nullptr;
+    std::recursive_mutex mu;
+    // scratch for the affine batch path
+    tiplhip::DeviceBuffer d_T;       // K x 12 floats
+    tiplhip::DeviceBuffer d_aux;     // per-candidate crop offsets, flags
+    tiplhip::DeviceBuffer d_hist;    // K x bins^2 int32
+    tiplhip::DeviceBuffer d_sums;    // K x 4 uint64
+    tiplhip::DeviceBuffer d_costs;   // K doubles
+    tiplhip::DeviceBuffer d_tmp;     // volume output / misc
+    tiplhip::DeviceBuffer d_tmp2;
+    tiplhip::PinnedBuffer h_T;
+    tiplhip::PinnedBuffer h_out;
+    // timing
+    hipEvent_t t0 = nullptr, t1 = nullptr, k0 = nullptr, k1 = nullptr;
+    bool kernel_timing = false;
+    double kernel_ms = 0.0;
+    int64_t kernel_launches = 0;
+};
+
+struct tiplhip_volume {
+    tiplhip_ctx* ctx = nullptr;
+    int dim[3] = {0, 0, 0};
+    size_t n = 0;
+    uint8_t* d = nullptr;            // device voxels
+    uint32_t* d_hist = nullptr;      // device value histograms: 256 bins then 64 bins
+    int64_t sum = 0, sum_sq = 0;     // exact
+    uint32_t hist256[256];
+    uint32_t hist64[64];
+};
+
+namespace tiplhip {
+
+// Allocates a volume object with device storage for `dim` voxels.
+tiplhip_volume* volume_alloc(tiplhip_ctx* ctx, const int* dim);
+// downsample_with_padding of a device uint8 volume into `out` ((dim+1)/2).
+void launch_downsample_u8(tiplhip_ctx* ctx, const uint8_t* in, const int* dim, uint8_t* out);
+// Computes sum / sum_sq / histograms of vol->d on the device and stores them.
+void volume_compute_stats(tiplhip_ctx* ctx, tiplhip_volume* vol);
+
+}  // namespace tiplhip
