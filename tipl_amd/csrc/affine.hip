@@ -26,12 +26,12 @@ namespace {
 
 struct BatchGeo {
     int fw, fh, fd;             // from grid
-    uint32_t nseg;              // segments per row: ceil(fw / 64)
-    uint32_t items;             // fh * fd * nseg
+    uint32_t ntx, nty;          // 16 x 4 wave tiles per row / per plane
+    uint32_t items;             // ntx * nty * fd
     uint32_t items_per_block;
     uint32_t K;
     uint32_t total_blocks;      // K * ceil(items / items_per_block)
-    uint32_t from_bytes, to_bytes;
+    uint32_t from_bytes, to_bytes8;
     VolGeo to;
 };
 
@@ -48,6 +48,8 @@ VolGeo make_geo(const int* dim)
     return g;
 }
 
+constexpr uint32_t kTileX = 16, kTileY = 4;  // one wave = 16 x 4 voxels of one plane
+
 BatchGeo make_batch(const tiplhip_volume* from, const tiplhip_volume* to, int K,
                     uint32_t max_items_per_block, uint32_t target_blocks)
 {
@@ -55,8 +57,9 @@ BatchGeo make_batch(const tiplhip_volume* from, const tiplhip_volume* to, int K,
     b.fw = from->dim[0];
     b.fh = from->dim[1];
     b.fd = from->dim[2];
-    b.nseg = uint32_t((b.fw + kWave - 1) / kWave);
-    b.items = b.nseg * uint32_t(b.fh) * uint32_t(b.fd);
+    b.ntx = (uint32_t(b.fw) + kTileX - 1) / kTileX;
+    b.nty = (uint32_t(b.fh) + kTileY - 1) / kTileY;
+    b.items = b.ntx * b.nty * uint32_t(b.fd);
     uint64_t want = (uint64_t(b.items) * uint64_t(K) + target_blocks - 1) / target_blocks;
     uint32_t ipb = uint32_t(std::min<uint64_t>(want, max_items_per_block));
     ipb = std::max<uint32_t>(ipb, 16u);
@@ -66,9 +69,27 @@ BatchGeo make_batch(const tiplhip_volume* from, const tiplhip_volume* to, int K,
     uint32_t chunks = (b.items + ipb - 1) / ipb;
     b.total_blocks = chunks * uint32_t(K);
     b.from_bytes = uint32_t(from->n);
-    b.to_bytes = uint32_t(to->n);
+    b.to_bytes8 = uint32_t(to->n * 8u);
     b.to = make_geo(to->dim);
     return b;
+}
+
+// Wave-uniform decode of a tile item into (tile x, tile y, z).
+struct Item {
+    uint32_t x, y, z;   // this lane's voxel
+    bool ok;
+};
+__device__ inline Item decode(const BatchGeo& g, uint32_t it, uint32_t lane)
+{
+    const uint32_t tx = it % g.ntx;
+    const uint32_t r = it / g.ntx;
+    const uint32_t ty = r % g.nty;
+    Item m;
+    m.z = r / g.nty;
+    m.x = tx * kTileX + (lane & (kTileX - 1));
+    m.y = ty * kTileY + (lane / kTileX);
+    m.ok = m.x < uint32_t(g.fw) && m.y < uint32_t(g.fh);
+    return m;
 }
 
 // Position of voxel (x, y, z) of the from grid in the to volume:
@@ -94,6 +115,7 @@ __device__ inline Pos apply(const float* T, float fx, float fy, float fz)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void corr_sums_kernel(const uint8_t* __restrict__ from,
                                                         const uint8_t* __restrict__ to,
+                                                        const uint64_t* __restrict__ to8,
                                                         const float* __restrict__ Ts,
                                                         const int4* __restrict__ crop, BatchGeo g,
                                                         unsigned long long* __restrict__ sums)
@@ -101,35 +123,35 @@ __global__ __launch_bounds__(256) void corr_sums_kernel(const uint8_t* __restric
     const uint32_t L = xcd_remap(blockIdx.x, g.total_blocks);
     const uint32_t cand = L % g.K;
     const uint32_t chunk = L / g.K;
-    const float* T = Ts + 12 * cand;
+    float T[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) T[i] = Ts[12 * cand + i];
     const int4 cr = crop[cand];
     const __amdgpu_buffer_rsrc_t rf = make_rsrc(from, g.from_bytes);
-    const __amdgpu_buffer_rsrc_t rt = make_rsrc(to, g.to_bytes);
+    const __amdgpu_buffer_rsrc_t rt = make_rsrc(to, g.to_bytes8 / 8u);
+    const __amdgpu_buffer_rsrc_t r8 = make_rsrc(to8, g.to_bytes8);
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t nwaves = blockDim.x >> 6;
     const uint32_t it0 = chunk * g.items_per_block;
     const uint32_t it1 = min(it0 + g.items_per_block, g.items);
+    const uint32_t fwh = uint32_t(g.fw) * uint32_t(g.fh);
     uint32_t sy = 0, syy = 0, sxy = 0;
     for (uint32_t it = it0 + wave; it < it1; it += nwaves) {
-        const uint32_t row = it / g.nseg;
-        const uint32_t seg = it - row * g.nseg;
-        const uint32_t y = row % uint32_t(g.fh);
-        const uint32_t z = row / uint32_t(g.fh);
-        const uint32_t x = seg * 64u + lane;
-        if (x < uint32_t(g.fw)) {
-            const uint32_t xv = load_u8(rf, row * uint32_t(g.fw) + x, 0);
+        const Item m = decode(g, it, lane);
+        if (m.ok) {
+            const uint32_t xv = load_u8(rf, m.z * fwh + m.y * uint32_t(g.fw) + m.x, 0);
             uint32_t yv;
             if (cr.x) {
-                int sx = int(x) + cr.y, s_y = int(y) + cr.z, sz = int(z) + cr.w;
+                int sx = int(m.x) + cr.y, s_y = int(m.y) + cr.z, sz = int(m.z) + cr.w;
                 bool in = sx >= 0 && sx < g.to.w && s_y >= 0 && s_y < g.to.h && sz >= 0 &&
                           sz < g.to.d;
                 yv = in ? load_u8(rt, uint32_t(sz) * g.to.wh + uint32_t(s_y) * uint32_t(g.to.w) +
                                           uint32_t(sx), 0)
                         : 0u;
             } else {
-                Pos p = apply(T, float(x), float(y), float(z));
-                yv = sample_u8(rt, g.to, p.x, p.y, p.z);
+                Pos p = apply(T, float(m.x), float(m.y), float(m.z));
+                yv = sample_u8_packed(r8, g.to, p.x, p.y, p.z);
             }
             sy += yv;
             syy += yv * yv;
@@ -218,8 +240,8 @@ struct MiTraits<64> {
 
 template <int BINS>
 __global__ __launch_bounds__(MiTraits<BINS>::kThreads) void mi_hist_kernel(
-    const uint8_t* __restrict__ from, const uint8_t* __restrict__ to, const float* __restrict__ Ts,
-    BatchGeo g, int* __restrict__ hist)
+    const uint8_t* __restrict__ from, const uint64_t* __restrict__ to8,
+    const float* __restrict__ Ts, BatchGeo g, int* __restrict__ hist)
 {
     using Tr = MiTraits<BINS>;
     __shared__ uint32_t lds[Tr::kWords];
@@ -229,32 +251,31 @@ __global__ __launch_bounds__(MiTraits<BINS>::kThreads) void mi_hist_kernel(
     const uint32_t L = xcd_remap(blockIdx.x, g.total_blocks);
     const uint32_t cand = L % g.K;
     const uint32_t chunk = L / g.K;
-    const float* T = Ts + 12 * cand;
+    float T[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) T[i] = Ts[12 * cand + i];
     const __amdgpu_buffer_rsrc_t rf = make_rsrc(from, g.from_bytes);
-    const __amdgpu_buffer_rsrc_t rt = make_rsrc(to, g.to_bytes);
+    const __amdgpu_buffer_rsrc_t r8 = make_rsrc(to8, g.to_bytes8);
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t nwaves = blockDim.x >> 6;
     const uint32_t it0 = chunk * g.items_per_block;
     const uint32_t it1 = min(it0 + g.items_per_block, g.items);
+    const uint32_t fwh = uint32_t(g.fw) * uint32_t(g.fh);
     for (uint32_t it = it0 + wave; it < it1; it += nwaves) {
-        const uint32_t row = it / g.nseg;
-        const uint32_t seg = it - row * g.nseg;
-        const uint32_t y = row % uint32_t(g.fh);
-        const uint32_t z = row / uint32_t(g.fh);
-        const uint32_t x = seg * 64u + lane;
-        if (x < uint32_t(g.fw)) {
-            const uint32_t fv = load_u8(rf, row * uint32_t(g.fw) + x, 0);
-            Pos p = apply(T, float(x), float(y), float(z));
-            const uint32_t tv = sample_u8(rt, g.to, p.x, p.y, p.z);
+        const Item m = decode(g, it, lane);
+        if (m.ok) {
+            const uint32_t fv = load_u8(rf, m.z * fwh + m.y * uint32_t(g.fw) + m.x, 0);
+            Pos p = apply(T, float(m.x), float(m.y), float(m.z));
+            const uint32_t tv = sample_u8_packed(r8, g.to, p.x, p.y, p.z);
             const uint32_t bin = ((fv >> Tr::kShift) << Tr::kBand) | (tv >> Tr::kShift);
             const uint32_t b0 = __builtin_amdgcn_readfirstlane(bin);
             const bool same = bin == b0;
-            const unsigned long long m = __ballot(same);
+            const unsigned long long mk = __ballot(same);
             if (same) {
-                const uint32_t leader = uint32_t(__ffsll((long long)m) - 1);
+                const uint32_t leader = uint32_t(__ffsll((long long)mk) - 1);
                 if (lane == leader)
-                    atomicAdd(&lds[Tr::word(b0)], Tr::inc(b0) * uint32_t(__popcll(m)));
+                    atomicAdd(&lds[Tr::word(b0)], Tr::inc(b0) * uint32_t(__popcll(mk)));
             } else {
                 atomicAdd(&lds[Tr::word(bin)], Tr::inc(bin));
             }
@@ -369,7 +390,9 @@ void check_volume(const tiplhip_ctx* ctx, const tiplhip_volume* v, const char* w
     TIPLHIP_REQUIRE(v != nullptr, std::string(what) + " is null");
     TIPLHIP_REQUIRE(v->ctx == ctx, std::string(what) + " belongs to another context");
     TIPLHIP_REQUIRE(v->n > 0, std::string(what) + " is empty");
-    TIPLHIP_REQUIRE(v->n < (size_t(1) << 31), std::string(what) + " exceeds 2^31 voxels");
+    TIPLHIP_REQUIRE(v->n < (size_t(1) << 29), std::string(what) + " exceeds 2^29 voxels");
+    TIPLHIP_REQUIRE(size_t(v->dim[0]) * size_t(v->dim[1]) < (size_t(1) << 24),
+                    std::string(what) + " plane exceeds 2^24 voxels");
 }
 
 void timing_begin(tiplhip_ctx* ctx)
@@ -428,12 +451,13 @@ void upload_transforms(tiplhip_ctx* ctx, const float* T, int K, const int4* crop
 // leaves Sy, Syy, Sxy in ctx->d_sums (K x 4 u64).
 void launch_corr_sums(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_volume* to, int K)
 {
+    volume_ensure_packed(ctx, const_cast<tiplhip_volume*>(to));
     ctx->d_sums.reserve(sizeof(unsigned long long) * 4 * size_t(K));
     zero_u64(ctx, ctx->d_sums.p, 4 * size_t(K));
     BatchGeo g = make_batch(from, to, K, 4096u, uint32_t(ctx->cu_count) * 8u);
     timing_begin(ctx);
     hipLaunchKernelGGL(corr_sums_kernel, dim3(g.total_blocks), dim3(256), 0, ctx->stream, from->d,
-                       to->d, ctx->d_T.as<float>(), ctx->d_aux.as<int4>(), g,
+                       to->d, to->d_pack, ctx->d_T.as<float>(), ctx->d_aux.as<int4>(), g,
                        ctx->d_sums.as<unsigned long long>());
     TIPLHIP_CHECK(hipGetLastError());
     timing_end(ctx);
@@ -443,6 +467,7 @@ void launch_corr_sums(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhi
 void launch_mi_hist(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_volume* to, int K,
                     int bins)
 {
+    volume_ensure_packed(ctx, const_cast<tiplhip_volume*>(to));
     size_t hb = sizeof(int) * size_t(bins) * bins * size_t(K);
     ctx->d_hist.reserve(hb);
     zero_bytes16(ctx, ctx->d_hist.p, hb);
@@ -451,11 +476,11 @@ void launch_mi_hist(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_
         // 1023 segments of <= 64 voxels keep every uint16 LDS counter below 65536.
         BatchGeo g = make_batch(from, to, K, 1023u, uint32_t(ctx->cu_count) * 4u);
         hipLaunchKernelGGL(mi_hist_kernel<256>, dim3(g.total_blocks), dim3(1024), 0, ctx->stream,
-                           from->d, to->d, ctx->d_T.as<float>(), g, ctx->d_hist.as<int>());
+                           from->d, to->d_pack, ctx->d_T.as<float>(), g, ctx->d_hist.as<int>());
     } else {
         BatchGeo g = make_batch(from, to, K, 4096u, uint32_t(ctx->cu_count) * 8u);
         hipLaunchKernelGGL(mi_hist_kernel<64>, dim3(g.total_blocks), dim3(256), 0, ctx->stream,
-                           from->d, to->d, ctx->d_T.as<float>(), g, ctx->d_hist.as<int>());
+                           from->d, to->d_pack, ctx->d_T.as<float>(), g, ctx->d_hist.as<int>());
     }
     TIPLHIP_CHECK(hipGetLastError());
     timing_end(ctx);

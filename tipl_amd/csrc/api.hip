@@ -228,6 +228,29 @@ __global__ __launch_bounds__(256) void downsample_u8_kernel(const uint8_t* __res
     }
 }
 
+// 8-tap packing of a uint8 volume: word v = bytes at v, v+1, v+w, v+w+1,
+// v+wh, v+wh+1, v+wh+w, v+wh+w+1 (0 past the end of the array).  Only cells
+// whose far corner is inside the volume are ever sampled, and for those the
+// word holds exactly the 8 trilinear taps.
+__global__ __launch_bounds__(256) void pack8_kernel(const uint8_t* __restrict__ v, size_t n,
+                                                    uint32_t w, uint32_t wh,
+                                                    uint64_t* __restrict__ out)
+{
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const size_t off[8] = {0, 1, w, size_t(w) + 1, wh, size_t(wh) + 1, size_t(wh) + w,
+                               size_t(wh) + w + 1};
+        uint64_t word = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const size_t j = i + off[k];
+            const uint64_t b = j < n ? uint64_t(v[j]) : 0ull;
+            word |= b << (8 * k);
+        }
+        out[i] = word;
+    }
+}
+
 int grid_for(size_t n, int cap = 16384)
 {
     return int(std::max<size_t>(1, std::min<size_t>((n + 255) / 256, size_t(cap))));
@@ -261,6 +284,23 @@ tiplhip_volume* volume_alloc(tiplhip_ctx* ctx, const int* dim)
                     std::string("volume allocation: ") + hipGetErrorString(e));
     }
     return v;
+}
+
+void volume_ensure_packed(tiplhip_ctx* ctx, tiplhip_volume* vol)
+{
+    if (vol->d_pack) return;
+    TIPLHIP_REQUIRE(vol->n * 8u < (size_t(1) << 32), "volume too large for the packed sampler");
+    void* p = nullptr;
+    TIPLHIP_CHECK(hipMalloc(&p, vol->n * 8u));
+    hipLaunchKernelGGL(pack8_kernel, dim3(grid_for(vol->n)), dim3(256), 0, ctx->stream, vol->d,
+                       vol->n, uint32_t(vol->dim[0]), uint32_t(vol->dim[0]) * uint32_t(vol->dim[1]),
+                       static_cast<uint64_t*>(p));
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        (void)hipFree(p);
+        TIPLHIP_CHECK(e);
+    }
+    vol->d_pack = static_cast<uint64_t*>(p);
 }
 
 void volume_compute_stats(tiplhip_ctx* ctx, tiplhip_volume* vol)
@@ -522,6 +562,7 @@ int tiplhip_volume_destroy(tiplhip_volume* vol)
         TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
         (void)hipFree(vol->d);
         (void)hipFree(vol->d_hist);
+        if (vol->d_pack) (void)hipFree(vol->d_pack);
         delete vol;
     });
 }

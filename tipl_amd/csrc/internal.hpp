@@ -169,6 +169,10 @@ struct tiplhip_volume {
     size_t n = 0;
     uint8_t* d = nullptr;            // device voxels
     uint32_t* d_hist = nullptr;      // device value histograms: 256 bins then 64 bins
+    // 8-tap packed copy (built on first use as a resampling source): word v
+    // holds the bytes at v, v+1, v+w, v+w+1, v+wh, v+wh+1, v+wh+w, v+wh+w+1
+    // (0 past the array end), so one 8-byte gather fetches a trilinear cell.
+    uint64_t* d_pack = nullptr;
     int64_t sum = 0, sum_sq = 0;     // exact
     uint32_t hist256[256];
     uint32_t hist64[64];
@@ -180,6 +184,8 @@ namespace tiplhip {
 tiplhip_volume* volume_alloc(tiplhip_ctx* ctx, const int* dim);
 // downsample_with_padding of a device uint8 volume into `out` ((dim+1)/2).
 void launch_downsample_u8(tiplhip_ctx* ctx, const uint8_t* in, const int* dim, uint8_t* out);
+// Builds vol->d_pack if it does not exist yet.
+void volume_ensure_packed(tiplhip_ctx* ctx, tiplhip_volume* vol);
 // Computes sum / sum_sq / histograms of vol->d on the device and stores them.
 void volume_compute_stats(tiplhip_ctx* ctx, tiplhip_volume* vol);
 
