@@ -1,0 +1,126 @@
+"""Pins the oracle (oracle/oracle.c) to the reference: every restated
+function against the reference headers compiled by oracle/Makefile into
+oracle/_ref/libtiplref.so, on seeded inputs, bit for bit.  Skipped where the
+reference build is not present (its results are also frozen in
+tests/golden/, see test_oracle_golden.py)."""
+import numpy as np
+import pytest
+
+from conftest import random_params, smooth_volume
+
+
+def _T(oracle, p, fshape, tshape, fvs=(1, 1, 1), tvs=(1, 1, 1)):
+    return oracle.affine_matrix(p, np.array(fshape[::-1], np.int32), fvs,
+                                np.array(tshape[::-1], np.int32), tvs)
+
+
+def _bits(a):
+    a = np.asarray(a)
+    return a.view(np.uint32) if a.dtype == np.float32 else a
+
+
+def test_affine_matrix_and_inverse(oracle, ref):
+    rng = np.random.default_rng(0)
+    for _ in range(100):
+        p = random_params(rng, 1, t=20, r=1.2, s=0.4, a=0.2)[0]
+        fd = rng.integers(5, 300, 3).astype(np.int32)
+        td = rng.integers(5, 300, 3).astype(np.int32)
+        fvs = rng.uniform(0.5, 3, 3).astype(np.float32)
+        tvs = rng.choice([1.0, 0.7, 2.0], 3).astype(np.float32)
+        a = oracle.affine_matrix(p, fd, fvs, td, tvs)
+        b = ref.affine_matrix(p, fd, fvs, td, tvs)
+        assert np.array_equal(_bits(a), _bits(b))
+        ia, oka = oracle.matrix_inverse(a)
+        ib, okb = ref.matrix_inverse(a)
+        assert oka == okb and np.array_equal(_bits(ia), _bits(ib))
+
+
+SHAPES = [((14, 18, 22), (14, 18, 22)), ((11, 19, 15), (13, 17, 21)), ((20, 20, 20), (12, 12, 12))]
+
+
+@pytest.mark.parametrize("fshape,tshape", SHAPES)
+def test_resample_and_costs(oracle, ref, fshape, tshape):
+    rng = np.random.default_rng(1)
+    A, B = smooth_volume(fshape, 2), smooth_volume(tshape, 3)
+    P = random_params(rng, 12)
+    Ts = [_T(oracle, p, fshape, tshape) for p in P]
+    Ts.append(np.array([1, 0, 0, 0, 1, 0, 0, 0, 1, 1, -2, 0], np.float32))   # crop shortcut
+    Ts.append(np.array([1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0], np.float32))
+    for T in Ts:
+        assert np.array_equal(oracle.resample(B, fshape, T), ref.resample(B, fshape, T))
+        assert oracle.corr_cost(A, B, T) == ref.corr_cost(A, B, T)
+        assert np.array_equal(oracle.mi_hist(A, B, T), ref.mi_hist(A, B, T))
+        assert oracle.mi_cost(A, B, T) == ref.mi_cost(A, B, T)
+
+
+def test_sample_edges(oracle, ref):
+    v = smooth_volume((6, 7, 8), 4, background=False)
+    pts = [(0, 0, 0), (6.999, 5.5, 4.5), (7.0, 1, 1), (-0.0, 1, 1), (-1e-7, 1, 1), (3.5, 5.999, 4.999),
+           (3.5, 6.0, 1), (float("nan"), 1, 1), (1e30, 1, 1), (2.25, 3.75, 1.5)]
+    for p in pts:
+        assert oracle.sample(v, *p) == ref.sample(v, *p), p
+
+
+def test_quantise(oracle, ref):
+    rng = np.random.default_rng(5)
+    v = rng.normal(0, 300, (9, 10, 11)).astype(np.float32)
+    assert np.array_equal(oracle.normalize(v), ref.normalize(v))
+    assert np.array_equal(oracle.normalize(v, 255.99), ref.normalize(v, 255.99))
+
+
+@pytest.mark.parametrize("shape", [(9, 10, 11), (16, 16, 16), (1, 3, 5), (2, 2, 2)])
+def test_pyramid(oracle, ref, shape):
+    v = np.random.default_rng(6).integers(0, 256, shape).astype(np.uint8)
+    assert np.array_equal(oracle.downsample(v), ref.downsample(v))
+    d = np.random.default_rng(7).standard_normal(shape + (3,)).astype(np.float32)
+    out = tuple(2 * s for s in shape)
+    assert np.array_equal(_bits(oracle.upsample(d, out)), _bits(ref.upsample(d, out)))
+
+
+def _field(shape, seed, scale=2.0, zero_frac=0.2):
+    from scipy.ndimage import gaussian_filter
+    r = np.random.default_rng(seed)
+    d = gaussian_filter(r.standard_normal(shape + (3,)), (2, 2, 2, 0)).astype(np.float32) * scale
+    d[r.random(shape) < zero_frac] = 0.0
+    return d
+
+
+@pytest.mark.parametrize("shape", [(12, 14, 16), (7, 11, 5)])
+def test_cdm_steps(oracle, ref, shape):
+    Is = smooth_volume(shape, 8, background=False)
+    It = smooth_volume(shape, 9)
+    d = _field(shape, 10, 3.0)
+    Js = oracle.compose(Is, d)
+    assert np.array_equal(Js, ref.compose(Is, d))
+    for kind in "rd":
+        a, ca = oracle.gradient(Js, It, kind)
+        b, cb = ref.gradient(Js, It, kind)
+        assert np.array_equal(_bits(a), _bits(b))
+        assert abs(ca - cb) <= 1e-6 * max(abs(cb), 1e-30)
+    nd = _field(shape, 11, 1.0, 0.0)
+    p = oracle.poisson(nd)
+    assert np.array_equal(_bits(p), _bits(ref.poisson(nd)))
+    assert oracle.max_length(p) == ref.max_length(p)
+    assert np.array_equal(_bits(oracle.accumulate(d, p)), _bits(ref.accumulate(d, p)))
+    for s in (0.8, 0.1, 0.0):
+        assert np.array_equal(_bits(oracle.smooth(p, d, s)), _bits(ref.smooth(p, d, s)))
+
+
+def test_cdm_improved(oracle, ref):
+    rng = np.random.default_rng(12)
+    for n in range(0, 10):
+        for _ in range(20):
+            c = list(np.cumsum(rng.normal(-0.01, 0.02, n)).astype(np.float32))
+            assert oracle.improved(c) == ref.improved(c)
+
+
+@pytest.mark.parametrize("shape,kind", [((36, 40, 32), "r"), ((33, 29, 35), "d"), ((20, 20, 20), "r")])
+def test_whole_cdm(oracle, ref, shape, kind):
+    from scipy.ndimage import shift
+    It = smooth_volume(shape, 13, sigma=2.5)
+    Is = shift(It.astype(np.float32), (1.0, -1.5, 2.0), order=1).astype(np.uint8)
+    ok_o, f_o, _ = oracle.cdm(It, Is, gradient_type=kind)
+    ok_r, f_r = ref.cdm(It, Is, gradient_type=kind)
+    assert ok_o == ok_r
+    rms = float(np.sqrt(((f_o - f_r) ** 2).sum(-1).mean()))
+    assert rms <= 1e-6, rms

@@ -240,18 +240,209 @@ __global__ __launch_bounds__(256) void gradient_kernel(const uint8_t* __restrict
     }
 }
 
+// ---------------------------------------------------------------------------
+// cdm_get_gradient 'r' with the window sums done as box filters.  A block
+// owns a 64 x 4 column of outputs over kZC planes and marches z: for every
+// plane it stages Js and It (with a 2-voxel halo, zero outside the volume)
+// in LDS, forms 5-tap x sums, then 5-tap y sums, and keeps the last five
+// plane sums in registers for the z sum.  Zero padding leaves the five
+// integer sums of the border-clipped window unchanged; the window size is
+// the product of the clipped extents, as get_window_at_width counts it.
+// ---------------------------------------------------------------------------
+constexpr int kZC = 16;
+constexpr int kPX = kTX + 4, kPY = kTY + 4;
+
+struct Sums5 {
+    uint32_t j, i, jj, ii, ji;
+};
+
+__device__ inline void regress_and_store(const uint8_t* __restrict__ Js, const FieldGeo& g,
+                                         const Vox& v, uint32_t js, uint32_t it, const Sums5& s,
+                                         int accumulate, float* __restrict__ nd, float& cost)
+{
+    float3 gr = make_float3(0.0f, 0.0f, 0.0f);
+    if (js != 0u && it != 0u) {
+        const int nx = min(v.x + 2, g.w - 1) - max(v.x - 2, 0) + 1;
+        const int ny = min(v.y + 2, g.h - 1) - max(v.y - 2, 0) + 1;
+        const int nz = min(v.z + 2, g.d - 1) - max(v.z - 2, 0) + 1;
+        const double dn = double(nx * ny * nz);
+        const double mean_x = double(s.j) / dn;
+        const double mean_y = double(s.i) / dn;
+        const double x_var = double(s.jj) / dn - mean_x * mean_x;
+        const double y_var = double(s.ii) / dn - mean_y * mean_y;
+        if (!(x_var == 0.0 || y_var == 0.0)) {
+            const double cov = double(s.ji) / dn - mean_x * mean_y;
+            const float a = float(cov / x_var);
+            if (a > 0.0f) {
+                const float b = float(mean_y - double(a) * mean_x);
+                const float r2 = float(cov * cov / x_var / y_var);
+                float sc = float(js) * a;
+                sc = sc + b;
+                sc = sc - float(it);
+                sc = r2 * sc;
+                cost = -r2;
+                const uint32_t w = uint32_t(g.w);
+                const float gx =
+                    (v.x + 1 < g.w && v.x) ? float(int(Js[v.p + 1]) - int(Js[v.p - 1])) : 0.0f;
+                const float gy =
+                    (v.y + 1 < g.h && v.y) ? float(int(Js[v.p + w]) - int(Js[v.p - w])) : 0.0f;
+                const float gz =
+                    (v.z + 1 < g.d && v.z) ? float(int(Js[v.p + g.wh]) - int(Js[v.p - g.wh])) : 0.0f;
+                gr.x = 0.0f + gx * sc;
+                gr.y = 0.0f + gy * sc;
+                gr.z = 0.0f + gz * sc;
+            }
+        }
+    }
+    if (accumulate) {
+        float3 o = ld3(nd, v.p);
+        o.x = o.x + gr.x;
+        o.y = o.y + gr.y;
+        o.z = o.z + gr.z;
+        st3(nd, v.p, o);
+    } else {
+        st3(nd, v.p, gr);
+    }
+}
+
+__global__ __launch_bounds__(256) void gradient_r_box_kernel(const uint8_t* __restrict__ Js,
+                                                             const uint8_t* __restrict__ It,
+                                                             FieldGeo g, int accumulate,
+                                                             float* __restrict__ nd,
+                                                             double* __restrict__ partials)
+{
+    __shared__ uint8_t pj[kPY][kPX], pi[kPY][kPX];
+    __shared__ Sums5 xs[kPY][kTX];
+    const int tx = int(threadIdx.x), ty = int(threadIdx.y);
+    const int tid = ty * kTX + tx;
+    const int x0 = int(blockIdx.x) * kTX, y0 = int(blockIdx.y) * kTY;
+    const int z0 = int(blockIdx.z) * kZC, z1 = min(z0 + kZC, g.d);
+    Sums5 ring[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) ring[k] = Sums5{0u, 0u, 0u, 0u, 0u};
+    float cost_acc = 0.0f;
+    double cost_sum = 0.0;
+    for (int pz = z0 - 2; pz < z1 + 2; ++pz) {
+        Sums5 P{0u, 0u, 0u, 0u, 0u};
+        if (pz >= 0 && pz < g.d) {
+            // stage the plane with a 2-voxel halo (zero outside the volume)
+            for (int c = tid; c < kPX * kPY; c += kTX * kTY) {
+                const int cy = c / kPX, cx = c - cy * kPX;
+                const int gx = x0 - 2 + cx, gy = y0 - 2 + cy;
+                uint8_t a = 0, b = 0;
+                if (gx >= 0 && gx < g.w && gy >= 0 && gy < g.h) {
+                    const uint32_t q = uint32_t(pz) * g.wh + uint32_t(gy) * uint32_t(g.w) + uint32_t(gx);
+                    a = Js[q];
+                    b = It[q];
+                }
+                pj[cy][cx] = a;
+                pi[cy][cx] = b;
+            }
+            __syncthreads();
+            // x sums for the 8 staged rows (2 per thread)
+            for (int r = ty; r < kPY; r += kTY) {
+                Sums5 s{0u, 0u, 0u, 0u, 0u};
+#pragma unroll
+                for (int k = 0; k < 5; ++k) {
+                    const uint32_t a = pj[r][tx + k], b = pi[r][tx + k];
+                    s.j += a;
+                    s.i += b;
+                    s.jj += a * a;
+                    s.ii += b * b;
+                    s.ji += a * b;
+                }
+                xs[r][tx] = s;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                const Sums5 s = xs[ty + k][tx];
+                P.j += s.j;
+                P.i += s.i;
+                P.jj += s.jj;
+                P.ii += s.ii;
+                P.ji += s.ji;
+            }
+            __syncthreads();  // pj/pi/xs are rewritten for the next plane
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) ring[k] = ring[k + 1];
+        ring[4] = P;
+        const int z = pz - 2;
+        if (z >= z0) {
+            Sums5 S = ring[0];
+#pragma unroll
+            for (int k = 1; k < 5; ++k) {
+                S.j += ring[k].j;
+                S.i += ring[k].i;
+                S.jj += ring[k].jj;
+                S.ii += ring[k].ii;
+                S.ji += ring[k].ji;
+            }
+            Vox v;
+            v.x = x0 + tx;
+            v.y = y0 + ty;
+            v.z = z;
+            v.ok = v.x < g.w && v.y < g.h;
+            v.p = uint32_t(v.z) * g.wh + uint32_t(v.y) * uint32_t(g.w) + uint32_t(v.x);
+            if (v.ok) {
+                float cost = 0.0f;
+                regress_and_store(Js, g, v, Js[v.p], It[v.p], S, accumulate, nd, cost);
+                cost_sum += double(cost);
+            }
+        }
+    }
+    (void)cost_acc;
+    __shared__ double part[4];
+    double c = wave_sum_f64(cost_sum);
+    if ((tid & 63) == 0) part[tid >> 6] = c;
+    __syncthreads();
+    if (tid == 0) {
+        const uint32_t b = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        partials[b] = (part[0] + part[1]) + (part[2] + part[3]);
+    }
+}
+
+dim3 box_grid(const FieldGeo& g)
+{
+    return dim3(uint32_t((g.w + kTX - 1) / kTX), uint32_t((g.h + kTY - 1) / kTY),
+                uint32_t((g.d + kZC - 1) / kZC));
+}
+
 // Sums the block partials in a fixed order and stores float(sum / n).
-__global__ __launch_bounds__(256) void cost_reduce_kernel(const double* __restrict__ partials,
-                                                          uint32_t count, double n,
-                                                          float* __restrict__ out)
+__global__ __launch_bounds__(1024) void cost_reduce_kernel(const double* __restrict__ partials,
+                                                           uint32_t count, double n,
+                                                           float* __restrict__ out)
 {
     double acc = 0.0;
     for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) acc += partials[i];
-    __shared__ double part[4];
+    __shared__ double part[16];
     acc = wave_sum_f64(acc);
     if ((threadIdx.x & 63u) == 0) part[threadIdx.x >> 6] = acc;
     __syncthreads();
-    if (threadIdx.x == 0) out[0] = float(((part[0] + part[1]) + (part[2] + part[3])) / n);
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (uint32_t w = 0; w < (blockDim.x >> 6); ++w) s += part[w];
+        out[0] = float(s / n);
+    }
+}
+
+// Max of the per-block maxima (non-negative floats), as float bits.
+__global__ __launch_bounds__(1024) void max_reduce_kernel(const float* __restrict__ parts,
+                                                          uint32_t count,
+                                                          unsigned int* __restrict__ out)
+{
+    float m = 0.0f;
+    for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) m = fmaxf(m, parts[i]);
+    __shared__ float part[16];
+    m = wave_max_f32(m);
+    if ((threadIdx.x & 63u) == 0) part[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float r = 0.0f;
+        for (uint32_t w = 0; w < (blockDim.x >> 6); ++w) r = fmaxf(r, part[w]);
+        out[0] = __float_as_uint(r);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -266,7 +457,7 @@ template <bool FIRST, bool MAXLEN>
 __global__ __launch_bounds__(256) void poisson_kernel(const float* __restrict__ nd,
                                                       const float* __restrict__ a, FieldGeo g,
                                                       float* __restrict__ b,
-                                                      unsigned int* __restrict__ maxlen)
+                                                      float* __restrict__ maxparts)
 {
     const Vox v = this_voxel(g);
     float len = 0.0f;
@@ -308,8 +499,100 @@ __global__ __launch_bounds__(256) void poisson_kernel(const float* __restrict__ 
         if ((tid & 63u) == 0) part[tid >> 6] = m;
         __syncthreads();
         if (tid == 0) {
-            m = fmaxf(fmaxf(part[0], part[1]), fmaxf(part[2], part[3]));
-            atomicMax(maxlen, __float_as_uint(m));  // lengths are >= 0: bit order = value order
+            const uint32_t blk = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+            maxparts[blk] = fmaxf(fmaxf(part[0], part[1]), fmaxf(part[2], part[3]));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The same Jacobi sweep, marching z.  A block owns a 64 x 8 column of voxels
+// over kPZ planes; each thread keeps the previous iterate of its own column
+// at z-1, z, z+1 in registers (the +-wh neighbours), takes x-1 / x+1 from
+// the neighbouring lanes of its wave, y-1 / y+1 from the block's LDS copy of
+// the plane, and loads the neighbours that fall outside the tile by flat
+// index -- which is exactly the reference's flat-index neighbourhood, row and
+// plane wrap included.  Every value is combined in the reference's order.
+// ---------------------------------------------------------------------------
+constexpr int kPTY = 8, kPZ = 16;
+
+dim3 poisson_grid(const FieldGeo& g)
+{
+    return dim3(uint32_t((g.w + kTX - 1) / kTX), uint32_t((g.h + kPTY - 1) / kPTY),
+                uint32_t((g.d + kPZ - 1) / kPZ));
+}
+
+template <bool FIRST, bool MAXLEN>
+__global__ __launch_bounds__(kTX * kPTY) void poisson_zm_kernel(const float* __restrict__ nd,
+                                                                const float* __restrict__ a,
+                                                                FieldGeo g, float* __restrict__ b,
+                                                                float* __restrict__ maxparts)
+{
+    __shared__ float3 plane[kPTY][kTX];
+    const int tx = int(threadIdx.x), ty = int(threadIdx.y);
+    const int x = int(blockIdx.x) * kTX + tx, y = int(blockIdx.y) * kPTY + ty;
+    const int z0 = int(blockIdx.z) * kPZ, z1 = min(z0 + kPZ, g.d);
+    const bool ok = x < g.w && y < g.h;
+    const uint32_t n = g.n, w = uint32_t(g.w), wh = g.wh;
+    auto A = [&](uint32_t q) -> float3 {
+        if (FIRST) {
+            const float3 t = ld3(nd, q);
+            return make_float3(t.x * kInvD2, t.y * kInvD2, t.z * kInvD2);
+        }
+        return ld3(a, q);
+    };
+    const float3 zero3 = make_float3(0.0f, 0.0f, 0.0f);
+    uint32_t p = uint32_t(z0) * wh + uint32_t(y) * w + uint32_t(x);
+    float3 am = (ok && z0 >= 1) ? A(p - wh) : zero3;
+    float3 ac = ok ? A(p) : zero3;
+    float len_max = 0.0f;
+    for (int z = z0; z < z1; ++z) {
+        const float3 ap = (ok && z + 1 < g.d) ? A(p + wh) : zero3;
+        plane[ty][tx] = ac;
+        // x neighbours from the adjacent lanes of this row's wave
+        const float lx0 = __shfl_up(ac.x, 1, kTX), lx1 = __shfl_up(ac.y, 1, kTX),
+                    lx2 = __shfl_up(ac.z, 1, kTX);
+        const float rx0 = __shfl_down(ac.x, 1, kTX), rx1 = __shfl_down(ac.y, 1, kTX),
+                    rx2 = __shfl_down(ac.z, 1, kTX);
+        __syncthreads();
+        if (ok) {
+            float3 r = ld3(nd, p);
+            auto sub = [&](float3 t) {
+                r.x = r.x - t.x;
+                r.y = r.y - t.y;
+                r.z = r.z - t.z;
+            };
+            if (p >= 1u) sub(tx > 0 ? make_float3(lx0, lx1, lx2) : A(p - 1u));
+            if (p < n - 1u) sub((tx < kTX - 1 && x + 1 < g.w) ? make_float3(rx0, rx1, rx2) : A(p + 1u));
+            if (p >= w) sub(ty > 0 ? plane[ty - 1][tx] : A(p - w));
+            if (p < n - w) sub((ty < kPTY - 1 && y + 1 < g.h) ? plane[ty + 1][tx] : A(p + w));
+            if (p >= wh) sub(am);
+            if (p < n - wh) sub(ap);
+            r.x = r.x * kInvD2;
+            r.y = r.y * kInvD2;
+            r.z = r.z * kInvD2;
+            st3(b, p, r);
+            if (MAXLEN) {
+                float s2 = r.x * r.x + r.y * r.y;
+                s2 = s2 + r.z * r.z;
+                len_max = fmaxf(len_max, float(sqrt(double(s2))));
+            }
+        }
+        __syncthreads();  // the plane buffer is rewritten next step
+        am = ac;
+        ac = ap;
+        p += wh;
+    }
+    if (MAXLEN) {
+        __shared__ float part[kPTY];
+        const int tid = ty * kTX + tx;
+        const float m = wave_max_f32(len_max);
+        if ((tid & 63) == 0) part[tid >> 6] = m;
+        __syncthreads();
+        if (tid == 0) {
+            float r = part[0];
+            for (int i = 1; i < kPTY; ++i) r = fmaxf(r, part[i]);
+            maxparts[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = r;
         }
     }
 }
@@ -466,41 +749,54 @@ void launch_compose(tiplhip_ctx* ctx, const uint8_t* Is, const float* d, const F
 void launch_gradient(tiplhip_ctx* ctx, const uint8_t* Js, const uint8_t* It, const FieldGeo& g,
                      char type, int accumulate, float* nd, double* partials, float* cost_out)
 {
-    if (type == 'd')
+    uint32_t parts;
+    if (type == 'd') {
         hipLaunchKernelGGL(gradient_kernel<'d'>, tile_grid(g), dim3(kTX, kTY), 0, ctx->stream, Js,
                            It, g, accumulate, nd, partials);
-    else
-        hipLaunchKernelGGL(gradient_kernel<'r'>, tile_grid(g), dim3(kTX, kTY), 0, ctx->stream, Js,
-                           It, g, accumulate, nd, partials);
+        parts = block_count(g);
+    } else {
+        const dim3 bg = box_grid(g);
+        hipLaunchKernelGGL(gradient_r_box_kernel, bg, dim3(kTX, kTY), 0, ctx->stream, Js, It, g,
+                           accumulate, nd, partials);
+        parts = bg.x * bg.y * bg.z;
+    }
     TIPLHIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(cost_reduce_kernel, dim3(1), dim3(256), 0, ctx->stream, partials,
-                       block_count(g), double(g.n), cost_out);
+    hipLaunchKernelGGL(cost_reduce_kernel, dim3(1), dim3(1024), 0, ctx->stream, partials, parts,
+                       double(g.n), cost_out);
     TIPLHIP_CHECK(hipGetLastError());
 }
 
 // `sweeps` Jacobi sweeps of nd; returns the buffer holding the result (a or b).
-// With maxlen != nullptr the last sweep also writes the max length bits.
+// With maxlen != nullptr the last sweep also forms per-block maxima of the
+// vector length in maxparts (block_count(g) floats) and reduces them into
+// *maxlen (float bits).
 float* launch_poisson(tiplhip_ctx* ctx, const float* nd, float* a, float* b, const FieldGeo& g,
-                      int sweeps, unsigned int* maxlen)
+                      int sweeps, unsigned int* maxlen, float* maxparts)
 {
     float* src = a;
     float* dst = b;
+    const dim3 grid = poisson_grid(g), block(kTX, kPTY);
     for (int it = 0; it < sweeps; ++it) {
         const bool first = it == 0, last = it + 1 == sweeps && maxlen;
         if (first && last)
-            hipLaunchKernelGGL((poisson_kernel<true, true>), tile_grid(g), dim3(kTX, kTY), 0,
-                               ctx->stream, nd, src, g, dst, maxlen);
+            hipLaunchKernelGGL((poisson_zm_kernel<true, true>), grid, block, 0, ctx->stream, nd, src,
+                               g, dst, maxparts);
         else if (first)
-            hipLaunchKernelGGL((poisson_kernel<true, false>), tile_grid(g), dim3(kTX, kTY), 0,
-                               ctx->stream, nd, src, g, dst, maxlen);
+            hipLaunchKernelGGL((poisson_zm_kernel<true, false>), grid, block, 0, ctx->stream, nd, src,
+                               g, dst, maxparts);
         else if (last)
-            hipLaunchKernelGGL((poisson_kernel<false, true>), tile_grid(g), dim3(kTX, kTY), 0,
-                               ctx->stream, nd, src, g, dst, maxlen);
+            hipLaunchKernelGGL((poisson_zm_kernel<false, true>), grid, block, 0, ctx->stream, nd, src,
+                               g, dst, maxparts);
         else
-            hipLaunchKernelGGL((poisson_kernel<false, false>), tile_grid(g), dim3(kTX, kTY), 0,
-                               ctx->stream, nd, src, g, dst, maxlen);
+            hipLaunchKernelGGL((poisson_zm_kernel<false, false>), grid, block, 0, ctx->stream, nd, src,
+                               g, dst, maxparts);
         TIPLHIP_CHECK(hipGetLastError());
         std::swap(src, dst);
+    }
+    if (maxlen) {
+        hipLaunchKernelGGL(max_reduce_kernel, dim3(1), dim3(1024), 0, ctx->stream, maxparts,
+                           grid.x * grid.y * grid.z, maxlen);
+        TIPLHIP_CHECK(hipGetLastError());
     }
     return src;
 }
@@ -598,7 +894,8 @@ bool cdm_level(CdmRun& run, const std::vector<const uint8_t*>& It,
 
     DevMem cur(fb), nd(fb), pa(fb), pb(fb), Js(n);
     const uint32_t nblocks = block_count(g);
-    DevMem partials(sizeof(double) * nblocks), scal(sizeof(float) * 16 + sizeof(unsigned int));
+    DevMem partials(sizeof(double) * nblocks), maxparts(sizeof(float) * nblocks);
+    DevMem scal(sizeof(float) * 16 + sizeof(unsigned int));
     float* d_cost = scal.as<float>();
     unsigned int* d_maxlen = reinterpret_cast<unsigned int*>(scal.as<float>() + 16);
     copy_d2d(ctx, cur.p, best_d, fb);
@@ -640,9 +937,8 @@ bool cdm_level(CdmRun& run, const std::vector<const uint8_t*>& It,
             cost.clear();
         }
         const bool need_theta = theta == 0.0f;
-        if (need_theta) zero(ctx, d_maxlen, sizeof(unsigned int));
         float* solved = launch_poisson(ctx, nd.as<float>(), pa.as<float>(), pb.as<float>(), g, 12,
-                                       need_theta ? d_maxlen : nullptr);
+                                       need_theta ? d_maxlen : nullptr, maxparts.as<float>());
         if (need_theta) {
             unsigned int bits = 0;
             TIPLHIP_CHECK(hipMemcpyAsync(&bits, d_maxlen, sizeof(bits), hipMemcpyDeviceToHost,
@@ -813,11 +1109,10 @@ int tiplhip_cdm_poisson(tiplhip_ctx* ctx, float* new_d, const int dim[3], int sw
         DeviceGuard dg(ctx->device);
         const FieldGeo g = field_geo(dim);
         const size_t fb = sizeof(float) * 3 * g.n;
-        DevMem dN(fb), a(fb), b(fb), ml(sizeof(unsigned int));
+        DevMem dN(fb), a(fb), b(fb), ml(sizeof(unsigned int)), mp(sizeof(float) * block_count(g));
         TIPLHIP_CHECK(hipMemcpyAsync(dN.p, new_d, fb, hipMemcpyHostToDevice, ctx->stream));
-        zero(ctx, ml.p, sizeof(unsigned int));
         float* out = launch_poisson(ctx, dN.as<float>(), a.as<float>(), b.as<float>(), g, sweeps,
-                                    ml.as<unsigned int>());
+                                    ml.as<unsigned int>(), mp.as<float>());
         TIPLHIP_CHECK(hipMemcpyAsync(new_d, out, fb, hipMemcpyDeviceToHost, ctx->stream));
         unsigned int bits = 0;
         TIPLHIP_CHECK(hipMemcpyAsync(&bits, ml.p, sizeof(bits), hipMemcpyDeviceToHost, ctx->stream));
