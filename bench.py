@@ -85,6 +85,45 @@ def cpu_baseline(F, M, Ts, seconds_target=12.0):
                       f"{threads} threads, {dt:.1f} s"}
 
 
+def cpu_corr_baseline(F, M, Ts, seconds_target=6.0):
+    """The reference's CPU correlation cost (resample + int64 sums) on a bounded sample."""
+    from oracle.bindings import load_reference
+    ref = load_reference()
+    if ref is None:
+        return None
+    threads = max(1, min(16, os.cpu_count() or 1))
+    ref.set_threads(threads)
+    t0 = time.perf_counter()
+    ref.cost_batch("corr", F, M, Ts[:threads])
+    one = time.perf_counter() - t0
+    reps = max(1, int(seconds_target / max(one, 1e-3)))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ref.cost_batch("corr", F, M, Ts[:threads])
+    dt = time.perf_counter() - t0
+    evals = reps * threads / dt
+    return {"value": round(F.size * evals / 1e9, 4), "unit": "GVox/s", "cores": threads,
+            "kind": "reference", "evals_per_s": round(evals, 3),
+            "sample": f"{reps * threads} correlation evaluations, {threads} threads, {dt:.1f} s"}
+
+
+def cpu_cdm_baseline(It, Is, seconds_target=12.0):
+    """The reference's cdm() loop body at one 182x218x182 level (its own step
+    functions and par_for threads), timed over a bounded number of iterations."""
+    from oracle.bindings import load_reference
+    threads = max(1, min(16, os.cpu_count() or 1))
+    ref = load_reference()
+    if ref is None:
+        return None
+    ref.set_threads(threads)
+    one = ref.cdm_iterations(It, Is, 1)
+    iters = max(1, min(20, int(seconds_target / max(one, 1e-3))))
+    dt = ref.cdm_iterations(It, Is, iters)
+    return {"value": round(iters / dt, 4), "unit": "iter/s", "cores": threads, "kind": "reference",
+            "sample": f"{iters} iterations of the reference cdm() loop body at 182x218x182, "
+                      f"{threads} threads, {dt:.1f} s"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,6 +180,8 @@ def main():
     log("CDM single level at 182x218x182")
     vt = ctx.volume(fixed, 255.99)
     vs = ctx.volume(phantoms.warp_field(fixed, phantoms.rbf_field(shape, 30, 6.0, 2000)), 255.99)
+    copy_gbps = ctx.stream_copy(1 << 30, 10)
+    log(f"device copy {copy_gbps:.0f} GB/s")
     ctx.cdm_level_iterations(vt, vs, 2)
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -157,13 +198,17 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": "mi_hist_kernel<64>", "kernel_ms": round(k_avg_ms, 4)}
 
-    cpu = None
+    cpu = cpu_corr = cpu_cdm = None
     if not args.no_cpu:
-        log("CPU reference baseline")
+        log("CPU reference baselines")
         F8 = vf.read()
         M8 = vm.read()
         cpu = cpu_baseline(F8, M8, Ts)
-        log(f"cpu {cpu}")
+        log(f"cpu mi {cpu}")
+        cpu_corr = cpu_corr_baseline(F8, M8, Ts)
+        log(f"cpu corr {cpu_corr}")
+        cpu_cdm = cpu_cdm_baseline(vt.read(), vs.read())
+        log(f"cpu cdm {cpu_cdm}")
 
     out = {
         "metric": METRIC,
@@ -185,6 +230,9 @@ def main():
         "cpu_baseline": cpu,
         "cdm_iter_per_s": round(cdm_ips, 3),
         "cdm_shape": list(shape),
+        "cdm_cpu_baseline": cpu_cdm,
+        "corr_cpu_baseline": cpu_corr,
+        "hbm_copy_gbps": round(copy_gbps, 1),
         **{k: round(v, 4) for k, v in res.items()},
     }
     print(json.dumps(out), flush=True)

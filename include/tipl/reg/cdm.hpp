@@ -1,0 +1,108 @@
+// reg/cdm.hpp -- tipl::reg::cdm / cdm_common on the MI355X path.
+//
+// Same entry points and parameters as the reference's reg/cdm.hpp:36-44,
+// 373-558; the whole multi-resolution loop runs in the library
+// (tiplhip_cdm) with volumes and fields resident on the device.  As in the
+// reference, a terminated run still returns its best field and reports
+// success; a zero cost returns false.
+#ifndef TIPL_HIP_REG_CDM_HPP
+#define TIPL_HIP_REG_CDM_HPP
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../def.hpp"
+#include "../hip.hpp"
+#include "../image.hpp"
+
+namespace tipl {
+namespace reg {
+
+struct cdm_param {
+    float resolution = 2.0f;
+    float speed = 0.3f;
+    float smoothing = 0.05f;
+    unsigned int min_dimension = 16;
+    enum { corr = 0, mi = 1 } cost_type = corr;  // unused by the reference loop as well
+    std::string gradient_type;                   // per modality 'r' (default) or 'd'
+    std::vector<std::pair<vector<3>, vector<3>>> anchor;  // not used by the loop
+    int max_iterations = 200;                    // cdm() hard-codes 200 (reg/cdm.hpp:418)
+};
+
+namespace detail {
+
+inline tiplhip_cdm_param to_c(const cdm_param& p)
+{
+    tiplhip_cdm_param c;
+    tiplhip_cdm_param_default(&c);
+    c.resolution = p.resolution;
+    c.speed = p.speed;
+    c.smoothing = p.smoothing;
+    c.min_dimension = int(p.min_dimension);
+    c.max_iterations = p.max_iterations;
+    for (size_t i = 0; i < 8; ++i) c.gradient_type[i] = i < p.gradient_type.size() ? p.gradient_type[i] : 'r';
+    return c;
+}
+
+}  // namespace detail
+
+// tipl::reg::cdm (reg/cdm.hpp:373-474): It / Is are lists of modalities of
+// one shape; dis receives the displacement field (x-fastest float3).
+template <typename out_type = void, typename value_type, int dim>
+bool cdm(const std::vector<const_pointer_image<dim, value_type>>& It,
+         const std::vector<const_pointer_image<dim, value_type>>& Is,
+         image<dim, vector<dim>>& dis, bool& terminated, cdm_param param = cdm_param())
+{
+    static_assert(dim == 3, "the MI355X path registers 3-D volumes");
+    const size_t n = std::min(It.size(), Is.size());
+    if (n == 0) throw std::invalid_argument("cdm: empty image list");
+    hip::context& ctx = hip::thread_context();
+    std::vector<std::unique_ptr<hip::volume>> vt, vs;
+    std::vector<const tiplhip_volume*> pt, ps;
+    for (size_t i = 0; i < n; ++i) {
+        if constexpr (std::is_same_v<value_type, unsigned char>) {
+            vt.emplace_back(new hip::volume(ctx, It[i].data(), It[i].shape()));
+            vs.emplace_back(new hip::volume(ctx, Is[i].data(), Is[i].shape()));
+        } else {
+            vt.emplace_back(new hip::volume(ctx, It[i].data(), It[i].shape(), 255.99f));
+            vs.emplace_back(new hip::volume(ctx, Is[i].data(), Is[i].shape(), 255.99f));
+        }
+        pt.push_back(vt.back()->get());
+        ps.push_back(vs.back()->get());
+    }
+    const tiplhip_cdm_param cp = detail::to_c(param);
+    dis.resize(It[0].shape());
+    // the flag is polled in place once per iteration (a bool is one byte)
+    static_assert(sizeof(bool) == 1, "the terminate flag is read as one byte");
+    const volatile uint8_t* stop = reinterpret_cast<const volatile uint8_t*>(&terminated);
+    int iterations = 0;
+    const int rc = tiplhip_cdm(ctx.get(), pt.data(), ps.data(), int(n), &cp, &dis[0][0], stop,
+                               &iterations);
+    if constexpr (!std::is_void_v<out_type>)
+        out_type() << "nonlinear registration using gpu: " << iterations << " iterations";
+    if (rc == TIPLHIP_ERR_FAILED) return false;
+    if (rc == TIPLHIP_ERR_TERMINATED) return true;
+    hip::check(rc);
+    return true;
+}
+
+// tipl::reg::cdm_common (reg/cdm.hpp:534-558).
+template <typename out_type = void, typename value_type, int dim>
+bool cdm_common(std::vector<const_pointer_image<dim, value_type>> It,
+                std::vector<const_pointer_image<dim, value_type>> Is, image<dim, vector<dim>>& dis,
+                bool& terminated, cdm_param param = cdm_param(), bool cuda = true)
+{
+    if (It.size() < Is.size()) Is.resize(It.size());
+    if (Is.size() < It.size()) It.resize(Is.size());
+    if (!(cuda && tipl::has_gpu()))
+        throw std::runtime_error("cdm_common: only the GPU path is provided");
+    return cdm<out_type>(It, Is, dis, terminated, param);
+}
+
+}  // namespace reg
+}  // namespace tipl
+
+#endif

@@ -1,0 +1,445 @@
+// reg/linear.hpp -- tipl::reg::linear on the MI355X path.
+//
+// The same entry points, parameters and optimisation schedule as the
+// reference's reg/linear.hpp:255-714: linear_reg_param, the bound tables,
+// the reg_type stage list (translation -> +scaling -> +rotation -> affine),
+// the resolution pyramid above 96^3, line search + L-BFGS per stage.
+// Differences, by design:
+//   * every optimiser candidate set is evaluated in one batched launch
+//     (tiplhip_cost_batch_params) instead of one thread per candidate;
+//   * the reference races line_search (on a thread) against repeated L-BFGS
+//     and lets them exchange their best point; here line search runs first,
+//     then L-BFGS from the better of the two -- deterministic, with the same
+//     candidate sets and update rules;
+//   * there is no CPU fallback: without a GPU the call throws.
+#ifndef TIPL_HIP_REG_LINEAR_HPP
+#define TIPL_HIP_REG_LINEAR_HPP
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <vector>
+
+#include "../def.hpp"
+#include "../hip.hpp"
+#include "../image.hpp"
+#include "../optimization.hpp"
+#include "../transformation.hpp"
+
+namespace tipl {
+namespace reg {
+
+enum reg_type { none = 0, translocation = 1, rotation = 2, rigid_body = 3, scaling = 4,
+                translocation_scaling = 5, rigid_scaling = 7, tilt = 8, affine = 15 };
+enum cost_type { corr = 0, mutual_info = 1 };
+
+// Bound tables (reg/linear.hpp:258-266): per axis {t+, t-, r+, r-, s+, s-, a+, a-}.
+constexpr float narrow_bound[3][8] = {{0.2f, -0.2f, 0.2f, -0.2f, 1.2f, 0.8f, 0.05f, -0.05f},
+                                      {0.2f, -0.2f, 0.1f, -0.1f, 1.2f, 0.8f, 0.05f, -0.05f},
+                                      {0.2f, -0.2f, 0.1f, -0.1f, 1.2f, 0.8f, 0.05f, -0.05f}};
+constexpr float reg_bound[3][8] = {{0.75f, -0.75f, 0.4f, -0.4f, 1.5f, 0.7f, 0.1f, -0.1f},
+                                   {0.75f, -0.75f, 0.25f, -0.25f, 1.5f, 0.7f, 0.1f, -0.1f},
+                                   {0.75f, -0.75f, 0.25f, -0.25f, 1.5f, 0.7f, 0.1f, -0.1f}};
+constexpr float large_bound[3][8] = {{1.0f, -1.0f, 0.5f, -0.5f, 2.0f, 0.5f, 0.25f, -0.25f},
+                                     {1.0f, -1.0f, 0.4f, -0.4f, 2.0f, 0.5f, 0.25f, -0.25f},
+                                     {1.0f, -1.0f, 0.4f, -0.4f, 2.0f, 0.5f, 0.25f, -0.25f}};
+
+struct linear_reg_param {
+    tipl::reg::reg_type reg_type = affine;
+    tipl::reg::cost_type cost_type = mutual_info;
+    const float (*bound)[8] = reg_bound;
+    unsigned int search_count = 16;
+    bool cuda = tipl::has_gpu();
+    bool absolute_bound = true;
+    int mi_bins = 256;  // 256 is the reference; 64 bins (value >> 2) is an extension
+};
+
+namespace detail {
+
+// The device state of one registration: volumes per modality, voxel sizes.
+struct linear_level {
+    std::vector<std::shared_ptr<hip::volume>> from, to;
+    vector<3> from_vs, to_vs;
+    linear_level down() const
+    {
+        linear_level d;
+        for (auto& v : from) d.from.push_back(std::make_shared<hip::volume>(v->downsample()));
+        for (auto& v : to) d.to.push_back(std::make_shared<hip::volume>(v->downsample()));
+        d.from_vs = from_vs;
+        d.to_vs = to_vs;
+        d.from_vs *= 2.0f;
+        d.to_vs *= 2.0f;
+        return d;
+    }
+};
+
+// linear_reg_runner (reg/linear.hpp:290-561).
+class runner {
+public:
+    affine_param<float, 3> arg_upper, arg_lower;
+    affine_param<float, 3>& arg_min;
+    unsigned int search_count = 16;
+    double precision = 0.001;
+    size_t max_iterations = 128;
+    std::vector<reg_type> reg_list = {translocation, translocation_scaling, rigid_scaling, affine};
+    cost_type cost = mutual_info;
+    int bins = 256;
+
+    runner(hip::context& ctx, linear_level lv, affine_param<float, 3>& arg)
+        : arg_min(arg), ctx_(ctx), lv_(std::move(lv)) {}
+
+    std::pair<affine_param<float, 3>, affine_param<float, 3>> current_bound(reg_type type) const
+    {
+        affine_param<float, 3> upper(arg_upper), lower(arg_lower);
+        for (int i = 0; i < 3; ++i) {
+            if (!(type & translocation)) upper.translocation[i] = lower.translocation[i] = arg_min.translocation[i];
+            if (!(type & scaling)) upper.scaling[i] = lower.scaling[i] = arg_min.scaling[i];
+            if (!(type & rotation)) upper.rotation[i] = lower.rotation[i] = arg_min.rotation[i];
+            if (!(type & tilt)) upper.affine[i] = lower.affine[i] = arg_min.affine[i];
+        }
+        return {upper, lower};
+    }
+
+    void set_bound(reg_type type, const float bound[3][8], bool absolute)
+    {
+        if (type == translocation) reg_list = {translocation};
+        if (type == rotation) reg_list = {rotation};
+        if (type == rigid_body) reg_list = {translocation, rigid_body};
+        if (type == scaling) reg_list = {scaling};
+        if (type == rigid_scaling) reg_list = {translocation, translocation_scaling, rigid_scaling};
+        if (type == affine) reg_list = {translocation, translocation_scaling, rigid_scaling, affine};
+        if (absolute) {
+            arg_upper.clear();
+            arg_lower.clear();
+        } else {
+            arg_upper = arg_lower = arg_min;
+        }
+        const shape<3>& fs = lv_.from[0]->shape();
+        const shape<3>& ts = lv_.to[0]->shape();
+        for (int i = 0; i < 3; ++i) {
+            if (type & translocation) {
+                const float range = std::max<float>(fs[i] * lv_.from_vs[i], ts[i] * lv_.to_vs[i]) * 0.5f;
+                arg_upper.translocation[i] += range * bound[i][0];
+                arg_lower.translocation[i] += range * bound[i][1];
+            }
+            if (type & scaling) {
+                arg_upper.scaling[i] *= bound[i][4];
+                arg_lower.scaling[i] *= bound[i][5];
+            }
+            if (type & rotation) {
+                arg_upper.rotation[i] += 1.57079632679f * bound[i][2];
+                arg_lower.rotation[i] += 1.57079632679f * bound[i][3];
+            }
+            if (type & tilt) {
+                arg_upper.affine[i] += bound[i][6];
+                arg_lower.affine[i] += bound[i][7];
+            }
+        }
+    }
+
+    // fun over a candidate set: per modality one batched launch, then the
+    // reference's mean over modalities divided by 2 (reg/linear.hpp:427-444).
+    std::vector<double> evaluate(const std::vector<optimization::param_vec>& xs)
+    {
+        std::vector<double> total(xs.size(), 0.0);
+        if (xs.empty()) return total;
+        std::vector<float> P(12 * xs.size());
+        for (size_t k = 0; k < xs.size(); ++k) std::copy_n(xs[k].data(), 12, &P[12 * k]);
+        std::vector<double> c(xs.size());
+        const size_t nmod = std::min(lv_.from.size(), lv_.to.size());
+        const int kind = cost == mutual_info ? TIPLHIP_COST_MI : TIPLHIP_COST_CORR;
+        for (size_t i = 0; i < nmod; ++i) {
+            hip::check(tiplhip_cost_batch_params(ctx_.get(), lv_.from[i]->get(), lv_.from_vs.begin(),
+                                                 lv_.to[i]->get(), lv_.to_vs.begin(), P.data(),
+                                                 int(xs.size()), kind, bins, c.data()));
+            for (size_t k = 0; k < xs.size(); ++k) total[k] += c[k];
+        }
+        for (double& v : total) v /= double(float(nmod) * 2.0f);
+        evaluations += xs.size();
+        return total;
+    }
+
+    template <class Term>
+    float run_optimize(unsigned int cur_search_count, Term&& is_terminated)
+    {
+        auto fun = [&](const std::vector<optimization::param_vec>& xs) { return evaluate(xs); };
+        optimization::param_vec x(arg_min.begin(), arg_min.end());
+        double optimal_value = evaluate({x})[0];
+        if (cur_search_count) {
+            for (size_t it = 0; it < reg_list.size() && !is_terminated(); ++it) {
+                const auto b = current_bound(reg_list[it]);
+                optimization::param_vec x2(x);
+                double v2 = optimal_value;
+                optimization::line_search(x2, b.first.begin(), b.second.begin(),
+                                          {0, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1}, fun, v2,
+                                          int(cur_search_count), is_terminated);
+                if (v2 < optimal_value) {
+                    optimal_value = v2;
+                    x = x2;
+                }
+                optimization::lbfgs(x, b.first.begin(), b.second.begin(), fun, optimal_value,
+                                    is_terminated, precision, int(max_iterations));
+                std::copy_n(x.data(), 12, arg_min.begin());
+            }
+        } else {
+            const auto b = current_bound(reg_list.back());
+            optimization::lbfgs(x, b.first.begin(), b.second.begin(), fun, optimal_value,
+                                is_terminated, precision * 0.2f, int(max_iterations));
+            std::copy_n(x.data(), 12, arg_min.begin());
+        }
+        return float(optimal_value);
+    }
+
+    // run_optimize_mr (reg/linear.hpp:527-550): above 96^3 optimise a
+    // half-resolution copy first (recursively); above 256^3 stop there.
+    template <class Term>
+    float run_optimize_mr(Term&& is_terminated)
+    {
+        const size_t sz = lv_.from[0]->shape().size();
+        const bool is_large = double(sz) > std::pow(96.0, 3);
+        if (is_large) {
+            runner low(ctx_, lv_.down(), arg_min);
+            low.arg_upper = arg_upper;
+            low.arg_lower = arg_lower;
+            low.search_count = search_count;
+            low.precision = precision;
+            low.max_iterations = max_iterations;
+            low.reg_list = reg_list;
+            low.cost = cost;
+            low.bins = bins;
+            const float v = low.run_optimize_mr(is_terminated);
+            evaluations += low.evaluations;
+            if (double(sz) > std::pow(256.0, 3)) return v;
+        }
+        return run_optimize(is_large ? 0 : search_count, is_terminated);
+    }
+
+    size_t evaluations = 0;
+
+private:
+    hip::context& ctx_;
+    linear_level lv_;
+};
+
+// has_mask (numerical/basic_op.hpp:811-830) with bounding_box (:697-731).
+template <class I>
+bool has_mask(const I& img)
+{
+    const size_t sz = img.size(), threshold = sz / 5;
+    size_t zeros = 0;
+    for (size_t i = 0; i < sz && zeros <= threshold; ++i)
+        if (img[i] == 0) ++zeros;
+    if (zeros > threshold) return true;
+    const shape<3>& s = img.shape();
+    int lo[3] = {int(s[0]) - 1, int(s[1]) - 1, int(s[2]) - 1}, hi[3] = {0, 0, 0};
+    size_t i = 0;
+    for (unsigned z = 0; z < s[2]; ++z)
+        for (unsigned y = 0; y < s[1]; ++y)
+            for (unsigned x = 0; x < s[0]; ++x, ++i) {
+                if (img[i] <= 0) continue;
+                const int p[3] = {int(x), int(y), int(z)};
+                for (int d = 0; d < 3; ++d) {
+                    lo[d] = std::min(lo[d], p[d]);
+                    hi[d] = std::max(hi[d], p[d]);
+                }
+            }
+    for (int d = 0; d < 3; ++d) {
+        if (hi[d] == 0) hi[d] = lo[d] = 0;
+        ++hi[d];
+        if (lo[d] > 0 && hi[d] < int(s[d])) return true;
+    }
+    return false;
+}
+
+// Symmetric 3x3 eigen decomposition (Jacobi rotations); eigenvalues
+// descending, eigenvectors as rows of V.
+inline void eigen_sym3(const double* A_in, double* V, double* d)
+{
+    double A[9], U[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    std::copy(A_in, A_in + 9, A);
+    for (int sweep = 0; sweep < 50; ++sweep) {
+        const double off = A[1] * A[1] + A[2] * A[2] + A[5] * A[5];
+        if (off < 1e-30) break;
+        for (int p = 0; p < 2; ++p)
+            for (int q = p + 1; q < 3; ++q) {
+                const double apq = A[p * 3 + q];
+                if (std::abs(apq) < 1e-300) continue;
+                const double theta = (A[q * 3 + q] - A[p * 3 + p]) / (2 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::abs(theta) + std::sqrt(theta * theta + 1));
+                const double c = 1 / std::sqrt(t * t + 1), s = t * c;
+                for (int k = 0; k < 3; ++k) {  // A = J^T A J
+                    const double akp = A[k * 3 + p], akq = A[k * 3 + q];
+                    A[k * 3 + p] = c * akp - s * akq;
+                    A[k * 3 + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < 3; ++k) {
+                    const double apk = A[p * 3 + k], aqk = A[q * 3 + k];
+                    A[p * 3 + k] = c * apk - s * aqk;
+                    A[q * 3 + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < 3; ++k) {
+                    const double ukp = U[k * 3 + p], ukq = U[k * 3 + q];
+                    U[k * 3 + p] = c * ukp - s * ukq;
+                    U[k * 3 + q] = s * ukp + c * ukq;
+                }
+            }
+    }
+    int order[3] = {0, 1, 2};
+    std::sort(order, order + 3, [&](int a, int b) { return A[a * 4] > A[b * 4]; });
+    for (int r = 0; r < 3; ++r) {
+        d[r] = A[order[r] * 4];
+        for (int k = 0; k < 3; ++k) V[r * 3 + k] = U[k * 3 + order[r]];
+    }
+}
+
+// estimate_affine_param (numerical/transformation.hpp:933-1021): initial
+// affine from the centroid and second moments of the non-zero voxels.  The
+// eigenvector signs and the eigen solver differ from the reference's, so
+// the initial guess agrees with it to within a principal-axis flip at most
+// (the result is only a starting point for the optimiser).
+template <class I>
+void estimate_affine_param(const I& src, const vector<3>& svs, const I& tgt, const vector<3>& tvs,
+                           affine_param<float, 3>& arg)
+{
+    arg.clear();
+    auto moments = [](const I& img, const vector<3>& vs, double* c, double* cov) {
+        const shape<3>& s = img.shape();
+        size_t n = 0, i = 0;
+        c[0] = c[1] = c[2] = 0.0;
+        for (unsigned z = 0; z < s[2]; ++z)
+            for (unsigned y = 0; y < s[1]; ++y)
+                for (unsigned x = 0; x < s[0]; ++x, ++i)
+                    if (img[i] > 0) {
+                        c[0] += x * vs[0];
+                        c[1] += y * vs[1];
+                        c[2] += z * vs[2];
+                        ++n;
+                    }
+        if (!n) return n;
+        for (int k = 0; k < 3; ++k) c[k] /= double(n);
+        std::fill(cov, cov + 9, 0.0);
+        i = 0;
+        for (unsigned z = 0; z < s[2]; ++z)
+            for (unsigned y = 0; y < s[1]; ++y)
+                for (unsigned x = 0; x < s[0]; ++x, ++i)
+                    if (img[i] > 0) {
+                        const double d[3] = {x * vs[0] - c[0], y * vs[1] - c[1], z * vs[2] - c[2]};
+                        cov[0] += d[0] * d[0];
+                        cov[1] += d[0] * d[1];
+                        cov[2] += d[0] * d[2];
+                        cov[4] += d[1] * d[1];
+                        cov[5] += d[1] * d[2];
+                        cov[8] += d[2] * d[2];
+                    }
+        cov[3] = cov[1];
+        cov[6] = cov[2];
+        cov[7] = cov[5];
+        for (int k = 0; k < 9; ++k) cov[k] /= double(n);
+        return n;
+    };
+    double cs[3], ct[3], Cs[9], Ct[9];
+    if (!moments(src, svs, cs, Cs) || !moments(tgt, tvs, ct, Ct)) return;
+    double Vs[9], Vt[9], Ls[3], Lt[3], R[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    eigen_sym3(Cs, Vs, Ls);
+    eigen_sym3(Ct, Vt, Lt);
+    for (int i = 0; i < 3; ++i)
+        arg.scaling[i] = float(std::sqrt(std::max(Lt[i], 0.0) / std::max(Ls[i], 1e-6)));
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            for (int k = 0; k < 3; ++k) R[i * 3 + j] += Vt[i + k * 3] * Vs[j + k * 3];
+    const double det = R[0] * (R[4] * R[8] - R[5] * R[7]) - R[1] * (R[3] * R[8] - R[5] * R[6]) +
+                       R[2] * (R[3] * R[7] - R[4] * R[6]);
+    if (det < 0.0)
+        for (int i = 0; i < 3; ++i) R[i * 3 + 2] = -R[i * 3 + 2];
+    arg.rotation[0] = float(std::atan2(R[7], R[8]));
+    arg.rotation[1] = float(std::atan2(-R[6], std::sqrt(R[7] * R[7] + R[8] * R[8])));
+    arg.rotation[2] = float(std::atan2(R[3], R[0]));
+    const float pi_half = 1.570796327f;
+    for (int i = 0; i < 3; ++i) arg.rotation[i] = std::remainder(arg.rotation[i], pi_half);
+    double dt[3];
+    for (int i = 0; i < 3; ++i)
+        dt[i] = (ct[i] - tgt.shape()[i] * tvs[i] * 0.5) / std::max<double>(arg.scaling[i], 1e-6);
+    for (int i = 0; i < 3; ++i) {
+        const double r = R[i] * dt[0] + R[3 + i] * dt[1] + R[6 + i] * dt[2];
+        arg.translocation[i] = float(r - (cs[i] - src.shape()[i] * svs[i] * 0.5));
+    }
+}
+
+}  // namespace detail
+
+// tipl::reg::linear (reg/linear.hpp:622-700): registers `from` onto `to`
+// (lists of modalities, uint8), arg in/out; returns the final cost.
+template <typename out_type = void, int dim>
+float linear(std::vector<const_pointer_image<dim, unsigned char>> from, vector<dim> from_vs,
+             std::vector<const_pointer_image<dim, unsigned char>> to, vector<dim> to_vs,
+             affine_param<float, dim>& arg, linear_reg_param param = linear_reg_param(),
+             bool& terminated = tipl::prog_aborted)
+{
+    static_assert(dim == 3, "the MI355X path registers 3-D volumes");
+    if (from.empty() || to.empty()) throw std::invalid_argument("linear: empty image list");
+    if (!param.cuda) throw std::runtime_error("linear: only the GPU path is provided (param.cuda)");
+    hip::context& ctx = hip::thread_context();
+    auto term = [&] { return bool(terminated); };
+
+    auto run = [&](const vector<3>& f_vs, const vector<3>& t_vs, affine_param<float, 3>& a, bool mr) {
+        detail::linear_level lv;
+        for (auto& f : from) lv.from.push_back(std::make_shared<hip::volume>(ctx, f.data(), f.shape()));
+        for (auto& t : to) lv.to.push_back(std::make_shared<hip::volume>(ctx, t.data(), t.shape()));
+        lv.from_vs = f_vs;
+        lv.to_vs = t_vs;
+        detail::runner r(ctx, std::move(lv), a);
+        r.set_bound(param.reg_type, param.bound, param.absolute_bound);
+        r.search_count = param.search_count;
+        r.cost = param.cost_type;
+        r.bins = param.mi_bins;
+        if constexpr (!std::is_void_v<out_type>)
+            out_type() << (param.reg_type == affine ? "affine" : "rigid body") << " registration by "
+                       << (param.cost_type == mutual_info ? "mutual info" : "correlation") << " using gpu";
+        return mr ? r.run_optimize_mr(term) : r.run_optimize(0, term);
+    };
+
+    if (param.bound == narrow_bound) {
+        param.absolute_bound = false;
+        param.search_count = 0;
+        return run(from_vs, to_vs, arg, false);
+    }
+    vector<3> new_to_vs = to_vs;
+    affine_param<float, 3> surrogate = arg;
+    bool adjust_vs = false;
+    if (param.reg_type == affine && detail::has_mask(from[0]) && detail::has_mask(to[0])) {
+        detail::estimate_affine_param(from[0], from_vs, to[0], to_vs, surrogate);
+        arg = surrogate;
+        for (int i = 0; i < 3; ++i) {
+            new_to_vs[i] = to_vs[i] / std::max<float>(surrogate.scaling[i], 0.01f);
+            surrogate.scaling[i] = 1.0f;
+        }
+        adjust_vs = true;
+    }
+    param.absolute_bound = true;
+    const float result = run(from_vs, new_to_vs, adjust_vs ? surrogate : arg, true);
+    if (adjust_vs)
+        arg = transformation_matrix<float, 3>(surrogate, from[0].shape(), from_vs, to[0].shape(), new_to_vs)
+                  .to_affine_param(from[0].shape(), from_vs, to[0].shape(), to_vs);
+    return result;
+}
+
+// The overload returning the transformation (reg/linear.hpp:703-714).
+template <typename out_type = void, int dim>
+transformation_matrix<float, dim> linear(std::vector<const_pointer_image<dim, unsigned char>> from,
+                                         vector<dim> from_vs,
+                                         std::vector<const_pointer_image<dim, unsigned char>> to,
+                                         vector<dim> to_vs,
+                                         const linear_reg_param& param = linear_reg_param(),
+                                         bool& terminated = tipl::prog_aborted)
+{
+    affine_param<float, dim> arg;
+    linear<out_type>(from, from_vs, to, to_vs, arg, param, terminated);
+    return transformation_matrix<float, dim>(arg, from[0].shape(), from_vs, to[0].shape(), to_vs);
+}
+
+}  // namespace reg
+}  // namespace tipl
+
+#endif

@@ -89,6 +89,12 @@ int tiplhip_ctx_kernel_time(tiplhip_ctx* ctx, double* total_ms, int64_t* launche
 int tiplhip_timer_start(tiplhip_ctx* ctx);
 int tiplhip_timer_stop(tiplhip_ctx* ctx, float* ms);
 
+/* Streaming device copy of `bytes` (rounded down to 16), repeated `reps`
+ * times on the context stream: the achievable HBM bandwidth (read + write
+ * bytes / time, GB/s) and a kernel of known traffic for calibrating memory
+ * counters.  No counterpart in the reference (measurement support). */
+int tiplhip_stream_copy(tiplhip_ctx* ctx, size_t bytes, int reps, double* gbps);
+
 /* ---- volumes ------------------------------------------------------------- */
 
 /* Upload a volume and keep it resident on the device as uint8.
@@ -183,11 +189,14 @@ void tiplhip_cdm_param_default(tiplhip_cdm_param* p);
 
 /* tipl::reg::cdm / cdm_cuda (reg/cdm.hpp:373-532) for n modality pairs of
  * equal shape.  field (host) receives vox(dim) float3.  terminated may be
- * NULL; it is polled once per iteration (reg/cdm.hpp:418).  Returns
- * TIPLHIP_ERR_FAILED where the reference returns false. */
+ * NULL; it is a one-byte flag (a C++ bool) polled once per iteration, as
+ * the reference polls its bool& (reg/cdm.hpp:418).  Returns
+ * TIPLHIP_ERR_FAILED where the reference returns false, and
+ * TIPLHIP_ERR_TERMINATED (with the best field so far written) when the flag
+ * stopped the run. */
 int tiplhip_cdm(tiplhip_ctx* ctx, const tiplhip_volume* const* It, const tiplhip_volume* const* Is,
                 int n, const tiplhip_cdm_param* param, float* field,
-                const volatile int* terminated, int* iterations);
+                const volatile uint8_t* terminated, int* iterations);
 
 /* One resolution level of the CDM loop with no pyramid: runs exactly
  * `iterations` loop iterations (no early exit) from a zero field, for

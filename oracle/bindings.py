@@ -259,6 +259,17 @@ class Reference(_Lib):
                              C.c_float, C.c_float, C.c_float, C.c_int, C.c_char, _f32p)
         self._linear = self._fn("tref_linear", C.c_float, _u8p, _i32p, _f32p, _u8p, _i32p, _f32p,
                                 C.c_int, C.c_int, _f32p)
+        self._cdm_iters = self._fn("tref_cdm_iterations", C.c_double, _u8p, _u8p, _i32p, C.c_int,
+                                   C.c_void_p)
+
+    def cdm_iterations(self, It, Is, iters, want_field=False):
+        """Wall seconds of `iters` reference cdm() loop-body iterations at one level."""
+        It = np.ascontiguousarray(It, np.uint8)
+        Is = np.ascontiguousarray(Is, np.uint8)
+        field = np.zeros(It.shape + (3,), np.float32) if want_field else None
+        sec = self._cdm_iters(It, Is, _dims(It), int(iters),
+                              field.ctypes.data if want_field else None)
+        return (float(sec), field) if want_field else float(sec)
 
     def set_threads(self, n: int):
         self._threads(int(n))

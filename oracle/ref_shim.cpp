@@ -8,6 +8,7 @@
 // reference's CPU path as its baseline.
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -223,6 +224,34 @@ int tref_cdm(const uint8_t* const* It, const uint8_t* const* Is, int nmod, const
     bool ok = tipl::reg::cdm(vt, vs, D, terminated, p);
     if (ok) from_field(D, field);
     return ok ? 1 : 0;
+}
+
+// `iters` iterations of the body of the reference's cdm() loop
+// (reg/cdm.hpp:418-470) at one resolution level, from a zero field, with
+// the reference's own step functions and par_for threading; no early exit.
+// Returns the wall time in seconds; the field is written to `field`.
+double tref_cdm_iterations(const uint8_t* It, const uint8_t* Is, const int* dim, int iters,
+                           float* field)
+{
+    tipl::const_pointer_image<3, uint8_t> T(It, shp(dim)), S(Is, shp(dim));
+    field_t cur(shp(dim));
+    float theta = 0.0f, cur_smoothing = 0.8f;
+    const float speed = 0.3f;
+    bool terminated = false;
+    auto t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < iters; ++i) {
+        field_t new_d;
+        (void)tipl::reg::cdm_get_gradient(tipl::compose_displacement(S, cur), T, new_d, 'r');
+        tipl::reg::cdm_solve_poisson(new_d, terminated);
+        if (theta == 0.0f) theta = tipl::reg::cdm_max_displacement_length(new_d);
+        if (theta == 0.0f) break;
+        new_d *= speed / theta;
+        tipl::accumulate_displacement(cur, new_d);
+        tipl::reg::cdm_smooth(new_d, cur, cur_smoothing);
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    if (field) from_field(cur, field);
+    return std::chrono::duration<double>(t1 - t0).count();
 }
 
 // tipl::reg::linear on the CPU; arg (12 floats) is in/out.

@@ -1,0 +1,72 @@
+// reg_cli.cpp -- drives the header-only C++ entry points (include/tipl) the
+// way a TIPL user would, for the GPU tests in tests/test_gpu_cpp_api.py.
+//
+//   reg_cli linear <from.u8> <to.u8> <w> <h> <d> <mi|corr> <out.txt>
+//       tipl::reg::linear on from -> to (voxel size 1), writes the 12 affine
+//       parameters and the final cost.
+//   reg_cli cdm <It.u8> <Is.u8> <w> <h> <d> <out.f32>
+//       tipl::reg::cdm_common, writes the displacement field (w*h*d*3 floats).
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "tipl/tipl.hpp"
+
+static std::vector<unsigned char> read_u8(const char* path, size_t n)
+{
+    std::vector<unsigned char> v(n);
+    std::ifstream f(path, std::ios::binary);
+    if (!f.read(reinterpret_cast<char*>(v.data()), std::streamsize(n)))
+        throw std::runtime_error(std::string("cannot read ") + path);
+    return v;
+}
+
+int main(int argc, char** argv)
+{
+    try {
+        if (argc < 8) {
+            std::cerr << "usage: reg_cli linear|cdm ...\n";
+            return 2;
+        }
+        const std::string mode = argv[1];
+        const tipl::shape<3> s(unsigned(std::atoi(argv[4])), unsigned(std::atoi(argv[5])),
+                               unsigned(std::atoi(argv[6])));
+        auto a = read_u8(argv[2], s.size());
+        auto b = read_u8(argv[3], s.size());
+        tipl::const_pointer_image<3, unsigned char> A(a.data(), s), B(b.data(), s);
+        if (mode == "linear" && argc >= 9) {
+            tipl::reg::linear_reg_param p;
+            p.cost_type = std::string(argv[7]) == "corr" ? tipl::reg::corr : tipl::reg::mutual_info;
+            tipl::affine_param<float, 3> arg;
+            bool terminated = false;
+            const float cost = tipl::reg::linear<void>(
+                std::vector<tipl::const_pointer_image<3, unsigned char>>{A}, tipl::vector<3>(1, 1, 1),
+                std::vector<tipl::const_pointer_image<3, unsigned char>>{B}, tipl::vector<3>(1, 1, 1),
+                arg, p, terminated);
+            std::ofstream out(argv[8]);
+            out.precision(9);
+            for (float v : arg) out << v << " ";
+            out << "\n" << cost << "\n";
+            return 0;
+        }
+        if (mode == "cdm") {
+            tipl::image<3, tipl::vector<3>> dis;
+            bool terminated = false;
+            const bool ok = tipl::reg::cdm_common<void>(
+                std::vector<tipl::const_pointer_image<3, unsigned char>>{A},
+                std::vector<tipl::const_pointer_image<3, unsigned char>>{B}, dis, terminated);
+            std::ofstream out(argv[7], std::ios::binary);
+            out.write(reinterpret_cast<const char*>(&dis[0][0]),
+                      std::streamsize(sizeof(float) * 3 * dis.size()));
+            return ok ? 0 : 1;
+        }
+        std::cerr << "unknown mode\n";
+        return 2;
+    } catch (const std::exception& e) {
+        std::cerr << "error: " << e.what() << "\n";
+        return 3;
+    }
+}

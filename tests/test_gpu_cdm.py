@@ -117,12 +117,30 @@ def test_cdm_terminated_flag(ctx):
     import ctypes
     import tipl_amd
     It, Is = _pair((32, 32, 32), 14)
-    flag = ctypes.c_int(1)
+    flag = ctypes.c_uint8(1)
     rc, field, iters = ctx.cdm(ctx.volume(It), ctx.volume(Is), terminated=flag)
     assert rc == tipl_amd.ERR_TERMINATED and iters == 0 and not field.any()
 
 
-def test_cdm_level_iterations_runs(ctx):
+def oracle_level_iterations(oracle, It, Is, iters, speed=0.3, smoothing=0.8):
+    """The body of cdm()'s loop (reg/cdm.hpp:418-470) `iters` times from a
+    zero field, no early exit, composed from the oracle's step functions."""
+    cur = np.zeros(It.shape + (3,), np.float32)
+    theta = 0.0
+    for _ in range(iters):
+        nd, _ = oracle.gradient(oracle.compose(Is, cur), It, "r")
+        nd = oracle.poisson(nd)
+        if theta == 0.0:
+            theta = oracle.max_length(nd)
+        if theta == 0.0:
+            break
+        k = np.float32(speed) / np.float32(theta)
+        nd = oracle.accumulate(cur, (nd * k).astype(np.float32))
+        cur = oracle.smooth(nd, cur, smoothing)
+    return cur
+
+
+def test_cdm_level_iterations_match_oracle(ctx, oracle):
     It, Is = _pair((30, 34, 26), 15)
-    f = ctx.cdm_level_iterations(ctx.volume(It), ctx.volume(Is), 5, want_field=True)
-    assert np.isfinite(f).all()
+    f = ctx.cdm_level_iterations(ctx.volume(It), ctx.volume(Is), 6, want_field=True)
+    assert _bits_equal(f, oracle_level_iterations(oracle, It, Is, 6))

@@ -114,6 +114,15 @@ def test_cdm_improved(oracle, ref):
             assert oracle.improved(c) == ref.improved(c)
 
 
+def test_cdm_loop_body(oracle, ref):
+    from test_gpu_cdm import oracle_level_iterations
+    from scipy.ndimage import shift
+    It = smooth_volume((26, 30, 22), 14, sigma=2.5)
+    Is = shift(It.astype(np.float32), (0.5, 1.0, -1.0), order=1).astype(np.uint8)
+    _, f_r = ref.cdm_iterations(It, Is, 5, want_field=True)
+    assert np.array_equal(_bits(oracle_level_iterations(oracle, It, Is, 5)), _bits(f_r))
+
+
 @pytest.mark.parametrize("shape,kind", [((36, 40, 32), "r"), ((33, 29, 35), "d"), ((20, 20, 20), "r")])
 def test_whole_cdm(oracle, ref, shape, kind):
     from scipy.ndimage import shift

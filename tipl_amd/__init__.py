@@ -25,7 +25,7 @@ COST_CORR, COST_MI = 0, 1
 EXPORTS = (
     "tiplhip_last_error", "tiplhip_device_count", "tiplhip_ctx_create", "tiplhip_ctx_destroy",
     "tiplhip_ctx_synchronize", "tiplhip_ctx_set_kernel_timing", "tiplhip_ctx_kernel_time",
-    "tiplhip_timer_start", "tiplhip_timer_stop", "tiplhip_volume_create",
+    "tiplhip_timer_start", "tiplhip_timer_stop", "tiplhip_stream_copy", "tiplhip_volume_create",
     "tiplhip_volume_create_device", "tiplhip_volume_destroy", "tiplhip_volume_dims",
     "tiplhip_volume_read", "tiplhip_volume_stats", "tiplhip_volume_downsample",
     "tiplhip_resample", "tiplhip_corr_sums", "tiplhip_mi_hist", "tiplhip_cost_batch",
@@ -73,6 +73,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "tiplhip_ctx_kernel_time": (ip, [vp, P(C.c_double), P(C.c_int64)]),
         "tiplhip_timer_start": (ip, [vp]),
         "tiplhip_timer_stop": (ip, [vp, P(C.c_float)]),
+        "tiplhip_stream_copy": (ip, [vp, C.c_size_t, ip, P(C.c_double)]),
         "tiplhip_volume_create": (ip, [vp, vp, ip, P(ip), C.c_float, P(vp)]),
         "tiplhip_volume_create_device": (ip, [vp, vp, ip, P(ip), C.c_float, P(vp)]),
         "tiplhip_volume_destroy": (ip, [vp]),
@@ -224,6 +225,12 @@ class Context:
         _check(load_library().tiplhip_timer_stop(self.handle, C.byref(ms)))
         return ms.value
 
+    def stream_copy(self, nbytes: int, reps: int = 20) -> float:
+        """Achievable HBM bandwidth of a plain device copy, GB/s (read + write)."""
+        g = C.c_double(0)
+        _check(load_library().tiplhip_stream_copy(self.handle, nbytes, reps, C.byref(g)))
+        return g.value
+
     # -- volumes -----------------------------------------------------------
     def volume(self, data: np.ndarray, upper_limit: float = 255.99) -> Volume:
         if data.dtype == np.uint8:
@@ -293,7 +300,7 @@ class Context:
         return p
 
     def cdm(self, It: Sequence[Volume], Is: Sequence[Volume], param: Optional[CdmParam] = None,
-            terminated: Optional[C.c_int] = None):
+            terminated: Optional[C.c_uint8] = None):
         if isinstance(It, Volume):
             It, Is = [It], [Is]
         n = len(It)

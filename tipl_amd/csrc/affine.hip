@@ -24,10 +24,13 @@
 namespace tiplhip {
 namespace {
 
+// Work item = one wave tile column (16 x 4 voxels of a plane) marched
+// through a run of zc planes.
 struct BatchGeo {
     int fw, fh, fd;             // from grid
-    uint32_t ntx, nty;          // 16 x 4 wave tiles per row / per plane
-    uint32_t items;             // ntx * nty * fd
+    uint32_t ntx, ncols;        // tiles per row, tile columns per plane
+    uint32_t zc, nzc;           // planes per item, z runs
+    uint32_t items;             // ncols * nzc
     uint32_t items_per_block;
     uint32_t K;
     uint32_t total_blocks;      // K * ceil(items / items_per_block)
@@ -50,20 +53,24 @@ VolGeo make_geo(const int* dim)
 
 constexpr uint32_t kTileX = 16, kTileY = 4;  // one wave = 16 x 4 voxels of one plane
 
+// max_voxels_per_block bounds the voxels one workgroup adds (the uint16 LDS
+// counters of the 256-bin histogram hold at most 65535).
 BatchGeo make_batch(const tiplhip_volume* from, const tiplhip_volume* to, int K,
-                    uint32_t max_items_per_block, uint32_t target_blocks)
+                    uint32_t max_voxels_per_block, uint32_t target_blocks)
 {
     BatchGeo b;
     b.fw = from->dim[0];
     b.fh = from->dim[1];
     b.fd = from->dim[2];
     b.ntx = (uint32_t(b.fw) + kTileX - 1) / kTileX;
-    b.nty = (uint32_t(b.fh) + kTileY - 1) / kTileY;
-    b.items = b.ntx * b.nty * uint32_t(b.fd);
+    b.ncols = b.ntx * ((uint32_t(b.fh) + kTileY - 1) / kTileY);
+    b.zc = std::min<uint32_t>(16u, uint32_t(b.fd));
+    b.nzc = (uint32_t(b.fd) + b.zc - 1) / b.zc;
+    b.items = b.ncols * b.nzc;
+    const uint32_t max_items = std::max<uint32_t>(1u, max_voxels_per_block / (64u * b.zc));
     uint64_t want = (uint64_t(b.items) * uint64_t(K) + target_blocks - 1) / target_blocks;
-    uint32_t ipb = uint32_t(std::min<uint64_t>(want, max_items_per_block));
-    ipb = std::max<uint32_t>(ipb, 16u);
-    ipb = std::min<uint32_t>(ipb, max_items_per_block);
+    uint32_t ipb = uint32_t(std::min<uint64_t>(want, max_items));
+    ipb = std::max<uint32_t>(ipb, std::min<uint32_t>(4u, max_items));
     b.items_per_block = ipb;
     b.K = uint32_t(K);
     uint32_t chunks = (b.items + ipb - 1) / ipb;
@@ -74,22 +81,40 @@ BatchGeo make_batch(const tiplhip_volume* from, const tiplhip_volume* to, int K,
     return b;
 }
 
-// Wave-uniform decode of a tile item into (tile x, tile y, z).
+// Wave-uniform decode of an item into this lane's (x, y) and the z run.
 struct Item {
-    uint32_t x, y, z;   // this lane's voxel
+    uint32_t x, y;      // this lane's voxel column
+    uint32_t z0, z1;    // planes [z0, z1)
     bool ok;
 };
 __device__ inline Item decode(const BatchGeo& g, uint32_t it, uint32_t lane)
 {
-    const uint32_t tx = it % g.ntx;
-    const uint32_t r = it / g.ntx;
-    const uint32_t ty = r % g.nty;
+    const uint32_t c = it % g.ncols;
+    const uint32_t j = it / g.ncols;
+    const uint32_t tx = c % g.ntx;
+    const uint32_t ty = c / g.ntx;
     Item m;
-    m.z = r / g.nty;
     m.x = tx * kTileX + (lane & (kTileX - 1));
     m.y = ty * kTileY + (lane / kTileX);
+    m.z0 = j * g.zc;
+    m.z1 = min(m.z0 + g.zc, uint32_t(g.fd));
     m.ok = m.x < uint32_t(g.fw) && m.y < uint32_t(g.fh);
     return m;
+}
+
+// The x/y part of vector_transformation's sums, ((x*r0 + y*r1) + z*r2) + s0
+// (numerical/transformation.hpp:58-73): x*r0 + y*r1 is the first rounded
+// partial sum, so it is formed once per column and reused for every z.
+struct ColumnSums {
+    float hx, hy, hz;
+};
+__device__ inline ColumnSums column_sums(const float* T, float fx, float fy)
+{
+    ColumnSums s;
+    s.hx = fx * T[0] + fy * T[1];
+    s.hy = fx * T[3] + fy * T[4];
+    s.hz = fx * T[6] + fy * T[7];
+    return s;
 }
 
 // Position of voxel (x, y, z) of the from grid in the to volume:
@@ -139,19 +164,25 @@ __global__ __launch_bounds__(256) void corr_sums_kernel(const uint8_t* __restric
     uint32_t sy = 0, syy = 0, sxy = 0;
     for (uint32_t it = it0 + wave; it < it1; it += nwaves) {
         const Item m = decode(g, it, lane);
-        if (m.ok) {
-            const uint32_t xv = load_u8(rf, m.z * fwh + m.y * uint32_t(g.fw) + m.x, 0);
+        if (!m.ok) continue;
+        const ColumnSums cs = column_sums(T, float(m.x), float(m.y));
+        uint32_t fidx = m.z0 * fwh + m.y * uint32_t(g.fw) + m.x;
+        for (uint32_t z = m.z0; z < m.z1; ++z, fidx += fwh) {
+            const uint32_t xv = load_u8(rf, fidx, 0);
             uint32_t yv;
             if (cr.x) {
-                int sx = int(m.x) + cr.y, s_y = int(m.y) + cr.z, sz = int(m.z) + cr.w;
+                int sx = int(m.x) + cr.y, s_y = int(m.y) + cr.z, sz = int(z) + cr.w;
                 bool in = sx >= 0 && sx < g.to.w && s_y >= 0 && s_y < g.to.h && sz >= 0 &&
                           sz < g.to.d;
                 yv = in ? load_u8(rt, uint32_t(sz) * g.to.wh + uint32_t(s_y) * uint32_t(g.to.w) +
                                           uint32_t(sx), 0)
                         : 0u;
             } else {
-                Pos p = apply(T, float(m.x), float(m.y), float(m.z));
-                yv = sample_u8_packed(r8, g.to, p.x, p.y, p.z);
+                const float fz = float(z);
+                const float px = cs.hx + fz * T[2] + T[9];
+                const float py = cs.hy + fz * T[5] + T[10];
+                const float pz = cs.hz + fz * T[8] + T[11];
+                yv = sample_u8_packed(r8, g.to, px, py, pz);
             }
             sy += yv;
             syy += yv * yv;
@@ -264,10 +295,16 @@ __global__ __launch_bounds__(MiTraits<BINS>::kThreads) void mi_hist_kernel(
     const uint32_t fwh = uint32_t(g.fw) * uint32_t(g.fh);
     for (uint32_t it = it0 + wave; it < it1; it += nwaves) {
         const Item m = decode(g, it, lane);
-        if (m.ok) {
-            const uint32_t fv = load_u8(rf, m.z * fwh + m.y * uint32_t(g.fw) + m.x, 0);
-            Pos p = apply(T, float(m.x), float(m.y), float(m.z));
-            const uint32_t tv = sample_u8_packed(r8, g.to, p.x, p.y, p.z);
+        if (!m.ok) continue;
+        const ColumnSums cs = column_sums(T, float(m.x), float(m.y));
+        uint32_t fidx = m.z0 * fwh + m.y * uint32_t(g.fw) + m.x;
+        for (uint32_t z = m.z0; z < m.z1; ++z, fidx += fwh) {
+            const uint32_t fv = load_u8(rf, fidx, 0);
+            const float fz = float(z);
+            const float px = cs.hx + fz * T[2] + T[9];
+            const float py = cs.hy + fz * T[5] + T[10];
+            const float pz = cs.hz + fz * T[8] + T[11];
+            const uint32_t tv = sample_u8_packed(r8, g.to, px, py, pz);
             const uint32_t bin = ((fv >> Tr::kShift) << Tr::kBand) | (tv >> Tr::kShift);
             const uint32_t b0 = __builtin_amdgcn_readfirstlane(bin);
             const bool same = bin == b0;
@@ -454,7 +491,8 @@ void launch_corr_sums(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhi
     volume_ensure_packed(ctx, const_cast<tiplhip_volume*>(to));
     ctx->d_sums.reserve(sizeof(unsigned long long) * 4 * size_t(K));
     zero_u64(ctx, ctx->d_sums.p, 4 * size_t(K));
-    BatchGeo g = make_batch(from, to, K, 4096u, uint32_t(ctx->cu_count) * 8u);
+    // per-lane uint32 sums of squares stay below 2^32 for < 65536 voxels a lane
+    BatchGeo g = make_batch(from, to, K, 1u << 23, uint32_t(ctx->cu_count) * 8u);
     timing_begin(ctx);
     hipLaunchKernelGGL(corr_sums_kernel, dim3(g.total_blocks), dim3(256), 0, ctx->stream, from->d,
                        to->d, to->d_pack, ctx->d_T.as<float>(), ctx->d_aux.as<int4>(), g,
@@ -473,12 +511,12 @@ void launch_mi_hist(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_
     zero_bytes16(ctx, ctx->d_hist.p, hb);
     timing_begin(ctx);
     if (bins == 256) {
-        // 1023 segments of <= 64 voxels keep every uint16 LDS counter below 65536.
-        BatchGeo g = make_batch(from, to, K, 1023u, uint32_t(ctx->cu_count) * 4u);
+        // at most 65535 voxels per workgroup keeps every uint16 LDS counter exact
+        BatchGeo g = make_batch(from, to, K, 65535u, uint32_t(ctx->cu_count) * 4u);
         hipLaunchKernelGGL(mi_hist_kernel<256>, dim3(g.total_blocks), dim3(1024), 0, ctx->stream,
                            from->d, to->d_pack, ctx->d_T.as<float>(), g, ctx->d_hist.as<int>());
     } else {
-        BatchGeo g = make_batch(from, to, K, 4096u, uint32_t(ctx->cu_count) * 8u);
+        BatchGeo g = make_batch(from, to, K, 1u << 23, uint32_t(ctx->cu_count) * 8u);
         hipLaunchKernelGGL(mi_hist_kernel<64>, dim3(g.total_blocks), dim3(256), 0, ctx->stream,
                            from->d, to->d_pack, ctx->d_T.as<float>(), g, ctx->d_hist.as<int>());
     }

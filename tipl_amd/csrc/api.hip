@@ -251,6 +251,22 @@ __global__ __launch_bounds__(256) void pack8_kernel(const uint8_t* __restrict__ 
     }
 }
 
+__global__ __launch_bounds__(256) void stream_copy_kernel(const uint4* __restrict__ in,
+                                                         uint4* __restrict__ out, size_t n)
+{
+    // four 16-byte loads in flight per thread, then four stores
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        const uint4 a = in[i], b = in[i + stride], c = in[i + 2 * stride], d = in[i + 3 * stride];
+        out[i] = a;
+        out[i + stride] = b;
+        out[i + 2 * stride] = c;
+        out[i + 3 * stride] = d;
+    }
+    for (; i < n; i += stride) out[i] = in[i];
+}
+
 int grid_for(size_t n, int cap = 16384)
 {
     return int(std::max<size_t>(1, std::min<size_t>((n + 255) / 256, size_t(cap))));
@@ -525,6 +541,41 @@ int tiplhip_timer_stop(tiplhip_ctx* ctx, float* ms)
         TIPLHIP_CHECK(hipEventRecord(ctx->t1, ctx->stream));
         TIPLHIP_CHECK(hipEventSynchronize(ctx->t1));
         TIPLHIP_CHECK(hipEventElapsedTime(ms, ctx->t0, ctx->t1));
+    });
+}
+
+int tiplhip_stream_copy(tiplhip_ctx* ctx, size_t bytes, int reps, double* gbps)
+{
+    return guarded([&] {
+        TIPLHIP_REQUIRE(ctx && gbps, "null argument");
+        TIPLHIP_REQUIRE(bytes >= 16 && reps >= 1, "need bytes >= 16 and reps >= 1");
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        DeviceGuard dg(ctx->device);
+        const size_t n = bytes / 16;
+        void *a = nullptr, *b = nullptr;
+        TIPLHIP_CHECK(hipMalloc(&a, n * 16));
+        hipError_t e = hipMalloc(&b, n * 16);
+        if (e != hipSuccess) {
+            (void)hipFree(a);
+            TIPLHIP_CHECK(e);
+        }
+        const int blocks = int(std::min<size_t>((n + 255) / 256, size_t(ctx->cu_count) * 16));
+        auto launch = [&] {
+            hipLaunchKernelGGL(stream_copy_kernel, dim3(blocks), dim3(256), 0, ctx->stream,
+                               static_cast<const uint4*>(a), static_cast<uint4*>(b), n);
+        };
+        (void)hipMemsetAsync(a, 1, n * 16, ctx->stream);
+        launch();  // warm up
+        (void)hipEventRecord(ctx->t0, ctx->stream);
+        for (int r = 0; r < reps; ++r) launch();
+        (void)hipEventRecord(ctx->t1, ctx->stream);
+        e = hipEventSynchronize(ctx->t1);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->t0, ctx->t1);
+        (void)hipFree(a);
+        (void)hipFree(b);
+        TIPLHIP_CHECK(e);
+        *gbps = 2.0 * double(n * 16) * reps / (double(ms) * 1e-3) / 1e9;
     });
 }
 

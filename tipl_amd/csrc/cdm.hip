@@ -95,7 +95,9 @@ struct Tri {
 
 __device__ inline bool locate(const FieldGeo& g, float x, float y, float z, Tri& t)
 {
-    if (!(x >= 0.0f && y >= 0.0f && z >= 0.0f && x < g.wm1 && y < g.hm1 && z < g.dm1)) return false;
+    // min(x,y,z) >= 0 is x,y,z >= 0 (a NaN coordinate fails its upper test)
+    const bool in = (fminf(fminf(x, y), z) >= 0.0f) & (x < g.wm1) & (y < g.hm1) & (z < g.dm1);
+    if (!in) return false;
     const uint32_t ix = uint32_t(x), iy = uint32_t(y), iz = uint32_t(z);
     const float p0 = x - float(ix), p1 = y - float(iy), p2 = z - float(iz);
     const float n0 = 1.0f - p0, n1 = 1.0f - p1, n2 = 1.0f - p2;
@@ -854,7 +856,7 @@ struct CdmRun {
     tiplhip_ctx* ctx;
     tiplhip_cdm_param param;
     int nmod;
-    const volatile int* terminated;
+    const volatile uint8_t* terminated;
     int iterations = 0;
     bool no_early_exit = false;  // timing mode: run max_iterations exactly
     bool stopped = false;        // terminated by the caller
@@ -951,6 +953,8 @@ bool cdm_level(CdmRun& run, const std::vector<const uint8_t*>& It,
         launch_accumulate(ctx, cur.as<float>(), solved, k, g);
         if (cur_smoothing != 0.0f) launch_smooth(ctx, solved, cur.as<float>(), cur_smoothing, g);
     }
+    // timing mode reports the running field after the fixed iteration count
+    if (run.no_early_exit) copy_d2d(ctx, best_d, cur.p, fb);
     TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
     return true;
 }
@@ -993,7 +997,7 @@ void tiplhip_cdm_param_default(tiplhip_cdm_param* p)
 }
 
 int tiplhip_cdm(tiplhip_ctx* ctx, const tiplhip_volume* const* It, const tiplhip_volume* const* Is,
-                int n, const tiplhip_cdm_param* param, float* field, const volatile int* terminated,
+                int n, const tiplhip_cdm_param* param, float* field, const volatile uint8_t* terminated,
                 int* iterations)
 {
     return guarded([&] {

@@ -42,8 +42,8 @@ struct VolGeo {
 __device__ inline uint32_t sample_u8(__amdgpu_buffer_rsrc_t r, const VolGeo& g, float x, float y,
                                      float z)
 {
-    if (!(x >= 0.0f && y >= 0.0f && z >= 0.0f && x < g.wm1 && y < g.hm1 && z < g.dm1))
-        return 0u;
+    const bool in = (fminf(fminf(x, y), z) >= 0.0f) & (x < g.wm1) & (y < g.hm1) & (z < g.dm1);
+    if (!in) return 0u;
     uint32_t ix = uint32_t(x), iy = uint32_t(y), iz = uint32_t(z);
     float p0 = x - float(ix);
     float p1 = y - float(iy);
@@ -83,8 +83,9 @@ __device__ inline uint32_t sample_u8(__amdgpu_buffer_rsrc_t r, const VolGeo& g, 
 __device__ inline uint32_t sample_u8_packed(__amdgpu_buffer_rsrc_t r8, const VolGeo& g, float x,
                                             float y, float z)
 {
-    if (!(x >= 0.0f && y >= 0.0f && z >= 0.0f && x < g.wm1 && y < g.hm1 && z < g.dm1))
-        return 0u;
+    // min(x,y,z) >= 0 is x,y,z >= 0 (a NaN coordinate fails its upper test)
+    const bool in = (fminf(fminf(x, y), z) >= 0.0f) & (x < g.wm1) & (y < g.hm1) & (z < g.dm1);
+    if (!in) return 0u;
     const uint32_t ix = uint32_t(x), iy = uint32_t(y), iz = uint32_t(z);
     const float p0 = x - float(ix);
     const float p1 = y - float(iy);
