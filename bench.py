@@ -51,10 +51,11 @@ def time_steps(ctx, fn, steps, warmup):
     return time.perf_counter() - t0
 
 
-def cpu_baseline(F, M, Ts, seconds_target=12.0):
+def cpu_baseline(F, M, Ts, seconds_target=15.0):
     """The reference's CPU MI cost (oracle/_ref, 256 bins) on a bounded
     sample of the same candidate set, candidates spread over host threads
-    as the reference optimisers do (each evaluation on one thread)."""
+    as the reference optimisers do (each evaluation on one thread); the
+    64-candidate set is repeated until about `seconds_target` of CPU work."""
     from oracle.bindings import load_oracle, load_reference
     threads = max(1, min(16, os.cpu_count() or 1))
     ref = load_reference()
@@ -64,28 +65,29 @@ def cpu_baseline(F, M, Ts, seconds_target=12.0):
         t0 = time.perf_counter()
         ref.cost_batch("mi", F, M, Ts[:threads])
         one = time.perf_counter() - t0
-        rounds = max(1, int(seconds_target / max(one, 1e-3)))
-        K = min(len(Ts), threads * rounds)
+        reps = max(1, int(round(seconds_target * threads / max(one * len(Ts), 1e-3))))
+        T = np.tile(np.asarray(Ts, np.float32).reshape(-1, 12), (reps, 1))
         t0 = time.perf_counter()
-        ref.cost_batch("mi", F, M, Ts[:K])
+        ref.cost_batch("mi", F, M, T)
         dt = time.perf_counter() - t0
+        K = len(T)
     else:
         orc = load_oracle()
         kind, threads = "port", 1
         t0 = time.perf_counter()
         K = 0
-        while time.perf_counter() - t0 < seconds_target and K < len(Ts):
-            orc.mi_cost(F, M, Ts[K])
+        while time.perf_counter() - t0 < seconds_target:
+            orc.mi_cost(F, M, Ts[K % len(Ts)])
             K += 1
         dt = time.perf_counter() - t0
     evals = K / dt
     return {"value": round(F.size * evals / 1e9, 4), "unit": "GVox/s", "cores": threads,
             "kind": kind, "evals_per_s": round(evals, 3),
-            "sample": f"{K} MI (256-bin) evaluations of the 182x218x182 pair, "
-                      f"{threads} threads, {dt:.1f} s"}
+            "sample": f"{K} MI (256-bin) evaluations of the 182x218x182 pair "
+                      f"({len(Ts)} candidates repeated), {threads} threads, {dt:.1f} s"}
 
 
-def cpu_corr_baseline(F, M, Ts, seconds_target=6.0):
+def cpu_corr_baseline(F, M, Ts, seconds_target=12.0):
     """The reference's CPU correlation cost (resample + int64 sums) on a bounded sample."""
     from oracle.bindings import load_reference
     ref = load_reference()
@@ -96,15 +98,27 @@ def cpu_corr_baseline(F, M, Ts, seconds_target=6.0):
     t0 = time.perf_counter()
     ref.cost_batch("corr", F, M, Ts[:threads])
     one = time.perf_counter() - t0
-    reps = max(1, int(seconds_target / max(one, 1e-3)))
+    reps = max(1, int(round(seconds_target * threads / max(one * len(Ts), 1e-3))))
+    T = np.tile(np.asarray(Ts, np.float32).reshape(-1, 12), (reps, 1))
     t0 = time.perf_counter()
-    for _ in range(reps):
-        ref.cost_batch("corr", F, M, Ts[:threads])
+    ref.cost_batch("corr", F, M, T)
     dt = time.perf_counter() - t0
-    evals = reps * threads / dt
+    evals = len(T) / dt
     return {"value": round(F.size * evals / 1e9, 4), "unit": "GVox/s", "cores": threads,
             "kind": "reference", "evals_per_s": round(evals, 3),
-            "sample": f"{reps * threads} correlation evaluations, {threads} threads, {dt:.1f} s"}
+            "sample": f"{len(T)} correlation evaluations ({len(Ts)} candidates repeated), "
+                      f"{threads} threads, {dt:.1f} s"}
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` as the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes measured them (calibrated against the copy kernel), or
+    None when no such profile is committed."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            return json.load(f)[kernel]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def cpu_cdm_baseline(It, Is, seconds_target=12.0):
@@ -124,6 +138,74 @@ def cpu_cdm_baseline(It, Is, seconds_target=12.0):
                       f"{threads} threads, {dt:.1f} s"}
 
 
+SERVE_SIZES = [(96, 96, 96), (128, 128, 128), (182, 218, 182), (256, 256, 256)]
+
+
+def serving_request(i):
+    """Request i of the serving workload: size from SERVE_SIZES, 70% affine /
+    30% affine+CDM, int16 or float32, pair generated from seed 4000 + i."""
+    from tipl_amd import phantoms
+    rng = np.random.default_rng(4000 + i)
+    shape = SERVE_SIZES[int(rng.integers(0, len(SERVE_SIZES)))]
+    nonlinear = bool(rng.random() < 0.3)
+    dtype = np.int16 if rng.random() < 0.5 else np.float32
+    fixed, moving, _ = phantoms.serving_pair(4000 + i, shape, nonlinear, dtype)
+    return fixed, moving, nonlinear
+
+
+def make_serving_requests(n):
+    """Generated in worker processes before the GPU is touched (fork start)."""
+    import multiprocessing as mp
+    with mp.get_context("fork").Pool(min(8, n)) as pool:
+        return pool.map(serving_request, range(n))
+
+
+def serving_bench(reqs, slots, rate_factor, seed=4000):
+    """The plain single-pair path (one registration at a time, tiplhip_register)
+    against the serving loop fed by a Poisson stream at rate_factor x the
+    plain path's rate; every served result is compared with the plain one."""
+    import tipl_amd
+    ctx = tipl_amd.Context(0)
+    tipl_amd.register(ctx, reqs[0][0], reqs[0][1])  # warm-up (code objects, pools)
+    plain = []
+    t0 = time.perf_counter()
+    for f, m, nl in reqs:
+        plain.append(tipl_amd.register(ctx, f, m, nonlinear=nl, want_field=False))
+    plain_dt = time.perf_counter() - t0
+    plain_rate = len(reqs) / plain_dt
+    ctx.close()
+    rate = rate_factor * plain_rate
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
+    with tipl_amd.Server(0, slots=slots, hbm_budget=192 << 30) as srv:
+        start = time.perf_counter()
+        tickets = []
+        for (f, m, nl), at in zip(reqs, arrivals):
+            delay = start + at - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            tickets.append(srv.submit(f, m, nonlinear=nl, want_field=False))
+        served = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    same = all(np.array_equal(a["T"].view(np.uint32), b["T"].view(np.uint32)) and
+               a["evaluations"] == b["evaluations"] and a["cdm_iterations"] == b["cdm_iterations"]
+               for a, b in zip(served, plain))
+    return {
+        "served_regs_per_s": round(st["regs_per_s"], 3),
+        "latency_p50_ms": round(st["latency_p50_ms"], 1),
+        "latency_p95_ms": round(st["latency_p95_ms"], 1),
+        "latency_p99_ms": round(st["latency_p99_ms"], 1),
+        "plain_regs_per_s": round(plain_rate, 3),
+        "served_over_plain": round(st["regs_per_s"] / plain_rate, 2),
+        "offered_rate_per_s": round(rate, 3),
+        "requests": len(reqs), "slots": slots,
+        "nonlinear_requests": int(sum(nl for _, _, nl in reqs)),
+        "batch_launches": int(st["batch_launches"]), "batched_pairs": int(st["batched_pairs"]),
+        "hbm_peak_gib": round(st["hbm_peak"] / 2 ** 30, 2),
+        "results_equal_plain": bool(same),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,11 +213,21 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cdm-iters", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--serve-requests", type=int, default=24,
+                    help="requests in the serving leg (0 skips it)")
+    ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-rate", type=float, default=4.0,
+                    help="Poisson arrival rate as a multiple of the plain path's rate")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (each pair or "
               f"candidate set fits one MI355X; sharding is described in DESIGN.md)", file=sys.stderr)
         return 2
+
+    reqs = []
+    if args.serve_requests > 0:
+        log(f"generating {args.serve_requests} serving requests")
+        reqs = make_serving_requests(args.serve_requests)
 
     import tipl_amd
     from tipl_amd import phantoms
@@ -194,9 +286,19 @@ def main():
     # --- roofline of the dominant kernel (MI 64-bin joint histogram) --------
     alg_bytes = 2.0 * N * CANDIDATES            # 1 B from + 1 B to footprint per voxel-candidate
     achieved = alg_bytes / (k_avg_ms * 1e-3) / 1e9
+    prof = profiled_traffic("mi_hist_kernel<64>")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "mi_hist_kernel<64>", "kernel_ms": round(k_avg_ms, 4)}
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": prof["bytes"] if prof else None,
+                "kernel": "mi_hist_kernel<64>", "kernel_ms": round(k_avg_ms, 4),
+                "algorithmic_bytes": alg_bytes,
+                "traffic_source": prof["source"] if prof else None}
+
+    serve = None
+    if reqs:
+        log("serving leg")
+        serve = serving_bench(reqs, args.serve_slots, args.serve_rate)
+        log(f"serving {serve}")
 
     cpu = cpu_corr = cpu_cdm = None
     if not args.no_cpu:
@@ -233,6 +335,7 @@ def main():
         "cdm_cpu_baseline": cpu_cdm,
         "corr_cpu_baseline": cpu_corr,
         "hbm_copy_gbps": round(copy_gbps, 1),
+        "serving": serve,
         **{k: round(v, 4) for k, v in res.items()},
     }
     print(json.dumps(out), flush=True)

@@ -33,9 +33,11 @@ inline int device_count()
 class context {
 public:
     explicit context(int device = 0) { check(tiplhip_ctx_create(device, &c_)); }
+    // wraps a context owned by the caller (not destroyed here)
+    explicit context(tiplhip_ctx* borrowed) : c_(borrowed), owned_(false) {}
     ~context()
     {
-        if (c_) tiplhip_ctx_destroy(c_);
+        if (c_ && owned_) tiplhip_ctx_destroy(c_);
     }
     context(const context&) = delete;
     context& operator=(const context&) = delete;
@@ -43,6 +45,7 @@ public:
 
 private:
     tiplhip_ctx* c_ = nullptr;
+    bool owned_ = true;
 };
 
 // One context per calling thread: the reference re-enters its cost functors

@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <vector>
@@ -58,6 +59,25 @@ struct linear_reg_param {
 
 namespace detail {
 
+// Evaluates K candidate affine parameter vectors (12 floats each) of one
+// modality pair: costs[k] = cost of params[k].  The default launches them on
+// the caller's context (tiplhip_cost_batch_params); a serving loop installs
+// one that merges the candidate sets of concurrent registrations into
+// multi-pair launches.  Either way each cost is the same value.
+using cost_evaluator =
+    std::function<void(const tiplhip_volume* from, const float* from_vs, const tiplhip_volume* to,
+                       const float* to_vs, const float* params, int K, int cost, int bins,
+                       double* costs)>;
+
+inline cost_evaluator direct_evaluator(hip::context& ctx)
+{
+    tiplhip_ctx* c = ctx.get();
+    return [c](const tiplhip_volume* from, const float* from_vs, const tiplhip_volume* to,
+               const float* to_vs, const float* params, int K, int cost, int bins, double* costs) {
+        hip::check(tiplhip_cost_batch_params(c, from, from_vs, to, to_vs, params, K, cost, bins, costs));
+    };
+}
+
 // The device state of one registration: volumes per modality, voxel sizes.
 struct linear_level {
     std::vector<std::shared_ptr<hip::volume>> from, to;
@@ -87,8 +107,8 @@ public:
     cost_type cost = mutual_info;
     int bins = 256;
 
-    runner(hip::context& ctx, linear_level lv, affine_param<float, 3>& arg)
-        : arg_min(arg), ctx_(ctx), lv_(std::move(lv)) {}
+    runner(cost_evaluator eval, linear_level lv, affine_param<float, 3>& arg)
+        : arg_min(arg), eval_(std::move(eval)), lv_(std::move(lv)) {}
 
     std::pair<affine_param<float, 3>, affine_param<float, 3>> current_bound(reg_type type) const
     {
@@ -151,9 +171,8 @@ public:
         const size_t nmod = std::min(lv_.from.size(), lv_.to.size());
         const int kind = cost == mutual_info ? TIPLHIP_COST_MI : TIPLHIP_COST_CORR;
         for (size_t i = 0; i < nmod; ++i) {
-            hip::check(tiplhip_cost_batch_params(ctx_.get(), lv_.from[i]->get(), lv_.from_vs.begin(),
-                                                 lv_.to[i]->get(), lv_.to_vs.begin(), P.data(),
-                                                 int(xs.size()), kind, bins, c.data()));
+            eval_(lv_.from[i]->get(), lv_.from_vs.begin(), lv_.to[i]->get(), lv_.to_vs.begin(),
+                  P.data(), int(xs.size()), kind, bins, c.data());
             for (size_t k = 0; k < xs.size(); ++k) total[k] += c[k];
         }
         for (double& v : total) v /= double(float(nmod) * 2.0f);
@@ -200,7 +219,7 @@ public:
         const size_t sz = lv_.from[0]->shape().size();
         const bool is_large = double(sz) > std::pow(96.0, 3);
         if (is_large) {
-            runner low(ctx_, lv_.down(), arg_min);
+            runner low(eval_, lv_.down(), arg_min);
             low.arg_upper = arg_upper;
             low.arg_lower = arg_lower;
             low.search_count = search_count;
@@ -219,7 +238,7 @@ public:
     size_t evaluations = 0;
 
 private:
-    hip::context& ctx_;
+    cost_evaluator eval_;
     linear_level lv_;
 };
 
@@ -367,6 +386,62 @@ void estimate_affine_param(const I& src, const vector<3>& svs, const I& tgt, con
     }
 }
 
+// The body of tipl::reg::linear (reg/linear.hpp:622-700) on volumes already on
+// the device.  from / to are the host uint8 images (has_mask and the moment
+// estimate read them); dev_from / dev_to are the same images on the device.
+template <typename out_type = void>
+float linear_impl(const std::vector<const_pointer_image<3, unsigned char>>& from, vector<3> from_vs,
+                  const std::vector<const_pointer_image<3, unsigned char>>& to, vector<3> to_vs,
+                  const std::vector<std::shared_ptr<hip::volume>>& dev_from,
+                  const std::vector<std::shared_ptr<hip::volume>>& dev_to,
+                  affine_param<float, 3>& arg, linear_reg_param param, bool& terminated,
+                  const cost_evaluator& eval, size_t* evaluations = nullptr)
+{
+    auto term = [&] { return bool(terminated); };
+    auto run = [&](const vector<3>& f_vs, const vector<3>& t_vs, affine_param<float, 3>& a, bool mr) {
+        linear_level lv;
+        lv.from = dev_from;
+        lv.to = dev_to;
+        lv.from_vs = f_vs;
+        lv.to_vs = t_vs;
+        runner r(eval, std::move(lv), a);
+        r.set_bound(param.reg_type, param.bound, param.absolute_bound);
+        r.search_count = param.search_count;
+        r.cost = param.cost_type;
+        r.bins = param.mi_bins;
+        if constexpr (!std::is_void_v<out_type>)
+            out_type() << (param.reg_type == affine ? "affine" : "rigid body") << " registration by "
+                       << (param.cost_type == mutual_info ? "mutual info" : "correlation") << " using gpu";
+        const float v = mr ? r.run_optimize_mr(term) : r.run_optimize(0, term);
+        if (evaluations) *evaluations += r.evaluations;
+        return v;
+    };
+
+    if (param.bound == narrow_bound) {
+        param.absolute_bound = false;
+        param.search_count = 0;
+        return run(from_vs, to_vs, arg, false);
+    }
+    vector<3> new_to_vs = to_vs;
+    affine_param<float, 3> surrogate = arg;
+    bool adjust_vs = false;
+    if (param.reg_type == affine && has_mask(from[0]) && has_mask(to[0])) {
+        estimate_affine_param(from[0], from_vs, to[0], to_vs, surrogate);
+        arg = surrogate;
+        for (int i = 0; i < 3; ++i) {
+            new_to_vs[i] = to_vs[i] / std::max<float>(surrogate.scaling[i], 0.01f);
+            surrogate.scaling[i] = 1.0f;
+        }
+        adjust_vs = true;
+    }
+    param.absolute_bound = true;
+    const float result = run(from_vs, new_to_vs, adjust_vs ? surrogate : arg, true);
+    if (adjust_vs)
+        arg = transformation_matrix<float, 3>(surrogate, from[0].shape(), from_vs, to[0].shape(), new_to_vs)
+                  .to_affine_param(from[0].shape(), from_vs, to[0].shape(), to_vs);
+    return result;
+}
+
 }  // namespace detail
 
 // tipl::reg::linear (reg/linear.hpp:622-700): registers `from` onto `to`
@@ -381,48 +456,11 @@ float linear(std::vector<const_pointer_image<dim, unsigned char>> from, vector<d
     if (from.empty() || to.empty()) throw std::invalid_argument("linear: empty image list");
     if (!param.cuda) throw std::runtime_error("linear: only the GPU path is provided (param.cuda)");
     hip::context& ctx = hip::thread_context();
-    auto term = [&] { return bool(terminated); };
-
-    auto run = [&](const vector<3>& f_vs, const vector<3>& t_vs, affine_param<float, 3>& a, bool mr) {
-        detail::linear_level lv;
-        for (auto& f : from) lv.from.push_back(std::make_shared<hip::volume>(ctx, f.data(), f.shape()));
-        for (auto& t : to) lv.to.push_back(std::make_shared<hip::volume>(ctx, t.data(), t.shape()));
-        lv.from_vs = f_vs;
-        lv.to_vs = t_vs;
-        detail::runner r(ctx, std::move(lv), a);
-        r.set_bound(param.reg_type, param.bound, param.absolute_bound);
-        r.search_count = param.search_count;
-        r.cost = param.cost_type;
-        r.bins = param.mi_bins;
-        if constexpr (!std::is_void_v<out_type>)
-            out_type() << (param.reg_type == affine ? "affine" : "rigid body") << " registration by "
-                       << (param.cost_type == mutual_info ? "mutual info" : "correlation") << " using gpu";
-        return mr ? r.run_optimize_mr(term) : r.run_optimize(0, term);
-    };
-
-    if (param.bound == narrow_bound) {
-        param.absolute_bound = false;
-        param.search_count = 0;
-        return run(from_vs, to_vs, arg, false);
-    }
-    vector<3> new_to_vs = to_vs;
-    affine_param<float, 3> surrogate = arg;
-    bool adjust_vs = false;
-    if (param.reg_type == affine && detail::has_mask(from[0]) && detail::has_mask(to[0])) {
-        detail::estimate_affine_param(from[0], from_vs, to[0], to_vs, surrogate);
-        arg = surrogate;
-        for (int i = 0; i < 3; ++i) {
-            new_to_vs[i] = to_vs[i] / std::max<float>(surrogate.scaling[i], 0.01f);
-            surrogate.scaling[i] = 1.0f;
-        }
-        adjust_vs = true;
-    }
-    param.absolute_bound = true;
-    const float result = run(from_vs, new_to_vs, adjust_vs ? surrogate : arg, true);
-    if (adjust_vs)
-        arg = transformation_matrix<float, 3>(surrogate, from[0].shape(), from_vs, to[0].shape(), new_to_vs)
-                  .to_affine_param(from[0].shape(), from_vs, to[0].shape(), to_vs);
-    return result;
+    std::vector<std::shared_ptr<hip::volume>> df, dt;
+    for (auto& f : from) df.push_back(std::make_shared<hip::volume>(ctx, f.data(), f.shape()));
+    for (auto& t : to) dt.push_back(std::make_shared<hip::volume>(ctx, t.data(), t.shape()));
+    return detail::linear_impl<out_type>(from, from_vs, to, to_vs, df, dt, arg, param, terminated,
+                                         detail::direct_evaluator(ctx));
 }
 
 // The overload returning the transformation (reg/linear.hpp:703-714).

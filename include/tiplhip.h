@@ -156,6 +156,25 @@ int tiplhip_mi_hist(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_
 int tiplhip_cost_batch(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_volume* to,
                        const float* T, int K, int cost, int bins, double* costs);
 
+/* One image pair's candidates in a multi-pair launch. */
+typedef struct tiplhip_pair_costs {
+    const tiplhip_volume* from;
+    const tiplhip_volume* to;
+    const float* T;          /* K transforms, 12 floats each, host memory */
+    int K;
+    double* costs;           /* K costs out, host memory */
+} tiplhip_pair_costs;
+
+/* tiplhip_cost_batch for n image pairs of any sizes in one launch (all
+ * candidates of all pairs, at most 4096, share one histogram or sum kernel
+ * and one finishing kernel).  Each cost equals what tiplhip_cost_batch
+ * returns for that pair alone.  Volumes may come from any context of the
+ * calling context's device.  No reference counterpart: the reference
+ * evaluates one candidate of one pair per call (reg/linear.hpp:40-78,
+ * 205-252). */
+int tiplhip_cost_batch_pairs(tiplhip_ctx* ctx, const tiplhip_pair_costs* pairs, int n, int cost,
+                             int bins);
+
 /* The same with the K transforms computed on the host from affine
  * parameters (transformation_matrix(arg, from, from_vs, to, to_vs),
  * numerical/transformation.hpp:612-666). */
@@ -220,6 +239,95 @@ int tiplhip_cdm_smooth(tiplhip_ctx* ctx, const float* d, float* dd, const int di
                        float smoothing);                                 /* reg/cdm.hpp:302-342 */
 int tiplhip_upsample_field(tiplhip_ctx* ctx, const float* in, const int in_dim[3], float* out,
                            const int out_dim[3]);                        /* numerical/resampling.hpp:849-858 */
+
+/* tipl::filter::gaussian(image, bandwidth) (filter/gaussian.hpp:18-105), in
+ * place on a host float image of dim[0]*dim[1]*dim[2] voxels with `channels`
+ * interleaved floats each (1: scalar image, 3: vector<3> field).  Bit-exact:
+ * each [1 2 1] pass is rounded to float as the reference's pass arrays are.
+ * bandwidth 0 leaves the image unchanged; bandwidth above 16 is rejected. */
+int tiplhip_gaussian(tiplhip_ctx* ctx, float* data, const int dim[3], int channels,
+                     unsigned bandwidth);
+
+/* tipl::resample (numerical/resampling.hpp:1239-1299) of a device volume
+ * into a new device volume of shape out_dim: out(p) = src(T p), trilinear,
+ * uint8 truncation; with crop_shortcut, an integer-shift T is an exact crop
+ * as on the CPU path.  The result lives on ctx. */
+int tiplhip_volume_resample(tiplhip_ctx* ctx, const tiplhip_volume* src, const int out_dim[3],
+                            const float T[12], int crop_shortcut, tiplhip_volume** out);
+
+/* ---------------------------------------------------------------------------
+ * Registration of one image pair: the reference's mm_reg order
+ * (reg/mm_reg.hpp:348-416) -- linear(fixed, moving) and, when nonlinear is
+ * set, the moving image resampled onto the fixed grid and cdm(fixed, J).
+ * fixed plays `from` / It, moving plays `to` / Is.  Inputs of type int16 or
+ * float32 are quantised on the device (normalize_upper_lower2, upper 255.99).
+ * ------------------------------------------------------------------------- */
+typedef struct tiplhip_reg_request {
+    const void* fixed;        /* host voxels, x fastest */
+    int fixed_dtype;          /* TIPLHIP_U8 / TIPLHIP_I16 / TIPLHIP_F32 */
+    int fixed_dim[3];
+    float fixed_vs[3];
+    const void* moving;
+    int moving_dtype;
+    int moving_dim[3];
+    float moving_vs[3];
+    int cost;                 /* TIPLHIP_COST_MI (default) or TIPLHIP_COST_CORR */
+    int bins;                 /* MI bins: 256 (reference) or 64 */
+    int nonlinear;            /* 0: affine only; 1: affine then CDM */
+    tiplhip_cdm_param cdm;
+    float* field;             /* host, vox(fixed_dim) x 3 floats, or NULL */
+} tiplhip_reg_request;
+
+typedef struct tiplhip_reg_result {
+    int status;               /* TIPLHIP_OK or the error code of the failing step */
+    float affine[12];         /* affine_param found by linear(): t, r, s, a */
+    float T[12];              /* transformation_matrix: fixed voxel -> moving voxel */
+    float cost;               /* linear()'s return value */
+    int cdm_iterations;
+    int64_t evaluations;      /* affine cost evaluations */
+    double queue_ms;          /* submit to start (served requests; 0 on the plain path) */
+    double run_ms;            /* start to done */
+} tiplhip_reg_result;
+
+void tiplhip_reg_request_default(tiplhip_reg_request* r);
+
+/* The plain single-pair path: the whole registration on the calling thread
+ * with ctx, each optimiser candidate set in its own launch.  The caller's
+ * input and field buffers are used in place. */
+int tiplhip_register(tiplhip_ctx* ctx, const tiplhip_reg_request* req, tiplhip_reg_result* res);
+
+/* ---------------------------------------------------------------------------
+ * Serving loop (no reference counterpart; the reference registers one pair
+ * per call).  `slots` worker threads, each with its own context (stream), run
+ * registrations concurrently; a request is admitted when a slot is free and
+ * its estimated device memory fits in hbm_budget bytes (FIFO).  The affine
+ * cost evaluations of all running registrations are merged by one batcher
+ * into multi-pair launches (tiplhip_cost_batch_pairs) as they arrive, so
+ * requests join and leave the batch at every optimiser step.  Each result
+ * equals what tiplhip_register returns for that request alone.
+ * A request's buffers must stay valid until tiplhip_server_wait returns it.
+ * ------------------------------------------------------------------------- */
+typedef struct tiplhip_server tiplhip_server;
+
+typedef struct tiplhip_server_stats {
+    int64_t submitted, completed, failed, running, queued;
+    int64_t batch_launches;   /* multi-pair cost launches */
+    int64_t batched_pairs;    /* pair segments over those launches */
+    size_t hbm_in_use, hbm_peak;
+    double latency_p50_ms, latency_p95_ms, latency_p99_ms;  /* submit to done */
+    double wall_s;            /* first submit to last completion */
+    double regs_per_s;        /* completed / wall_s */
+} tiplhip_server_stats;
+
+int tiplhip_server_create(int device, int slots, size_t hbm_budget, tiplhip_server** out);
+/* Estimated device bytes a request holds while it runs (what admission counts). */
+size_t tiplhip_request_bytes(const tiplhip_reg_request* req);
+int tiplhip_server_submit(tiplhip_server* s, const tiplhip_reg_request* req, uint64_t* ticket);
+/* Blocks until the ticket is done; returns its status and forgets the ticket. */
+int tiplhip_server_wait(tiplhip_server* s, uint64_t ticket, tiplhip_reg_result* res);
+int tiplhip_server_stats_get(tiplhip_server* s, tiplhip_server_stats* st);
+/* Finishes every submitted request, then stops the workers. */
+int tiplhip_server_destroy(tiplhip_server* s);
 
 #ifdef __cplusplus
 }

@@ -79,6 +79,7 @@ class Oracle(_Lib):
         self._acc = self._fn("orc_accumulate_displacement", None, _f32p, _f32p, _i32p)
         self._smooth = self._fn("orc_cdm_smooth", None, _f32p, _f32p, _i32p, C.c_float)
         self._improved = self._fn("orc_cdm_improved", C.c_int, _f32p, P(C.c_int))
+        self._gauss = self._fn("orc_gaussian_f32", None, _f32p, _i32p, C.c_int, C.c_uint)
         self._cdm = self._fn("orc_cdm", C.c_int, P(C.c_void_p), P(C.c_void_p), C.c_int, _i32p,
                              C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, C.c_char, _f32p,
                              P(C.c_int))
@@ -206,6 +207,14 @@ class Oracle(_Lib):
         self._smooth(d, dd, np.array(d.shape[:3][::-1], np.int32), s)
         return dd
 
+    def gaussian(self, img, bandwidth=1) -> np.ndarray:
+        """filter::gaussian on a float32 (z, y, x) image or (z, y, x, 3) field; returns a copy."""
+        out = np.array(img, np.float32, order="C")
+        ch = 3 if out.ndim == 4 else 1
+        assert out.ndim == 3 or out.shape[3] == 3, "scalar image or vector<3> field"
+        self._gauss(out.reshape(-1), np.array(out.shape[:3][::-1], np.int32), ch, int(bandwidth))
+        return out
+
     def improved(self, cost):
         c = np.array(list(cost) + [0.0] * 2, np.float32)
         n = C.c_int(len(cost))
@@ -259,6 +268,7 @@ class Reference(_Lib):
                              C.c_float, C.c_float, C.c_float, C.c_int, C.c_char, _f32p)
         self._linear = self._fn("tref_linear", C.c_float, _u8p, _i32p, _f32p, _u8p, _i32p, _f32p,
                                 C.c_int, C.c_int, _f32p)
+        self._gauss = self._fn("tref_gaussian_f32", None, _f32p, _i32p, C.c_int, C.c_uint)
         self._cdm_iters = self._fn("tref_cdm_iterations", C.c_double, _u8p, _u8p, _i32p, C.c_int,
                                    C.c_void_p)
 
@@ -379,6 +389,13 @@ class Reference(_Lib):
         dd = _f3(dd).copy()
         self._smooth(d, dd, np.array(d.shape[:3][::-1], np.int32), s)
         return dd
+
+    def gaussian(self, img, bandwidth=1):
+        out = np.array(img, np.float32, order="C")
+        ch = 3 if out.ndim == 4 else 1
+        assert out.ndim == 3 or out.shape[3] == 3, "scalar image or vector<3> field"
+        self._gauss(out.reshape(-1), np.array(out.shape[:3][::-1], np.int32), ch, int(bandwidth))
+        return out
 
     def improved(self, cost):
         c = np.array(list(cost) + [0.0] * 2, np.float32)

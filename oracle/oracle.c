@@ -955,3 +955,53 @@ ORC_API int orc_cdm(const uint8_t* const* It, const uint8_t* const* Is, int nmod
     if (iterations) *iterations = it;
     return ok;
 }
+
+/* filter::gaussian filter/gaussian.hpp:18-89 on a float image of `ch`
+ * interleaved channels (1 = scalar image, 3 = vector<3> field): `bandwidth`
+ * rounds of the separable [1 2 1] stencil along x, then y, then z, each pass
+ * written to its own buffer as b = (a[left] + (c + c)) + a[right] with the
+ * edge neighbour replaced by the centre (replicate padding); finally every
+ * value is scaled by 2^(-6*bandwidth).  In place on `img`. */
+ORC_API void orc_gaussian_f32(float* img, const int* dim, int ch, unsigned bandwidth)
+{
+    size_t n = vox(dim);
+    if (n == 0 || bandwidth == 0)
+        return;
+    size_t w = dim[0], h = dim[1], d = dim[2], wh = w * h;
+    float* a = (float*)malloc(sizeof(float) * n * ch);
+    float* b = (float*)malloc(sizeof(float) * n * ch);
+    memcpy(a, img, sizeof(float) * n * ch);
+    for (unsigned k = 0; k < bandwidth; ++k) {
+        for (int axis = 0; axis < 3; ++axis) {
+            for (size_t i = 0; i < n; ++i) {
+                size_t lo, hi;
+                if (axis == 0) {
+                    size_t x = i % w;
+                    lo = x ? i - 1 : i;
+                    hi = x + 1 < w ? i + 1 : i;
+                } else if (axis == 1) {
+                    size_t y = (i / w) % h;
+                    lo = y ? i - w : i;
+                    hi = y + 1 < h ? i + w : i;
+                } else {
+                    size_t z = i / wh;
+                    lo = z ? i - wh : i;
+                    hi = z + 1 < d ? i + wh : i;
+                }
+                for (int c = 0; c < ch; ++c) {
+                    float cc = a[i * ch + c];
+                    cc += cc;
+                    b[i * ch + c] = (a[lo * ch + c] + cc) + a[hi * ch + c];
+                }
+            }
+            float* t = a;
+            a = b;
+            b = t;
+        }
+    }
+    double scale = ldexp(1.0, -(int)(6 * bandwidth));
+    for (size_t j = 0; j < n * (size_t)ch; ++j)
+        img[j] = (float)((double)a[j] * scale);
+    free(a);
+    free(b);
+}

@@ -13,6 +13,7 @@
 #include <cstring>
 #include <vector>
 
+#include "filter/gaussian.hpp"
 #include "reg/cdm.hpp"
 #include "reg/linear.hpp"
 
@@ -193,6 +194,20 @@ void tref_cdm_smooth(const float* d, float* dd, const int* dim, float smoothing)
     field_t D = to_field(d, dim), DD = to_field(dd, dim);
     tipl::reg::cdm_smooth(D, DD, smoothing);
     from_field(DD, dd);
+}
+
+void tref_gaussian_f32(float* img, const int* dim, int ch, unsigned bandwidth)
+{
+    if (ch == 3) {
+        field_t F = to_field(img, dim);
+        tipl::filter::gaussian(F, bandwidth);
+        from_field(F, img);
+    } else {
+        tipl::image<3, float> I(shp(dim));
+        std::memcpy(&I[0], img, sizeof(float) * I.size());
+        tipl::filter::gaussian(I, bandwidth);
+        std::memcpy(img, &I[0], sizeof(float) * I.size());
+    }
 }
 
 int tref_cdm_improved(float* cost, int* n)

@@ -164,3 +164,36 @@ def test_affine_matrix_identity_and_shift(oracle):
     # a rotation about z by pi/2 maps x to y (reference's left-handed Euler)
     T = oracle.affine_matrix([0, 0, 0, 0, 0, math.pi / 2, 1, 1, 1, 0, 0, 0], d, (1, 1, 1), d, (1, 1, 1))
     assert np.allclose(T[:9].reshape(3, 3), [[0, 1, 0], [-1, 0, 0], [0, 0, 1]], atol=1e-6)
+
+
+def test_gaussian_closed_forms(oracle):
+    """filter::gaussian: constants are fixed points, an interior impulse
+    spreads as the outer product of binomial rows C(2b, k) / 4^b, mass is
+    kept away from the edges, variance is b/2 per axis, and the three
+    channels of a field are filtered independently."""
+    from math import comb
+    const = np.full((7, 8, 9), 37.25, np.float32)
+    for b in (1, 2, 3):
+        assert np.array_equal(oracle.gaussian(const, b), const)
+    n = 15
+    imp = np.zeros((n, n, n), np.float32)
+    imp[7, 7, 7] = 4096.0
+    for b in (1, 2, 3):
+        out = oracle.gaussian(imp, b).astype(np.float64)
+        row = np.zeros(n)
+        row[7 - b:8 + b] = [comb(2 * b, k) / 4.0 ** b for k in range(2 * b + 1)]
+        expect = 4096.0 * np.einsum("i,j,k->ijk", row, row, row)
+        assert np.allclose(out, expect, rtol=1e-6, atol=1e-9)
+        assert abs(out.sum() - 4096.0) < 1e-3
+        pos = np.arange(n) - 7.0
+        var_x = (out.sum((0, 1)) * pos ** 2).sum() / out.sum()
+        assert abs(var_x - b / 2.0) < 1e-6
+    rng = np.random.default_rng(3)
+    f = rng.standard_normal((6, 5, 4, 3)).astype(np.float32)
+    g = oracle.gaussian(f, 2)
+    for c in range(3):
+        assert np.array_equal(g[..., c], oracle.gaussian(np.ascontiguousarray(f[..., c]), 2))
+    # replicate edges: a ramp along x keeps its interior slope and flattens at the ends
+    ramp = np.broadcast_to(np.arange(9, dtype=np.float32), (3, 4, 9)).copy()
+    r = oracle.gaussian(ramp, 1)[1, 1]
+    assert np.allclose(r[1:-1], np.arange(1, 8)) and r[0] == 0.25 and r[-1] == 7.75

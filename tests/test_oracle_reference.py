@@ -133,3 +133,14 @@ def test_whole_cdm(oracle, ref, shape, kind):
     assert ok_o == ok_r
     rms = float(np.sqrt(((f_o - f_r) ** 2).sum(-1).mean()))
     assert rms <= 1e-6, rms
+
+
+@pytest.mark.parametrize("shape", [(13, 17, 11), (1, 1, 9), (9, 1, 1), (2, 3, 1), (20, 9, 14)])
+@pytest.mark.parametrize("bandwidth", [0, 1, 2, 3])
+def test_gaussian(oracle, ref, shape, bandwidth):
+    rng = np.random.default_rng(bandwidth * 100 + sum(shape))
+    img = (rng.standard_normal(shape) * 300).astype(np.float32)
+    field = rng.standard_normal(shape + (3,)).astype(np.float32)
+    assert np.array_equal(_bits(oracle.gaussian(img, bandwidth)), _bits(ref.gaussian(img, bandwidth)))
+    assert np.array_equal(_bits(oracle.gaussian(field, bandwidth)),
+                          _bits(ref.gaussian(field, bandwidth)))

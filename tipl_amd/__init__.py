@@ -33,6 +33,10 @@ EXPORTS = (
     "tiplhip_cdm_param_default", "tiplhip_cdm", "tiplhip_cdm_level_iterations",
     "tiplhip_compose_displacement", "tiplhip_cdm_gradient", "tiplhip_cdm_poisson",
     "tiplhip_accumulate_displacement", "tiplhip_cdm_smooth", "tiplhip_upsample_field",
+    "tiplhip_gaussian", "tiplhip_cost_batch_pairs", "tiplhip_volume_resample",
+    "tiplhip_reg_request_default", "tiplhip_register", "tiplhip_server_create",
+    "tiplhip_request_bytes", "tiplhip_server_submit", "tiplhip_server_wait",
+    "tiplhip_server_stats_get", "tiplhip_server_destroy",
 )
 
 
@@ -42,11 +46,38 @@ class TiplHipError(RuntimeError):
         self.code = code
 
 
+class PairCosts(C.Structure):
+    _fields_ = [("from_", C.c_void_p), ("to", C.c_void_p), ("T", C.c_void_p), ("K", C.c_int),
+                ("costs", C.c_void_p)]
+
+
 class CdmParam(C.Structure):
     """tiplhip_cdm_param (cdm_param, reg/cdm.hpp:36-44, plus the iteration cap)."""
     _fields_ = [("resolution", C.c_float), ("speed", C.c_float), ("smoothing", C.c_float),
                 ("min_dimension", C.c_int), ("max_iterations", C.c_int),
                 ("gradient_type", C.c_char * 8)]
+
+
+class RegRequest(C.Structure):
+    """tiplhip_reg_request: one image pair to register (fixed = from / It, moving = to / Is)."""
+    _fields_ = [("fixed", C.c_void_p), ("fixed_dtype", C.c_int), ("fixed_dim", C.c_int * 3),
+                ("fixed_vs", C.c_float * 3), ("moving", C.c_void_p), ("moving_dtype", C.c_int),
+                ("moving_dim", C.c_int * 3), ("moving_vs", C.c_float * 3), ("cost", C.c_int),
+                ("bins", C.c_int), ("nonlinear", C.c_int), ("cdm", CdmParam), ("field", C.c_void_p)]
+
+
+class RegResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("affine", C.c_float * 12), ("T", C.c_float * 12),
+                ("cost", C.c_float), ("cdm_iterations", C.c_int), ("evaluations", C.c_int64),
+                ("queue_ms", C.c_double), ("run_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64),
+                ("running", C.c_int64), ("queued", C.c_int64), ("batch_launches", C.c_int64),
+                ("batched_pairs", C.c_int64), ("hbm_in_use", C.c_size_t), ("hbm_peak", C.c_size_t),
+                ("latency_p50_ms", C.c_double), ("latency_p95_ms", C.c_double),
+                ("latency_p99_ms", C.c_double), ("wall_s", C.c_double), ("regs_per_s", C.c_double)]
 
 
 _lib = None
@@ -97,6 +128,17 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "tiplhip_accumulate_displacement": (ip, [vp, vp, vp, P(ip), C.c_float]),
         "tiplhip_cdm_smooth": (ip, [vp, vp, vp, P(ip), C.c_float]),
         "tiplhip_upsample_field": (ip, [vp, vp, P(ip), vp, P(ip)]),
+        "tiplhip_gaussian": (ip, [vp, vp, P(ip), ip, C.c_uint]),
+        "tiplhip_cost_batch_pairs": (ip, [vp, P(PairCosts), ip, ip, ip]),
+        "tiplhip_volume_resample": (ip, [vp, vp, P(ip), vp, ip, P(vp)]),
+        "tiplhip_reg_request_default": (None, [P(RegRequest)]),
+        "tiplhip_register": (ip, [vp, P(RegRequest), P(RegResult)]),
+        "tiplhip_server_create": (ip, [ip, ip, C.c_size_t, P(vp)]),
+        "tiplhip_request_bytes": (C.c_size_t, [P(RegRequest)]),
+        "tiplhip_server_submit": (ip, [vp, P(RegRequest), P(C.c_uint64)]),
+        "tiplhip_server_wait": (ip, [vp, C.c_uint64, P(RegResult)]),
+        "tiplhip_server_stats_get": (ip, [vp, P(ServerStats)]),
+        "tiplhip_server_destroy": (ip, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -276,6 +318,21 @@ class Context:
                                                  bins, _ptr(out)))
         return out
 
+    def cost_batch_pairs(self, pairs, cost: str = "mi", bins: int = 256) -> list:
+        """One launch over several image pairs: pairs = [(from, to, Ts (K x 12)), ...];
+        returns one cost array per pair."""
+        keep, outs = [], []
+        arr = (PairCosts * len(pairs))()
+        for i, (frm, to, Ts) in enumerate(pairs):
+            Ts = np.ascontiguousarray(Ts, np.float32).reshape(-1, 12)
+            out = np.zeros(Ts.shape[0], np.float64)
+            keep.append(Ts)
+            outs.append(out)
+            arr[i] = PairCosts(frm.handle, to.handle, Ts.ctypes.data, Ts.shape[0], out.ctypes.data)
+        _check(load_library().tiplhip_cost_batch_pairs(self.handle, arr, len(pairs),
+                                                       COST_MI if cost == "mi" else COST_CORR, bins))
+        return outs
+
     def cost_batch_params(self, frm: Volume, from_vs, to: Volume, to_vs, params, cost="mi",
                           bins=256) -> np.ndarray:
         P = np.ascontiguousarray(params, np.float32).reshape(-1, 12)
@@ -367,3 +424,109 @@ class Context:
         _check(load_library().tiplhip_upsample_field(self.handle, _ptr(field), _dims(field.shape[:3]),
                                                      _ptr(out), _dims(out_shape)))
         return out
+
+    def gaussian(self, img: np.ndarray, bandwidth: int = 1) -> np.ndarray:
+        """filter::gaussian of a float32 (z, y, x) image or (z, y, x, 3) field; returns a copy."""
+        out = np.array(img, np.float32, order="C")
+        ch = out.shape[3] if out.ndim == 4 else 1
+        _check(load_library().tiplhip_gaussian(self.handle, _ptr(out), _dims(out.shape[:3]), ch,
+                                               C.c_uint(bandwidth)))
+        return out
+
+
+# ---------------------------------------------------------------------------
+# Registration of image pairs: the plain path and the serving loop
+# ---------------------------------------------------------------------------
+_DTYPES = {np.dtype(np.uint8): U8, np.dtype(np.int16): I16, np.dtype(np.float32): F32}
+
+
+def make_request(fixed, moving, fixed_vs=(1, 1, 1), moving_vs=(1, 1, 1), cost="mi", bins=256,
+                 nonlinear=False, want_field=True, **cdm):
+    """A RegRequest over numpy (z, y, x) volumes of uint8 / int16 / float32.
+    Returns (request, keep) where keep holds the arrays the request points to
+    (and the field output, when nonlinear and want_field)."""
+    fixed = np.ascontiguousarray(fixed)
+    moving = np.ascontiguousarray(moving)
+    if fixed.dtype not in _DTYPES or moving.dtype not in _DTYPES:
+        raise TypeError("volumes must be uint8, int16 or float32")
+    r = RegRequest()
+    load_library().tiplhip_reg_request_default(C.byref(r))
+    r.fixed, r.fixed_dtype = fixed.ctypes.data, _DTYPES[fixed.dtype]
+    r.moving, r.moving_dtype = moving.ctypes.data, _DTYPES[moving.dtype]
+    r.fixed_dim[:] = list(fixed.shape[::-1])
+    r.moving_dim[:] = list(moving.shape[::-1])
+    r.fixed_vs[:] = list(map(float, fixed_vs))
+    r.moving_vs[:] = list(map(float, moving_vs))
+    r.cost = COST_MI if cost == "mi" else COST_CORR
+    r.bins = int(bins)
+    r.nonlinear = int(bool(nonlinear))
+    gt = cdm.pop("gradient_type", None)
+    for k, v in cdm.items():
+        setattr(r.cdm, k, v)
+    if gt is not None:
+        r.cdm.gradient_type = (gt * 8)[:8].encode() if len(gt) == 1 else gt.encode()[:8].ljust(8, b"r")
+    field = None
+    if nonlinear and want_field:
+        field = np.zeros(fixed.shape + (3,), np.float32)
+        r.field = field.ctypes.data
+    return r, {"fixed": fixed, "moving": moving, "field": field}
+
+
+def _result(res: RegResult, field):
+    return {"status": res.status, "affine": np.array(res.affine[:], np.float32),
+            "T": np.array(res.T[:], np.float32), "cost": float(res.cost),
+            "cdm_iterations": int(res.cdm_iterations), "evaluations": int(res.evaluations),
+            "queue_ms": float(res.queue_ms), "run_ms": float(res.run_ms), "field": field}
+
+
+def register(ctx: "Context", fixed, moving, **kw) -> dict:
+    """The plain single-pair path (tiplhip_register) on ctx's thread and stream."""
+    req, keep = make_request(fixed, moving, **kw)
+    res = RegResult()
+    _check(load_library().tiplhip_register(ctx.handle, C.byref(req), C.byref(res)))
+    return _result(res, keep["field"])
+
+
+def request_bytes(fixed, moving, **kw) -> int:
+    req, _ = make_request(fixed, moving, want_field=False, **kw)
+    return int(load_library().tiplhip_request_bytes(C.byref(req)))
+
+
+class Server:
+    """The serving loop (tiplhip_server_*): slots, HBM-budget admission, one cost batcher."""
+
+    def __init__(self, device: int = 0, slots: int = 8, hbm_budget: int = 64 << 30):
+        h = C.c_void_p()
+        _check(load_library().tiplhip_server_create(device, slots, C.c_size_t(hbm_budget), C.byref(h)))
+        self.handle = h.value
+        self._keep = {}
+
+    def submit(self, fixed, moving, **kw) -> int:
+        req, keep = make_request(fixed, moving, **kw)
+        t = C.c_uint64(0)
+        _check(load_library().tiplhip_server_submit(self.handle, C.byref(req), C.byref(t)))
+        self._keep[t.value] = keep
+        return t.value
+
+    def wait(self, ticket: int) -> dict:
+        res = RegResult()
+        keep = self._keep.pop(ticket, {"field": None})
+        _check(load_library().tiplhip_server_wait(self.handle, C.c_uint64(ticket), C.byref(res)))
+        return _result(res, keep["field"])
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _check(load_library().tiplhip_server_stats_get(self.handle, C.byref(st)))
+        return {name: getattr(st, name) for name, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.handle:
+            _check(load_library().tiplhip_server_destroy(self.handle))
+            self.handle = None
+            self._keep.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

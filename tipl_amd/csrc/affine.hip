@@ -8,17 +8,20 @@
 //   reg::get_mutual_info      reg/linear.hpp:138-169
 //   reg::get_mutual_info_mean reg/linear.hpp:87-134
 //
-// Work decomposition.  The `from` grid is cut into row segments of 64
-// voxels (one wave, one row, x contiguous), so y and z are wave-uniform and
-// every load of `from` is one coalesced 64-byte access.  A launch covers K
-// candidates x C chunks of segments; workgroups are remapped so that each
-// XCD works through a contiguous slab of the volume for all candidates.
+// Work decomposition.  The `from` grid is cut into wave tiles of 16 x 4
+// voxels of one plane, each marched through a run of planes (an item).  A
+// launch covers one or more image pairs (segments), each with its own K
+// candidate transforms; a segment contributes K x C workgroups, each adding
+// a chunk of items for one candidate.  Workgroups are remapped so that each
+// XCD works through a contiguous range of one segment, and each workgroup
+// finds its segment by a binary search over the segments' first blocks.
 // Integer results (sums, histograms) are accumulated with integer atomics
-// only, so they are exact and independent of scheduling.
+// only, so they are exact and independent of scheduling and batching.
 #include "device.hpp"
 #include "internal.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 namespace tiplhip {
@@ -53,10 +56,8 @@ VolGeo make_geo(const int* dim)
 
 constexpr uint32_t kTileX = 16, kTileY = 4;  // one wave = 16 x 4 voxels of one plane
 
-// max_voxels_per_block bounds the voxels one workgroup adds (the uint16 LDS
-// counters of the 256-bin histogram hold at most 65535).
-BatchGeo make_batch(const tiplhip_volume* from, const tiplhip_volume* to, int K,
-                    uint32_t max_voxels_per_block, uint32_t target_blocks)
+// Item geometry of one pair for K candidates (blocks chosen by batch_blocks).
+BatchGeo batch_geometry(const tiplhip_volume* from, const tiplhip_volume* to, int K)
 {
     BatchGeo b;
     b.fw = from->dim[0];
@@ -67,19 +68,68 @@ BatchGeo make_batch(const tiplhip_volume* from, const tiplhip_volume* to, int K,
     b.zc = std::min<uint32_t>(16u, uint32_t(b.fd));
     b.nzc = (uint32_t(b.fd) + b.zc - 1) / b.zc;
     b.items = b.ncols * b.nzc;
-    const uint32_t max_items = std::max<uint32_t>(1u, max_voxels_per_block / (64u * b.zc));
-    uint64_t want = (uint64_t(b.items) * uint64_t(K) + target_blocks - 1) / target_blocks;
-    uint32_t ipb = uint32_t(std::min<uint64_t>(want, max_items));
-    ipb = std::max<uint32_t>(ipb, std::min<uint32_t>(4u, max_items));
-    b.items_per_block = ipb;
     b.K = uint32_t(K);
-    uint32_t chunks = (b.items + ipb - 1) / ipb;
-    b.total_blocks = chunks * uint32_t(K);
+    b.items_per_block = 1;
+    b.total_blocks = 0;
     b.from_bytes = uint32_t(from->n);
     b.to_bytes8 = uint32_t(to->n * 8u);
     b.to = make_geo(to->dim);
     return b;
 }
+
+// Chooses items per workgroup so that the pair's K candidates take about
+// `target_blocks` workgroups; max_voxels_per_block bounds the voxels one
+// workgroup adds (the uint16 LDS counters of the 256-bin histogram hold at
+// most 65535).
+void batch_blocks(BatchGeo& b, uint32_t max_voxels_per_block, uint32_t target_blocks)
+{
+    target_blocks = std::max<uint32_t>(1u, target_blocks);
+    const uint32_t max_items = std::max<uint32_t>(1u, max_voxels_per_block / (64u * b.zc));
+    uint64_t want = (uint64_t(b.items) * uint64_t(b.K) + target_blocks - 1) / target_blocks;
+    uint32_t ipb = uint32_t(std::min<uint64_t>(want, max_items));
+    ipb = std::max<uint32_t>(ipb, std::min<uint32_t>(4u, max_items));
+    // at most one plane of tile columns per workgroup, so that its items
+    // span at most two z runs (the kernels tabulate z products per run)
+    ipb = std::min<uint32_t>(ipb, std::max<uint32_t>(1u, b.ncols));
+    b.items_per_block = ipb;
+    b.total_blocks = ((b.items + ipb - 1) / ipb) * b.K;
+}
+
+// One image pair of a launch.  Candidates are numbered globally: the
+// segment's candidates are cand_begin .. cand_begin + g.K - 1, its
+// workgroups block_begin .. block_begin + g.total_blocks - 1.
+struct Segment {
+    BatchGeo g;
+    const uint8_t* from;
+    const uint8_t* to;
+    const uint64_t* to8;
+    const uint32_t* from_hist;   // value histogram of `from` at the launch's bin count
+    long long sx, sxx;           // exact sums of `from` (correlation)
+    double n;                    // voxels of `from`
+    uint32_t block_begin;
+    uint32_t cand_begin;
+};
+
+// Largest i with key(segs[i]) <= v, for a wave-uniform v (scalar loads).
+template <class Key>
+__device__ inline uint32_t find_segment(const Segment* __restrict__ segs, uint32_t nseg, uint32_t v,
+                                        Key key)
+{
+    uint32_t lo = 0, hi = nseg;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (key(segs[mid]) <= v) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+struct ByBlock {
+    __device__ uint32_t operator()(const Segment& s) const { return s.block_begin; }
+};
+struct ByCandidate {
+    __device__ uint32_t operator()(const Segment& s) const { return s.cand_begin; }
+};
 
 // Wave-uniform decode of an item into this lane's (x, y) and the z run.
 struct Item {
@@ -132,35 +182,57 @@ __device__ inline Pos apply(const float* T, float fx, float fy, float fz)
     return p;
 }
 
+// z*r2, z*r5, z*r8 (the z terms of vector_transformation) for the planes a
+// workgroup visits: its items [it0, it1) span at most two z runs, so at most
+// 2 x 16 planes starting at plane zbase.  Returns zbase; needs a barrier
+// before use.
+constexpr uint32_t kZTable = 32;
+__device__ inline uint32_t build_z_table(const BatchGeo& g, const float* T, uint32_t it0,
+                                         uint32_t it1, float4* zt)
+{
+    const uint32_t zbase = (it0 / g.ncols) * g.zc;
+    const uint32_t ztop = min(((it1 - 1) / g.ncols + 1) * g.zc, uint32_t(g.fd));
+    for (uint32_t i = threadIdx.x; i < ztop - zbase; i += blockDim.x) {
+        const float fz = float(zbase + i);
+        zt[i] = make_float4(fz * T[2], fz * T[5], fz * T[8], 0.0f);
+    }
+    return zbase;
+}
+
 // ---------------------------------------------------------------------------
 // Correlation: per candidate the exact sums Sy, Syy, Sxy (Sx, Sxx of `from`
 // do not depend on the transform and are kept with the volume).
 // crop[k] = {flag, sx, sy, sz}: flag 1 reproduces the CPU resample's exact
 // crop for an integer-shift transform (numerical/resampling.hpp:1244-1262).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void corr_sums_kernel(const uint8_t* __restrict__ from,
-                                                        const uint8_t* __restrict__ to,
-                                                        const uint64_t* __restrict__ to8,
+__global__ __launch_bounds__(256) void corr_sums_kernel(const Segment* __restrict__ segs,
+                                                        uint32_t nseg, uint32_t total_blocks,
                                                         const float* __restrict__ Ts,
-                                                        const int4* __restrict__ crop, BatchGeo g,
+                                                        const int4* __restrict__ crop,
                                                         unsigned long long* __restrict__ sums)
 {
-    const uint32_t L = xcd_remap(blockIdx.x, g.total_blocks);
-    const uint32_t cand = L % g.K;
-    const uint32_t chunk = L / g.K;
+    const uint32_t L = xcd_remap(blockIdx.x, total_blocks);
+    const Segment& sg = segs[find_segment(segs, nseg, L, ByBlock{})];
+    const BatchGeo g = sg.g;
+    const uint32_t local = L - sg.block_begin;
+    const uint32_t cand = sg.cand_begin + local % g.K;
+    const uint32_t chunk = local / g.K;
     float T[12];
 #pragma unroll
     for (int i = 0; i < 12; ++i) T[i] = Ts[12 * cand + i];
     const int4 cr = crop[cand];
-    const __amdgpu_buffer_rsrc_t rf = make_rsrc(from, g.from_bytes);
-    const __amdgpu_buffer_rsrc_t rt = make_rsrc(to, g.to_bytes8 / 8u);
-    const __amdgpu_buffer_rsrc_t r8 = make_rsrc(to8, g.to_bytes8);
+    const __amdgpu_buffer_rsrc_t rf = make_rsrc(sg.from, g.from_bytes);
+    const __amdgpu_buffer_rsrc_t rt = make_rsrc(sg.to, g.to_bytes8 / 8u);
+    const __amdgpu_buffer_rsrc_t r8 = make_rsrc(sg.to8, g.to_bytes8);
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t nwaves = blockDim.x >> 6;
     const uint32_t it0 = chunk * g.items_per_block;
     const uint32_t it1 = min(it0 + g.items_per_block, g.items);
     const uint32_t fwh = uint32_t(g.fw) * uint32_t(g.fh);
+    __shared__ float4 zt[kZTable];
+    const uint32_t zbase = build_z_table(g, T, it0, it1, zt);
+    __syncthreads();
     uint32_t sy = 0, syy = 0, sxy = 0;
     for (uint32_t it = it0 + wave; it < it1; it += nwaves) {
         const Item m = decode(g, it, lane);
@@ -178,10 +250,10 @@ __global__ __launch_bounds__(256) void corr_sums_kernel(const uint8_t* __restric
                                           uint32_t(sx), 0)
                         : 0u;
             } else {
-                const float fz = float(z);
-                const float px = cs.hx + fz * T[2] + T[9];
-                const float py = cs.hy + fz * T[5] + T[10];
-                const float pz = cs.hz + fz * T[8] + T[11];
+                const float4 zr = zt[z - zbase];
+                const float px = (cs.hx + zr.x) + T[9];
+                const float py = (cs.hy + zr.y) + T[10];
+                const float pz = (cs.hz + zr.z) + T[11];
                 yv = sample_u8_packed(r8, g.to, px, py, pz);
             }
             sy += yv;
@@ -206,12 +278,15 @@ __global__ __launch_bounds__(256) void corr_sums_kernel(const uint8_t* __restric
 
 // tipl::correlation from exact sums (numerical/statistics.hpp:555-586,
 // 407-443) in FP64, then `float c; 1.0f - c*c` (reg/linear.hpp:34-35).
-__global__ void corr_finalize_kernel(const unsigned long long* __restrict__ sums, long long sx,
-                                     long long sxx, double n, int K, double* __restrict__ costs,
-                                     long long* __restrict__ sums_out)
+__global__ void corr_finalize_kernel(const Segment* __restrict__ segs, uint32_t nseg,
+                                     const unsigned long long* __restrict__ sums, int K,
+                                     double* __restrict__ costs, long long* __restrict__ sums_out)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= K) return;
+    const Segment& sg = segs[find_segment(segs, nseg, uint32_t(k), ByCandidate{})];
+    const long long sx = sg.sx, sxx = sg.sxx;
+    const double n = sg.n;
     long long sy = (long long)sums[k * 4 + 0];
     long long syy = (long long)sums[k * 4 + 1];
     long long sxy = (long long)sums[k * 4 + 2];
@@ -271,28 +346,33 @@ struct MiTraits<64> {
 
 template <int BINS>
 __global__ __launch_bounds__(MiTraits<BINS>::kThreads) void mi_hist_kernel(
-    const uint8_t* __restrict__ from, const uint64_t* __restrict__ to8,
-    const float* __restrict__ Ts, BatchGeo g, int* __restrict__ hist)
+    const Segment* __restrict__ segs, uint32_t nseg, uint32_t total_blocks,
+    const float* __restrict__ Ts, int* __restrict__ hist)
 {
     using Tr = MiTraits<BINS>;
     __shared__ uint32_t lds[Tr::kWords];
+    __shared__ float4 zt[kZTable];
     for (uint32_t i = threadIdx.x; i < uint32_t(Tr::kWords); i += blockDim.x) lds[i] = 0u;
-    __syncthreads();
 
-    const uint32_t L = xcd_remap(blockIdx.x, g.total_blocks);
-    const uint32_t cand = L % g.K;
-    const uint32_t chunk = L / g.K;
+    const uint32_t L = xcd_remap(blockIdx.x, total_blocks);
+    const Segment& sg = segs[find_segment(segs, nseg, L, ByBlock{})];
+    const BatchGeo g = sg.g;
+    const uint32_t local = L - sg.block_begin;
+    const uint32_t cand = sg.cand_begin + local % g.K;
+    const uint32_t chunk = local / g.K;
     float T[12];
 #pragma unroll
     for (int i = 0; i < 12; ++i) T[i] = Ts[12 * cand + i];
-    const __amdgpu_buffer_rsrc_t rf = make_rsrc(from, g.from_bytes);
-    const __amdgpu_buffer_rsrc_t r8 = make_rsrc(to8, g.to_bytes8);
+    const __amdgpu_buffer_rsrc_t rf = make_rsrc(sg.from, g.from_bytes);
+    const __amdgpu_buffer_rsrc_t r8 = make_rsrc(sg.to8, g.to_bytes8);
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t nwaves = blockDim.x >> 6;
     const uint32_t it0 = chunk * g.items_per_block;
     const uint32_t it1 = min(it0 + g.items_per_block, g.items);
     const uint32_t fwh = uint32_t(g.fw) * uint32_t(g.fh);
+    const uint32_t zbase = build_z_table(g, T, it0, it1, zt);
+    __syncthreads();
     for (uint32_t it = it0 + wave; it < it1; it += nwaves) {
         const Item m = decode(g, it, lane);
         if (!m.ok) continue;
@@ -300,10 +380,10 @@ __global__ __launch_bounds__(MiTraits<BINS>::kThreads) void mi_hist_kernel(
         uint32_t fidx = m.z0 * fwh + m.y * uint32_t(g.fw) + m.x;
         for (uint32_t z = m.z0; z < m.z1; ++z, fidx += fwh) {
             const uint32_t fv = load_u8(rf, fidx, 0);
-            const float fz = float(z);
-            const float px = cs.hx + fz * T[2] + T[9];
-            const float py = cs.hy + fz * T[5] + T[10];
-            const float pz = cs.hz + fz * T[8] + T[11];
+            const float4 zr = zt[z - zbase];
+            const float px = (cs.hx + zr.x) + T[9];
+            const float py = (cs.hy + zr.y) + T[10];
+            const float pz = (cs.hz + zr.z) + T[11];
             const uint32_t tv = sample_u8_packed(r8, g.to, px, py, pz);
             const uint32_t bin = ((fv >> Tr::kShift) << Tr::kBand) | (tv >> Tr::kShift);
             const uint32_t b0 = __builtin_amdgcn_readfirstlane(bin);
@@ -339,11 +419,14 @@ __global__ __launch_bounds__(MiTraits<BINS>::kThreads) void mi_hist_kernel(
 // deterministic (the reference's sum is sequential in row-major order; the
 // two agree to rounding).
 template <int BINS>
-__global__ __launch_bounds__(256) void mi_entropy_kernel(const int* __restrict__ hist,
-                                                         const uint32_t* __restrict__ from_hist,
+__global__ __launch_bounds__(256) void mi_entropy_kernel(const Segment* __restrict__ segs,
+                                                         uint32_t nseg,
+                                                         const int* __restrict__ hist,
                                                          double* __restrict__ costs)
 {
     const int* h = hist + size_t(blockIdx.x) * BINS * BINS;
+    const uint32_t* __restrict__ from_hist =
+        segs[find_segment(segs, nseg, blockIdx.x, ByCandidate{})].from_hist;
     __shared__ uint32_t to_h[BINS];
     __shared__ double part[4];
     for (int c = threadIdx.x; c < BINS; c += blockDim.x) {
@@ -420,12 +503,13 @@ __global__ void fill_u32x4_kernel(uint4* p, size_t n)
 // Host side
 // ---------------------------------------------------------------------------
 
-constexpr int kMaxK = 4096;
+constexpr int kMaxK = 4096;  // candidates per launch, over all pairs
 
 void check_volume(const tiplhip_ctx* ctx, const tiplhip_volume* v, const char* what)
 {
     TIPLHIP_REQUIRE(v != nullptr, std::string(what) + " is null");
-    TIPLHIP_REQUIRE(v->ctx == ctx, std::string(what) + " belongs to another context");
+    TIPLHIP_REQUIRE(v->ctx != nullptr && v->ctx->device == ctx->device,
+                    std::string(what) + " lives on another device");
     TIPLHIP_REQUIRE(v->n > 0, std::string(what) + " is empty");
     TIPLHIP_REQUIRE(v->n < (size_t(1) << 29), std::string(what) + " exceeds 2^29 voxels");
     TIPLHIP_REQUIRE(size_t(v->dim[0]) * size_t(v->dim[1]) < (size_t(1) << 24),
@@ -467,72 +551,132 @@ void zero_bytes16(tiplhip_ctx* ctx, void* p, size_t bytes)
     TIPLHIP_CHECK(hipGetLastError());
 }
 
-// Uploads K transforms (and per-candidate crop info) to the context scratch.
-void upload_transforms(tiplhip_ctx* ctx, const float* T, int K, const int4* crop)
+// K candidate transforms of one image pair.
+struct PairJob {
+    const tiplhip_volume* from;
+    const tiplhip_volume* to;
+    int K;
+};
+
+// A launch's device inputs: transforms, crop flags and the segment table.
+struct Staged {
+    uint32_t nseg = 0, total_blocks = 0, K = 0;
+    const Segment* segs = nullptr;
+    const float* T = nullptr;
+    const int4* crop = nullptr;
+};
+
+bool hist256(int cost, int bins) { return cost == TIPLHIP_COST_MI && bins == 256; }
+
+// Builds the segment table of `jobs` (T holds their transforms, job after
+// job; crop, when not null, one entry per candidate) and uploads all of it
+// in one copy.  Each pair gets a share of about cu_count x 8 workgroups (x 4
+// for the 1024-thread 256-bin histogram) in proportion to its voxel-candidate
+// work.
+Staged stage_jobs(tiplhip_ctx* ctx, const std::vector<PairJob>& jobs, const float* T,
+                  const int4* crop, int cost, int bins)
 {
-    size_t tbytes = sizeof(float) * 12 * size_t(K);
-    size_t cbytes = sizeof(int4) * size_t(K);
-    ctx->h_T.reserve(tbytes + cbytes);
-    ctx->d_T.reserve(tbytes);
-    ctx->d_aux.reserve(cbytes);
-    std::memcpy(ctx->h_T.p, T, tbytes);
-    TIPLHIP_CHECK(hipMemcpyAsync(ctx->d_T.p, ctx->h_T.p, tbytes, hipMemcpyHostToDevice, ctx->stream));
-    if (crop) {
-        std::memcpy(static_cast<char*>(ctx->h_T.p) + tbytes, crop, cbytes);
-        TIPLHIP_CHECK(hipMemcpyAsync(ctx->d_aux.p, static_cast<char*>(ctx->h_T.p) + tbytes, cbytes,
-                                     hipMemcpyHostToDevice, ctx->stream));
+    Staged st;
+    std::vector<Segment> segs(jobs.size());
+    const uint32_t max_vox = hist256(cost, bins) ? 65535u : (1u << 23);
+    const uint32_t target = uint32_t(ctx->cu_count) * (hist256(cost, bins) ? 4u : 8u);
+    double work = 0.0;
+    for (size_t j = 0; j < jobs.size(); ++j) {
+        segs[j].g = batch_geometry(jobs[j].from, jobs[j].to, jobs[j].K);
+        work += double(segs[j].g.items) * double(jobs[j].K);
     }
+    uint64_t blocks = 0;
+    for (size_t j = 0; j < jobs.size(); ++j) {
+        Segment& sg = segs[j];
+        const double share = double(sg.g.items) * double(jobs[j].K) / work;
+        batch_blocks(sg.g, max_vox, uint32_t(std::ceil(share * double(target))));
+        sg.from = jobs[j].from->d;
+        sg.to = jobs[j].to->d;
+        sg.to8 = jobs[j].to->d_pack;
+        sg.from_hist = jobs[j].from->d_hist + (bins == 64 ? 256 : 0);
+        sg.sx = (long long)jobs[j].from->sum;
+        sg.sxx = (long long)jobs[j].from->sum_sq;
+        sg.n = double(jobs[j].from->n);
+        sg.block_begin = uint32_t(blocks);
+        sg.cand_begin = st.K;
+        blocks += sg.g.total_blocks;
+        st.K += uint32_t(jobs[j].K);
+    }
+    TIPLHIP_REQUIRE(blocks < (uint64_t(1) << 31), "launch exceeds 2^31 workgroups");
+    st.nseg = uint32_t(segs.size());
+    st.total_blocks = uint32_t(blocks);
+    const size_t tb = sizeof(float) * 12 * size_t(st.K);   // a multiple of 16
+    const size_t cb = crop ? sizeof(int4) * size_t(st.K) : 0;
+    const size_t sb = sizeof(Segment) * segs.size();
+    ctx->h_T.reserve(tb + cb + sb);
+    ctx->d_T.reserve(tb + cb + sb);
+    char* h = static_cast<char*>(ctx->h_T.p);
+    std::memcpy(h, T, tb);
+    // a zero shift component is stored as +0.0 (the packed sampler's bounds
+    // test needs coordinates that are never -0.0; the sample is unchanged)
+    float* hT = reinterpret_cast<float*>(h);
+    for (size_t k = 0; k < size_t(st.K); ++k)
+        for (int i = 9; i < 12; ++i)
+            if (hT[12 * k + size_t(i)] == 0.0f) hT[12 * k + size_t(i)] = 0.0f;
+    if (crop) std::memcpy(h + tb, crop, cb);
+    std::memcpy(h + tb + cb, segs.data(), sb);
+    TIPLHIP_CHECK(hipMemcpyAsync(ctx->d_T.p, h, tb + cb + sb, hipMemcpyHostToDevice, ctx->stream));
+    char* d = static_cast<char*>(ctx->d_T.p);
+    st.T = reinterpret_cast<const float*>(d);
+    st.crop = crop ? reinterpret_cast<const int4*>(d + tb) : nullptr;
+    st.segs = reinterpret_cast<const Segment*>(d + tb + cb);
+    return st;
 }
 
-// Launches the correlation sums for K transforms already in ctx->d_T / d_aux;
-// leaves Sy, Syy, Sxy in ctx->d_sums (K x 4 u64).
-void launch_corr_sums(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_volume* to, int K)
+// Correlation sums Sy, Syy, Sxy of every staged candidate into ctx->d_sums (K x 4 u64).
+void launch_corr_sums(tiplhip_ctx* ctx, const Staged& st)
 {
-    volume_ensure_packed(ctx, const_cast<tiplhip_volume*>(to));
-    ctx->d_sums.reserve(sizeof(unsigned long long) * 4 * size_t(K));
-    zero_u64(ctx, ctx->d_sums.p, 4 * size_t(K));
+    ctx->d_sums.reserve(sizeof(unsigned long long) * 4 * size_t(st.K));
+    zero_u64(ctx, ctx->d_sums.p, 4 * size_t(st.K));
     // per-lane uint32 sums of squares stay below 2^32 for < 65536 voxels a lane
-    BatchGeo g = make_batch(from, to, K, 1u << 23, uint32_t(ctx->cu_count) * 8u);
     timing_begin(ctx);
-    hipLaunchKernelGGL(corr_sums_kernel, dim3(g.total_blocks), dim3(256), 0, ctx->stream, from->d,
-                       to->d, to->d_pack, ctx->d_T.as<float>(), ctx->d_aux.as<int4>(), g,
+    hipLaunchKernelGGL(corr_sums_kernel, dim3(st.total_blocks), dim3(256), 0, ctx->stream, st.segs,
+                       st.nseg, st.total_blocks, st.T, st.crop,
                        ctx->d_sums.as<unsigned long long>());
     TIPLHIP_CHECK(hipGetLastError());
     timing_end(ctx);
 }
 
-// Launches the joint histograms for K transforms in ctx->d_T into ctx->d_hist.
-void launch_mi_hist(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_volume* to, int K,
-                    int bins)
+void launch_corr_finalize(tiplhip_ctx* ctx, const Staged& st, long long* sums_out)
 {
-    volume_ensure_packed(ctx, const_cast<tiplhip_volume*>(to));
-    size_t hb = sizeof(int) * size_t(bins) * bins * size_t(K);
+    ctx->d_costs.reserve(sizeof(double) * size_t(st.K));
+    hipLaunchKernelGGL(corr_finalize_kernel, dim3((st.K + 63) / 64), dim3(64), 0, ctx->stream,
+                       st.segs, st.nseg, ctx->d_sums.as<unsigned long long>(), int(st.K),
+                       ctx->d_costs.as<double>(), sums_out);
+    TIPLHIP_CHECK(hipGetLastError());
+}
+
+// Joint histograms of every staged candidate into ctx->d_hist.
+void launch_mi_hist(tiplhip_ctx* ctx, const Staged& st, int bins)
+{
+    size_t hb = sizeof(int) * size_t(bins) * bins * size_t(st.K);
     ctx->d_hist.reserve(hb);
     zero_bytes16(ctx, ctx->d_hist.p, hb);
     timing_begin(ctx);
-    if (bins == 256) {
-        // at most 65535 voxels per workgroup keeps every uint16 LDS counter exact
-        BatchGeo g = make_batch(from, to, K, 65535u, uint32_t(ctx->cu_count) * 4u);
-        hipLaunchKernelGGL(mi_hist_kernel<256>, dim3(g.total_blocks), dim3(1024), 0, ctx->stream,
-                           from->d, to->d_pack, ctx->d_T.as<float>(), g, ctx->d_hist.as<int>());
-    } else {
-        BatchGeo g = make_batch(from, to, K, 1u << 23, uint32_t(ctx->cu_count) * 8u);
-        hipLaunchKernelGGL(mi_hist_kernel<64>, dim3(g.total_blocks), dim3(256), 0, ctx->stream,
-                           from->d, to->d_pack, ctx->d_T.as<float>(), g, ctx->d_hist.as<int>());
-    }
+    if (bins == 256)
+        hipLaunchKernelGGL(mi_hist_kernel<256>, dim3(st.total_blocks), dim3(1024), 0, ctx->stream,
+                           st.segs, st.nseg, st.total_blocks, st.T, ctx->d_hist.as<int>());
+    else
+        hipLaunchKernelGGL(mi_hist_kernel<64>, dim3(st.total_blocks), dim3(256), 0, ctx->stream,
+                           st.segs, st.nseg, st.total_blocks, st.T, ctx->d_hist.as<int>());
     TIPLHIP_CHECK(hipGetLastError());
     timing_end(ctx);
 }
 
-void launch_mi_entropy(tiplhip_ctx* ctx, const tiplhip_volume* from, int K, int bins)
+void launch_mi_entropy(tiplhip_ctx* ctx, const Staged& st, int bins)
 {
-    ctx->d_costs.reserve(sizeof(double) * size_t(K));
+    ctx->d_costs.reserve(sizeof(double) * size_t(st.K));
     if (bins == 256)
-        hipLaunchKernelGGL(mi_entropy_kernel<256>, dim3(K), dim3(256), 0, ctx->stream,
-                           ctx->d_hist.as<int>(), from->d_hist, ctx->d_costs.as<double>());
+        hipLaunchKernelGGL(mi_entropy_kernel<256>, dim3(st.K), dim3(256), 0, ctx->stream, st.segs,
+                           st.nseg, ctx->d_hist.as<int>(), ctx->d_costs.as<double>());
     else
-        hipLaunchKernelGGL(mi_entropy_kernel<64>, dim3(K), dim3(256), 0, ctx->stream,
-                           ctx->d_hist.as<int>(), from->d_hist + 256, ctx->d_costs.as<double>());
+        hipLaunchKernelGGL(mi_entropy_kernel<64>, dim3(st.K), dim3(256), 0, ctx->stream, st.segs,
+                           st.nseg, ctx->d_hist.as<int>(), ctx->d_costs.as<double>());
     TIPLHIP_CHECK(hipGetLastError());
 }
 
@@ -544,45 +688,75 @@ void read_back(tiplhip_ctx* ctx, const void* dev, void* host, size_t bytes)
     std::memcpy(host, ctx->h_out.p, bytes);
 }
 
-void cost_batch(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_volume* to,
-                const float* T, int K, int cost, int bins, double* costs)
+// tipl::resample of src into a device grid of shape out_dim.
+void launch_resample(tiplhip_ctx* ctx, const tiplhip_volume* src, const int* out_dim, const float* T,
+                     bool crop_shortcut, uint8_t* out)
 {
-    check_volume(ctx, from, "from");
-    check_volume(ctx, to, "to");
+    TransformArg ta;
+    std::memcpy(ta.t, T, sizeof(ta.t));
+    int pos[3] = {0, 0, 0};
+    bool c = crop_shortcut && is_integer_shift(T, pos);
+    int4 crop = make_int4(c ? 1 : 0, pos[0], pos[1], pos[2]);
+    const size_t n = vox(out_dim);
+    int blocks = int(std::min<size_t>((n + 255) / 256, 65536));
+    hipLaunchKernelGGL(resample_kernel, dim3(blocks), dim3(256), 0, ctx->stream, src->d,
+                       uint32_t(src->n), make_geo(src->dim), out_dim[0], out_dim[1], out_dim[2], ta,
+                       crop, out);
+    TIPLHIP_CHECK(hipGetLastError());
+}
+
+// Costs of every candidate of every job in one launch (plus one finishing
+// launch); costs[] receives them job after job.  T holds the jobs'
+// transforms in the same order.
+void cost_jobs(tiplhip_ctx* ctx, std::vector<PairJob> jobs, const float* T, int cost, int bins,
+               double* costs)
+{
+    TIPLHIP_REQUIRE(!jobs.empty(), "no image pairs");
     TIPLHIP_REQUIRE(T != nullptr && costs != nullptr, "null transform or cost buffer");
-    TIPLHIP_REQUIRE(K >= 1 && K <= kMaxK, "K must be in [1, 4096]");
     TIPLHIP_REQUIRE(cost == TIPLHIP_COST_CORR || cost == TIPLHIP_COST_MI, "unknown cost type");
+    if (cost == TIPLHIP_COST_MI) TIPLHIP_REQUIRE(bins == 256 || bins == 64, "bins must be 256 or 64");
+    size_t K = 0;
+    for (const PairJob& j : jobs) {
+        check_volume(ctx, j.from, "from");
+        check_volume(ctx, j.to, "to");
+        TIPLHIP_REQUIRE(j.K >= 1 && j.K <= kMaxK, "K must be in [1, 4096]");
+        K += size_t(j.K);
+    }
+    TIPLHIP_REQUIRE(K <= size_t(kMaxK), "more than 4096 candidates in one launch");
     DeviceGuard dg(ctx->device);
+    std::vector<float> Tk(T, T + 12 * K);
+    std::vector<int4> crop;
     if (cost == TIPLHIP_COST_CORR) {
         // reg::correlation swaps the roles when `from` is larger
         // (reg/linear.hpp:28-33); a singular inverse leaves T unchanged, as
         // transformation_matrix::inverse does.
-        std::vector<float> Tk(T, T + 12 * size_t(K));
-        if (from->n > to->n) {
-            std::swap(from, to);
-            for (int k = 0; k < K; ++k) (void)matrix_inverse(&Tk[12 * size_t(k)]);
+        crop.resize(K);
+        size_t base = 0;
+        for (PairJob& j : jobs) {
+            float* Tj = &Tk[12 * base];
+            if (j.from->n > j.to->n) {
+                std::swap(j.from, j.to);
+                for (int k = 0; k < j.K; ++k) (void)matrix_inverse(&Tj[12 * size_t(k)]);
+            }
+            for (int k = 0; k < j.K; ++k) {
+                int pos[3];
+                bool c = is_integer_shift(&Tj[12 * size_t(k)], pos);
+                crop[base + size_t(k)] = make_int4(c ? 1 : 0, pos[0], pos[1], pos[2]);
+            }
+            base += size_t(j.K);
         }
-        std::vector<int4> crop(static_cast<size_t>(K));
-        for (int k = 0; k < K; ++k) {
-            int pos[3];
-            bool c = is_integer_shift(&Tk[12 * size_t(k)], pos);
-            crop[size_t(k)] = make_int4(c ? 1 : 0, pos[0], pos[1], pos[2]);
-        }
-        upload_transforms(ctx, Tk.data(), K, crop.data());
-        launch_corr_sums(ctx, from, to, K);
-        ctx->d_costs.reserve(sizeof(double) * size_t(K));
-        hipLaunchKernelGGL(corr_finalize_kernel, dim3((K + 63) / 64), dim3(64), 0, ctx->stream,
-                           ctx->d_sums.as<unsigned long long>(), (long long)from->sum,
-                           (long long)from->sum_sq, double(from->n), K,
-                           ctx->d_costs.as<double>(), (long long*)nullptr);
-        TIPLHIP_CHECK(hipGetLastError());
-    } else {
-        TIPLHIP_REQUIRE(bins == 256 || bins == 64, "bins must be 256 or 64");
-        upload_transforms(ctx, T, K, nullptr);
-        launch_mi_hist(ctx, from, to, K, bins);
-        launch_mi_entropy(ctx, from, K, bins);
     }
-    read_back(ctx, ctx->d_costs.p, costs, sizeof(double) * size_t(K));
+    for (const PairJob& j : jobs) volume_ensure_packed(ctx, const_cast<tiplhip_volume*>(j.to));
+    const Staged st = stage_jobs(ctx, jobs, Tk.data(), crop.empty() ? nullptr : crop.data(), cost,
+                                 bins);
+    if (cost == TIPLHIP_COST_CORR) {
+        launch_corr_sums(ctx, st);
+        launch_corr_finalize(ctx, st, nullptr);
+    } else {
+        launch_mi_hist(ctx, st, bins);
+        launch_mi_entropy(ctx, st, bins);
+    }
+    read_back(ctx, ctx->d_costs.p, costs, sizeof(double) * K);
 }
 
 }  // namespace
@@ -605,17 +779,33 @@ int tiplhip_resample(tiplhip_ctx* ctx, const tiplhip_volume* src, const int out_
         DeviceGuard dg(ctx->device);
         size_t n = vox(out_dim);
         ctx->d_tmp.reserve(n);
-        TransformArg ta;
-        std::memcpy(ta.t, T, sizeof(ta.t));
-        int pos[3] = {0, 0, 0};
-        bool c = crop_shortcut && is_integer_shift(T, pos);
-        int4 crop = make_int4(c ? 1 : 0, pos[0], pos[1], pos[2]);
-        int blocks = int(std::min<size_t>((n + 255) / 256, 65536));
-        hipLaunchKernelGGL(resample_kernel, dim3(blocks), dim3(256), 0, ctx->stream, src->d,
-                           uint32_t(src->n), make_geo(src->dim), out_dim[0], out_dim[1],
-                           out_dim[2], ta, crop, ctx->d_tmp.as<uint8_t>());
-        TIPLHIP_CHECK(hipGetLastError());
+        launch_resample(ctx, src, out_dim, T, crop_shortcut != 0, ctx->d_tmp.as<uint8_t>());
         read_back(ctx, ctx->d_tmp.p, out, n);
+    });
+}
+
+int tiplhip_volume_resample(tiplhip_ctx* ctx, const tiplhip_volume* src, const int out_dim[3],
+                            const float T[12], int crop_shortcut, tiplhip_volume** out)
+{
+    return guarded([&] {
+        TIPLHIP_REQUIRE(out != nullptr, "null output");
+        *out = nullptr;
+        TIPLHIP_REQUIRE(ctx != nullptr, "null context");
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        check_volume(ctx, src, "src");
+        TIPLHIP_REQUIRE(out_dim && out_dim[0] > 0 && out_dim[1] > 0 && out_dim[2] > 0,
+                        "bad output shape");
+        TIPLHIP_REQUIRE(T != nullptr, "null transform");
+        DeviceGuard dg(ctx->device);
+        tiplhip_volume* v = volume_alloc(ctx, out_dim);
+        try {
+            launch_resample(ctx, src, out_dim, T, crop_shortcut != 0, v->d);
+            volume_compute_stats(ctx, v);
+        } catch (...) {
+            volume_release(v);
+            throw;
+        }
+        *out = v;
     });
 }
 
@@ -630,16 +820,13 @@ int tiplhip_corr_sums(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhi
         TIPLHIP_REQUIRE(T && sums, "null transform or output");
         TIPLHIP_REQUIRE(K >= 1 && K <= kMaxK, "K must be in [1, 4096]");
         DeviceGuard dg(ctx->device);
+        volume_ensure_packed(ctx, const_cast<tiplhip_volume*>(to));
         std::vector<int4> crop(size_t(K), make_int4(0, 0, 0, 0));
-        upload_transforms(ctx, T, K, crop.data());
-        launch_corr_sums(ctx, from, to, K);
-        ctx->d_costs.reserve(sizeof(double) * size_t(K));
+        const Staged st = stage_jobs(ctx, {PairJob{from, to, K}}, T, crop.data(),
+                                     TIPLHIP_COST_CORR, 256);
+        launch_corr_sums(ctx, st);
         ctx->d_tmp2.reserve(sizeof(long long) * 5 * size_t(K));
-        hipLaunchKernelGGL(corr_finalize_kernel, dim3((K + 63) / 64), dim3(64), 0, ctx->stream,
-                           ctx->d_sums.as<unsigned long long>(), (long long)from->sum,
-                           (long long)from->sum_sq, double(from->n), K,
-                           ctx->d_costs.as<double>(), ctx->d_tmp2.as<long long>());
-        TIPLHIP_CHECK(hipGetLastError());
+        launch_corr_finalize(ctx, st, ctx->d_tmp2.as<long long>());
         read_back(ctx, ctx->d_tmp2.p, sums, sizeof(int64_t) * 5 * size_t(K));
     });
 }
@@ -656,8 +843,9 @@ int tiplhip_mi_hist(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_
         TIPLHIP_REQUIRE(K >= 1 && K <= kMaxK, "K must be in [1, 4096]");
         TIPLHIP_REQUIRE(bins == 256 || bins == 64, "bins must be 256 or 64");
         DeviceGuard dg(ctx->device);
-        upload_transforms(ctx, T, K, nullptr);
-        launch_mi_hist(ctx, from, to, K, bins);
+        volume_ensure_packed(ctx, const_cast<tiplhip_volume*>(to));
+        const Staged st = stage_jobs(ctx, {PairJob{from, to, K}}, T, nullptr, TIPLHIP_COST_MI, bins);
+        launch_mi_hist(ctx, st, bins);
         read_back(ctx, ctx->d_hist.p, hist, sizeof(int) * size_t(bins) * bins * size_t(K));
     });
 }
@@ -668,7 +856,36 @@ int tiplhip_cost_batch(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplh
     return guarded([&] {
         TIPLHIP_REQUIRE(ctx != nullptr, "null context");
         std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-        cost_batch(ctx, from, to, T, K, cost, bins, costs);
+        cost_jobs(ctx, {PairJob{from, to, K}}, T, cost, bins, costs);
+    });
+}
+
+int tiplhip_cost_batch_pairs(tiplhip_ctx* ctx, const tiplhip_pair_costs* pairs, int n, int cost,
+                             int bins)
+{
+    return guarded([&] {
+        TIPLHIP_REQUIRE(ctx != nullptr && pairs != nullptr, "null argument");
+        TIPLHIP_REQUIRE(n >= 1, "need at least one pair");
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        std::vector<PairJob> jobs;
+        std::vector<float> T;
+        size_t K = 0;
+        for (int i = 0; i < n; ++i) {
+            const tiplhip_pair_costs& p = pairs[i];
+            TIPLHIP_REQUIRE(p.T != nullptr && p.costs != nullptr, "null transform or cost buffer");
+            TIPLHIP_REQUIRE(p.K >= 1 && p.K <= kMaxK, "K must be in [1, 4096]");
+            K += size_t(p.K);
+            TIPLHIP_REQUIRE(K <= size_t(kMaxK), "more than 4096 candidates in one launch");
+            jobs.push_back(PairJob{p.from, p.to, p.K});
+            T.insert(T.end(), p.T, p.T + 12 * size_t(p.K));
+        }
+        std::vector<double> costs(K);
+        cost_jobs(ctx, jobs, T.data(), cost, bins, costs.data());
+        size_t base = 0;
+        for (int i = 0; i < n; ++i) {
+            std::copy_n(costs.begin() + std::ptrdiff_t(base), pairs[i].K, pairs[i].costs);
+            base += size_t(pairs[i].K);
+        }
     });
 }
 
@@ -687,7 +904,7 @@ int tiplhip_cost_batch_params(tiplhip_ctx* ctx, const tiplhip_volume* from, cons
         for (int k = 0; k < K; ++k)
             affine_matrix(params + 12 * size_t(k), from->dim, from_vs, to->dim, to_vs,
                           &T[12 * size_t(k)]);
-        cost_batch(ctx, from, to, T.data(), K, cost, bins, costs);
+        cost_jobs(ctx, {PairJob{from, to, K}}, T.data(), cost, bins, costs);
     });
 }
 

@@ -17,6 +17,49 @@ void set_last_error(const std::string& msg) { g_last_error = msg; }
 void clear_last_error() { g_last_error.clear(); }
 
 // ---------------------------------------------------------------------------
+// Stream-ordered device memory
+// ---------------------------------------------------------------------------
+
+namespace {
+std::mutex g_pool_mu;
+std::vector<hipMemPool_t> g_pools;  // one per device, kept for the process lifetime
+}
+
+hipMemPool_t device_pool(int device)
+{
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    if (g_pools.size() <= size_t(device)) g_pools.resize(size_t(device) + 1, nullptr);
+    if (!g_pools[size_t(device)]) {
+        hipMemPoolProps props;
+        std::memset(&props, 0, sizeof(props));
+        props.allocType = hipMemAllocationTypePinned;
+        props.handleTypes = hipMemHandleTypeNone;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id = device;
+        hipMemPool_t pool = nullptr;
+        TIPLHIP_CHECK(hipMemPoolCreate(&pool, &props));
+        // keep freed blocks for reuse instead of returning them at every sync
+        uint64_t keep = ~uint64_t(0);
+        TIPLHIP_CHECK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+        g_pools[size_t(device)] = pool;
+    }
+    return g_pools[size_t(device)];
+}
+
+void* pool_alloc(tiplhip_ctx* ctx, size_t bytes)
+{
+    if (!bytes) return nullptr;
+    void* p = nullptr;
+    TIPLHIP_CHECK(hipMallocFromPoolAsync(&p, bytes, device_pool(ctx->device), ctx->stream));
+    return p;
+}
+
+void pool_free(tiplhip_ctx* ctx, void* p)
+{
+    if (p) (void)hipFreeAsync(p, ctx->stream);
+}
+
+// ---------------------------------------------------------------------------
 // Host geometry, float arithmetic in the reference's order
 // ---------------------------------------------------------------------------
 
@@ -291,29 +334,41 @@ tiplhip_volume* volume_alloc(tiplhip_ctx* ctx, const int* dim)
     v->ctx = ctx;
     std::copy(dim, dim + 3, v->dim);
     v->n = n;
-    hipError_t e = hipMalloc(&v->d, n);
-    if (e == hipSuccess) e = hipMalloc(&v->d_hist, sizeof(uint32_t) * (256 + 64));
-    if (e != hipSuccess) {
-        if (v->d) (void)hipFree(v->d);
-        delete v;
-        throw Error(e == hipErrorOutOfMemory ? TIPLHIP_ERR_MEMORY : TIPLHIP_ERR_DEVICE,
-                    std::string("volume allocation: ") + hipGetErrorString(e));
+    try {
+        v->d = static_cast<uint8_t*>(pool_alloc(ctx, n));
+        v->d_hist = static_cast<uint32_t*>(pool_alloc(ctx, sizeof(uint32_t) * (256 + 64)));
+    } catch (...) {
+        volume_release(v);
+        throw;
     }
     return v;
 }
 
+void volume_release(tiplhip_volume* v)
+{
+    if (!v) return;
+    pool_free(v->ctx, v->d);
+    pool_free(v->ctx, v->d_hist);
+    pool_free(v->ctx, v->d_pack);
+    delete v;
+}
+
 void volume_ensure_packed(tiplhip_ctx* ctx, tiplhip_volume* vol)
 {
+    // A volume may be used from several contexts of its device: the first
+    // user builds the packed copy and waits for it, so every later user, on
+    // any stream, reads a finished copy.
+    std::lock_guard<std::mutex> lk(vol->pack_mu);
     if (vol->d_pack) return;
     TIPLHIP_REQUIRE(vol->n * 8u < (size_t(1) << 32), "volume too large for the packed sampler");
-    void* p = nullptr;
-    TIPLHIP_CHECK(hipMalloc(&p, vol->n * 8u));
+    void* p = pool_alloc(ctx, vol->n * 8u);
     hipLaunchKernelGGL(pack8_kernel, dim3(grid_for(vol->n)), dim3(256), 0, ctx->stream, vol->d,
                        vol->n, uint32_t(vol->dim[0]), uint32_t(vol->dim[0]) * uint32_t(vol->dim[1]),
                        static_cast<uint64_t*>(p));
     hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
-        (void)hipFree(p);
+        pool_free(ctx, p);
         TIPLHIP_CHECK(e);
     }
     vol->d_pack = static_cast<uint64_t*>(p);
@@ -413,9 +468,7 @@ tiplhip_volume* volume_create(tiplhip_ctx* ctx, const void* src, bool src_on_dev
         }
         volume_compute_stats(ctx, v);
     } catch (...) {
-        (void)hipFree(v->d);
-        (void)hipFree(v->d_hist);
-        delete v;
+        volume_release(v);
         throw;
     }
     return v;
@@ -480,7 +533,6 @@ int tiplhip_ctx_destroy(tiplhip_ctx* ctx)
             for (hipEvent_t e : {ctx->t0, ctx->t1, ctx->k0, ctx->k1})
                 if (e) (void)hipEventDestroy(e);
             ctx->d_T.release();
-            ctx->d_aux.release();
             ctx->d_hist.release();
             ctx->d_sums.release();
             ctx->d_costs.release();
@@ -610,11 +662,9 @@ int tiplhip_volume_destroy(tiplhip_volume* vol)
         tiplhip_ctx* ctx = vol->ctx;
         std::lock_guard<std::recursive_mutex> lk(ctx->mu);
         DeviceGuard dg(ctx->device);
-        TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
-        (void)hipFree(vol->d);
-        (void)hipFree(vol->d_hist);
-        if (vol->d_pack) (void)hipFree(vol->d_pack);
-        delete vol;
+        // stream-ordered: the memory is reused only after the owner's
+        // queued work has finished; no device-wide synchronisation
+        volume_release(vol);
     });
 }
 
@@ -655,7 +705,7 @@ int tiplhip_volume_downsample(tiplhip_ctx* ctx, const tiplhip_volume* in, tiplhi
         TIPLHIP_REQUIRE(out != nullptr, "null output");
         *out = nullptr;
         TIPLHIP_REQUIRE(ctx && in, "null argument");
-        TIPLHIP_REQUIRE(in->ctx == ctx, "volume belongs to another context");
+        TIPLHIP_REQUIRE(in->ctx && in->ctx->device == ctx->device, "volume lives on another device");
         std::lock_guard<std::recursive_mutex> lk(ctx->mu);
         DeviceGuard dg(ctx->device);
         int od[3] = {(in->dim[0] + 1) / 2, (in->dim[1] + 1) / 2, (in->dim[2] + 1) / 2};
@@ -664,9 +714,7 @@ int tiplhip_volume_downsample(tiplhip_ctx* ctx, const tiplhip_volume* in, tiplhi
             launch_downsample_u8(ctx, in->d, in->dim, v->d);
             volume_compute_stats(ctx, v);
         } catch (...) {
-            (void)hipFree(v->d);
-            (void)hipFree(v->d_hist);
-            delete v;
+            volume_release(v);
             throw;
         }
         *out = v;

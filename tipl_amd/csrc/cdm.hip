@@ -707,20 +707,15 @@ __global__ __launch_bounds__(256) void upsample_kernel(const float* __restrict__
 // Host side
 // ---------------------------------------------------------------------------
 
+// Scratch memory of one CDM call, stream-ordered on the context's stream.
 struct DevMem {
+    tiplhip_ctx* ctx = nullptr;
     void* p = nullptr;
-    DevMem() = default;
-    explicit DevMem(size_t bytes)
-    {
-        if (bytes) TIPLHIP_CHECK(hipMalloc(&p, bytes));
-    }
+    DevMem(tiplhip_ctx* c, size_t bytes) : ctx(c), p(pool_alloc(c, bytes)) {}
     DevMem(const DevMem&) = delete;
     DevMem& operator=(const DevMem&) = delete;
-    DevMem(DevMem&& o) noexcept : p(o.p) { o.p = nullptr; }
-    ~DevMem()
-    {
-        if (p) (void)hipFree(p);
-    }
+    DevMem(DevMem&& o) noexcept : ctx(o.ctx), p(o.p) { o.p = nullptr; }
+    ~DevMem() { pool_free(ctx, p); }
     template <class T>
     T* as() const { return static_cast<T*>(p); }
 };
@@ -875,7 +870,7 @@ bool cdm_level(CdmRun& run, const std::vector<const uint8_t*>& It,
     if (!run.no_early_exit && std::min({dim[0], dim[1], dim[2]}) > run.param.min_dimension) {
         const int rd[3] = {(dim[0] + 1) / 2, (dim[1] + 1) / 2, (dim[2] + 1) / 2};
         const size_t rn = vox(rd);
-        DevMem rbuf(2 * size_t(run.nmod) * rn);
+        DevMem rbuf(ctx, 2 * size_t(run.nmod) * rn);
         std::vector<const uint8_t*> rIt, rIs;
         for (int i = 0; i < run.nmod; ++i) {
             uint8_t* t = rbuf.as<uint8_t>() + size_t(2 * i) * rn;
@@ -885,19 +880,18 @@ bool cdm_level(CdmRun& run, const std::vector<const uint8_t*>& It,
             rIt.push_back(t);
             rIs.push_back(s);
         }
-        DevMem rbest(sizeof(float) * 3 * rn);
+        DevMem rbest(ctx, sizeof(float) * 3 * rn);
         zero(ctx, rbest.p, sizeof(float) * 3 * rn);
         if (!cdm_level(run, rIt, rIs, rd, rbest.as<float>(), resolution / 2.0f)) return false;
         launch_upsample(ctx, rbest.as<float>(), field_geo(rd), g, best_d);
-        TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));  // rbuf / rbest are freed next
         if (resolution > 1.0f) return true;
     }
     if (run.stopped) return true;
 
-    DevMem cur(fb), nd(fb), pa(fb), pb(fb), Js(n);
+    DevMem cur(ctx, fb), nd(ctx, fb), pa(ctx, fb), pb(ctx, fb), Js(ctx, n);
     const uint32_t nblocks = block_count(g);
-    DevMem partials(sizeof(double) * nblocks), maxparts(sizeof(float) * nblocks);
-    DevMem scal(sizeof(float) * 16 + sizeof(unsigned int));
+    DevMem partials(ctx, sizeof(double) * nblocks), maxparts(ctx, sizeof(float) * nblocks);
+    DevMem scal(ctx, sizeof(float) * 16 + sizeof(unsigned int));
     float* d_cost = scal.as<float>();
     unsigned int* d_maxlen = reinterpret_cast<unsigned int*>(scal.as<float>() + 16);
     copy_d2d(ctx, cur.p, best_d, fb);
@@ -965,7 +959,7 @@ std::vector<const uint8_t*> volume_ptrs(tiplhip_ctx* ctx, const tiplhip_volume* 
     std::vector<const uint8_t*> out;
     for (int i = 0; i < n; ++i) {
         TIPLHIP_REQUIRE(v[i] != nullptr, "null volume");
-        TIPLHIP_REQUIRE(v[i]->ctx == ctx, "volume belongs to another context");
+        TIPLHIP_REQUIRE(v[i]->ctx && v[i]->ctx->device == ctx->device, "volume lives on another device");
         TIPLHIP_REQUIRE(std::equal(dim, dim + 3, v[i]->dim), "all CDM volumes must share one shape");
         out.push_back(v[i]->d);
     }
@@ -1016,7 +1010,7 @@ int tiplhip_cdm(tiplhip_ctx* ctx, const tiplhip_volume* const* It, const tiplhip
             tiplhip_cdm_param_default(&run.param);
         TIPLHIP_REQUIRE(run.param.max_iterations >= 0, "negative iteration cap");
         const size_t fb = sizeof(float) * 3 * vox(dim);
-        DevMem best(fb);
+        DevMem best(ctx, fb);
         zero(ctx, best.p, fb);
         const bool ok = cdm_level(run, vt, vs, dim, best.as<float>(), run.param.resolution);
         if (iterations) *iterations = run.iterations;
@@ -1048,7 +1042,7 @@ int tiplhip_cdm_level_iterations(tiplhip_ctx* ctx, const tiplhip_volume* It,
         run.param.max_iterations = iterations;
         run.no_early_exit = true;
         const size_t fb = sizeof(float) * 3 * vox(dim);
-        DevMem best(fb);
+        DevMem best(ctx, fb);
         zero(ctx, best.p, fb);
         (void)cdm_level(run, vt, vs, dim, best.as<float>(), 1.0f);
         if (field) {
@@ -1069,7 +1063,7 @@ int tiplhip_compose_displacement(tiplhip_ctx* ctx, const uint8_t* Is, const int 
         std::lock_guard<std::recursive_mutex> lk(ctx->mu);
         DeviceGuard dg(ctx->device);
         const FieldGeo g = field_geo(dim);
-        DevMem dI(g.n), dD(sizeof(float) * 3 * g.n), dJ(g.n);
+        DevMem dI(ctx, g.n), dD(ctx, sizeof(float) * 3 * g.n), dJ(ctx, g.n);
         TIPLHIP_CHECK(hipMemcpyAsync(dI.p, Is, g.n, hipMemcpyHostToDevice, ctx->stream));
         TIPLHIP_CHECK(hipMemcpyAsync(dD.p, d, sizeof(float) * 3 * g.n, hipMemcpyHostToDevice, ctx->stream));
         launch_compose(ctx, dI.as<uint8_t>(), dD.as<float>(), g, dJ.as<uint8_t>());
@@ -1088,8 +1082,8 @@ int tiplhip_cdm_gradient(tiplhip_ctx* ctx, const uint8_t* Js, const uint8_t* It,
         std::lock_guard<std::recursive_mutex> lk(ctx->mu);
         DeviceGuard dg(ctx->device);
         const FieldGeo g = field_geo(dim);
-        DevMem dJ(g.n), dI(g.n), dN(sizeof(float) * 3 * g.n);
-        DevMem part(sizeof(double) * block_count(g)), dc(sizeof(float));
+        DevMem dJ(ctx, g.n), dI(ctx, g.n), dN(ctx, sizeof(float) * 3 * g.n);
+        DevMem part(ctx, sizeof(double) * block_count(g)), dc(ctx, sizeof(float));
         TIPLHIP_CHECK(hipMemcpyAsync(dJ.p, Js, g.n, hipMemcpyHostToDevice, ctx->stream));
         TIPLHIP_CHECK(hipMemcpyAsync(dI.p, It, g.n, hipMemcpyHostToDevice, ctx->stream));
         launch_gradient(ctx, dJ.as<uint8_t>(), dI.as<uint8_t>(), g, type, 0, dN.as<float>(),
@@ -1113,7 +1107,7 @@ int tiplhip_cdm_poisson(tiplhip_ctx* ctx, float* new_d, const int dim[3], int sw
         DeviceGuard dg(ctx->device);
         const FieldGeo g = field_geo(dim);
         const size_t fb = sizeof(float) * 3 * g.n;
-        DevMem dN(fb), a(fb), b(fb), ml(sizeof(unsigned int)), mp(sizeof(float) * block_count(g));
+        DevMem dN(ctx, fb), a(ctx, fb), b(ctx, fb), ml(ctx, sizeof(unsigned int)), mp(ctx, sizeof(float) * block_count(g));
         TIPLHIP_CHECK(hipMemcpyAsync(dN.p, new_d, fb, hipMemcpyHostToDevice, ctx->stream));
         float* out = launch_poisson(ctx, dN.as<float>(), a.as<float>(), b.as<float>(), g, sweeps,
                                     ml.as<unsigned int>(), mp.as<float>());
@@ -1135,7 +1129,7 @@ int tiplhip_accumulate_displacement(tiplhip_ctx* ctx, const float* cur, float* n
         DeviceGuard dg(ctx->device);
         const FieldGeo g = field_geo(dim);
         const size_t fb = sizeof(float) * 3 * g.n;
-        DevMem dC(fb), dN(fb);
+        DevMem dC(ctx, fb), dN(ctx, fb);
         TIPLHIP_CHECK(hipMemcpyAsync(dC.p, cur, fb, hipMemcpyHostToDevice, ctx->stream));
         TIPLHIP_CHECK(hipMemcpyAsync(dN.p, new_d, fb, hipMemcpyHostToDevice, ctx->stream));
         launch_accumulate(ctx, dC.as<float>(), dN.as<float>(), scale, g);
@@ -1154,7 +1148,7 @@ int tiplhip_cdm_smooth(tiplhip_ctx* ctx, const float* d, float* dd, const int di
         DeviceGuard dg(ctx->device);
         const FieldGeo g = field_geo(dim);
         const size_t fb = sizeof(float) * 3 * g.n;
-        DevMem dD(fb), dO(fb);
+        DevMem dD(ctx, fb), dO(ctx, fb);
         TIPLHIP_CHECK(hipMemcpyAsync(dD.p, d, fb, hipMemcpyHostToDevice, ctx->stream));
         launch_smooth(ctx, dD.as<float>(), dO.as<float>(), smoothing, g);
         TIPLHIP_CHECK(hipMemcpyAsync(dd, dO.p, fb, hipMemcpyDeviceToHost, ctx->stream));
@@ -1172,7 +1166,7 @@ int tiplhip_upsample_field(tiplhip_ctx* ctx, const float* in, const int in_dim[3
         std::lock_guard<std::recursive_mutex> lk(ctx->mu);
         DeviceGuard dg(ctx->device);
         const FieldGeo gi = field_geo(in_dim), go = field_geo(out_dim);
-        DevMem dI(sizeof(float) * 3 * gi.n), dO(sizeof(float) * 3 * go.n);
+        DevMem dI(ctx, sizeof(float) * 3 * gi.n), dO(ctx, sizeof(float) * 3 * go.n);
         TIPLHIP_CHECK(hipMemcpyAsync(dI.p, in, sizeof(float) * 3 * gi.n, hipMemcpyHostToDevice, ctx->stream));
         launch_upsample(ctx, dI.as<float>(), gi, go, dO.as<float>());
         TIPLHIP_CHECK(hipMemcpyAsync(out, dO.p, sizeof(float) * 3 * go.n, hipMemcpyDeviceToHost, ctx->stream));

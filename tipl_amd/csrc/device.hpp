@@ -79,17 +79,31 @@ __device__ inline uint32_t sample_u8(__amdgpu_buffer_rsrc_t r, const VolGeo& g, 
 
 // The same interpolation with the 8 taps fetched as one 8-byte word from the
 // packed copy of the volume (byte k of word v = tap k of the cell at v, in
-// the reference's dindex order).  Same arithmetic, one gather per sample.
+// the reference's dindex order).  Same arithmetic, one gather per sample,
+// and fewer instructions for the same results:
+//   * inside iff 0 <= x < w-1 (and y, z) is one unsigned compare per axis:
+//     non-negative floats order like their bit patterns, and negative values
+//     and NaNs have patterns above any non-negative finite float.  This
+//     needs x != -0.0 (which the reference counts as inside): callers build
+//     x = ((x*r0 + y*r1) + z*r2) + s0 with every zero shift component s0
+//     stored as +0.0, and a round-to-nearest sum with a +0.0 term, or of
+//     two non-zero terms, is never -0.0.  (The sign of a zero x does not
+//     change the sampled value.)
+//   * x - float(uint(x)) is exactly x - floor(x) for 0 <= x < 2^24, which
+//     v_fract_f32 returns in one instruction.
+//   * taps and weights are non-negative, so the reference's `v <= 0 ? 0 :
+//     T(v)` is plain truncation.
 __device__ inline uint32_t sample_u8_packed(__amdgpu_buffer_rsrc_t r8, const VolGeo& g, float x,
                                             float y, float z)
 {
-    // min(x,y,z) >= 0 is x,y,z >= 0 (a NaN coordinate fails its upper test)
-    const bool in = (fminf(fminf(x, y), z) >= 0.0f) & (x < g.wm1) & (y < g.hm1) & (z < g.dm1);
+    const bool in = (__float_as_uint(x) < __float_as_uint(g.wm1)) &
+                    (__float_as_uint(y) < __float_as_uint(g.hm1)) &
+                    (__float_as_uint(z) < __float_as_uint(g.dm1));
     if (!in) return 0u;
     const uint32_t ix = uint32_t(x), iy = uint32_t(y), iz = uint32_t(z);
-    const float p0 = x - float(ix);
-    const float p1 = y - float(iy);
-    const float p2 = z - float(iz);
+    const float p0 = __builtin_amdgcn_fractf(x);
+    const float p1 = __builtin_amdgcn_fractf(y);
+    const float p2 = __builtin_amdgcn_fractf(z);
     const float n0 = 1.0f - p0;
     const float n1 = 1.0f - p1;
     const float n2 = 1.0f - p2;
@@ -109,7 +123,7 @@ __device__ inline uint32_t sample_u8_packed(__amdgpu_buffer_rsrc_t r8, const Vol
     acc = acc + float((hi >> 8) & 0xffu) * (p0n1 * p2);
     acc = acc + float((hi >> 16) & 0xffu) * (n0p1 * p2);
     acc = acc + float(hi >> 24) * (p0p1 * p2);
-    return acc <= 0.0f ? 0u : uint32_t(acc);
+    return uint32_t(acc);
 }
 
 // Hardware dispatches workgroup b of a 1-D grid to XCD b % 8.  This remap

@@ -147,8 +147,7 @@ struct tiplhip_ctx {
     hipStream_t stream = nullptr;
     std::recursive_mutex mu;
     // scratch for the affine batch path
-    tiplhip::DeviceBuffer d_T;       // K x 12 floats
-    tiplhip::DeviceBuffer d_aux;     // per-candidate crop offsets, flags
+    tiplhip::DeviceBuffer d_T;       // K x 12 floats, crop flags, segment table
     tiplhip::DeviceBuffer d_hist;    // K x bins^2 int32
     tiplhip::DeviceBuffer d_sums;    // K x 4 uint64
     tiplhip::DeviceBuffer d_costs;   // K doubles
@@ -173,6 +172,7 @@ struct tiplhip_volume {
     // holds the bytes at v, v+1, v+w, v+w+1, v+wh, v+wh+1, v+wh+w, v+wh+w+1
     // (0 past the array end), so one 8-byte gather fetches a trilinear cell.
     uint64_t* d_pack = nullptr;
+    std::mutex pack_mu;              // guards the lazy build of d_pack
     int64_t sum = 0, sum_sq = 0;     // exact
     uint32_t hist256[256];
     uint32_t hist64[64];
@@ -180,13 +180,30 @@ struct tiplhip_volume {
 
 namespace tiplhip {
 
+// Stream-ordered device memory from a per-device pool (hipMallocFromPoolAsync /
+// hipFreeAsync on ctx->stream).  Unlike hipFree, a free does not synchronise
+// the device, so registrations running on other streams are not stalled.
+// Memory allocated here and used on another context's stream must be
+// finished there before it is freed (every blocking entry point is).
+void* pool_alloc(tiplhip_ctx* ctx, size_t bytes);
+void pool_free(tiplhip_ctx* ctx, void* p);
+
 // Allocates a volume object with device storage for `dim` voxels.
 tiplhip_volume* volume_alloc(tiplhip_ctx* ctx, const int* dim);
+// Frees a volume's device storage (stream-ordered, on its owner's stream) and the object.
+void volume_release(tiplhip_volume* v);
 // downsample_with_padding of a device uint8 volume into `out` ((dim+1)/2).
 void launch_downsample_u8(tiplhip_ctx* ctx, const uint8_t* in, const int* dim, uint8_t* out);
 // Builds vol->d_pack if it does not exist yet.
 void volume_ensure_packed(tiplhip_ctx* ctx, tiplhip_volume* vol);
 // Computes sum / sum_sq / histograms of vol->d on the device and stores them.
 void volume_compute_stats(tiplhip_ctx* ctx, tiplhip_volume* vol);
+
+// filter::gaussian (filter/gaussian.hpp:18-89) of a device float image with
+// `channels` interleaved channels (1 or 3), ping-ponging between `a` and
+// `tmp`; returns whichever of the two holds the result.
+constexpr unsigned kMaxGaussianBandwidth = 16;
+float* launch_gaussian(tiplhip_ctx* ctx, float* a, float* tmp, const int* dim, int channels,
+                       unsigned bandwidth);
 
 }  // namespace tiplhip

@@ -124,13 +124,17 @@ def tissue_pair(seed=1000, shape=(182, 218, 182)):
     return fixed, moving, truth
 
 
-def rbf_field(shape, n=30, max_disp=6.0, seed=2000) -> np.ndarray:
+def rbf_field(shape, n=30, max_disp=6.0, seed=2000, step=1) -> np.ndarray:
     """Smooth displacement (z, y, x, 3) in voxels, components (dx, dy, dz),
     sum of n Gaussian radial-basis displacements (sigma 8-20) scaled so that
-    the largest displacement is max_disp; raises if the Jacobian is not > 0."""
+    the largest displacement is max_disp; raises if the Jacobian is not > 0.
+    step > 1 evaluates the field on every step-th voxel and interpolates it
+    linearly (a cheaper field for large volumes; sigma >= 8 keeps it smooth)."""
     rng = np.random.default_rng(seed)
-    zz, yy, xx = np.meshgrid(*[np.arange(s, dtype=np.float32) for s in shape], indexing="ij")
-    d = np.zeros(shape + (3,), np.float32)
+    grids = [np.arange(0, s, step, dtype=np.float32) for s in shape]
+    zz, yy, xx = np.meshgrid(*grids, indexing="ij")
+    cshape = zz.shape
+    d = np.zeros(cshape + (3,), np.float32)
     for _ in range(n):
         c = [rng.uniform(0, s) for s in shape]
         sig = rng.uniform(8, 20)
@@ -139,16 +143,22 @@ def rbf_field(shape, n=30, max_disp=6.0, seed=2000) -> np.ndarray:
         d += w[..., None] * vec[None, None, None, :]
     mag = np.sqrt((d ** 2).sum(-1)).max()
     d *= max_disp / mag
-    # Jacobian determinant of x -> x + d(x), FP64
-    g = [np.gradient(d[..., k].astype(np.float64)) for k in range(3)]  # g[k][axis], axis order z,y,x
-    J = np.empty(shape + (3, 3))
+    # Jacobian determinant of x -> x + d(x), FP64, in voxels of the full grid
+    g = [np.gradient(d[..., k].astype(np.float64), float(step)) for k in range(3)]
+    J = np.empty(cshape + (3, 3))
     for k in range(3):          # component (x, y, z)
         for j in range(3):      # derivative wrt (x, y, z)
             J[..., k, j] = g[k][2 - j] + (1.0 if k == j else 0.0)
     det = np.linalg.det(J)
     if det.min() <= 0:
         raise ValueError("generated field is not diffeomorphic")
-    return d
+    if step == 1:
+        return d
+    # linear interpolation onto the full grid (positions i / step of the coarse grid)
+    coords = np.meshgrid(*[np.minimum(np.arange(s) / step, cs - 1) for s, cs in zip(shape, cshape)],
+                         indexing="ij")
+    return np.stack([ndimage.map_coordinates(d[..., k], coords, order=1, mode="nearest")
+                     for k in range(3)], -1).astype(np.float32)
 
 
 def warp_field(vol: np.ndarray, d: np.ndarray) -> np.ndarray:
@@ -164,6 +174,24 @@ def warped_pair(seed=2000, shape=(160, 192, 160)):
     d = rbf_field(shape, 30, 6.0, seed)
     moving = warp_field(fixed, d)
     return fixed, moving, d
+
+
+def serving_pair(seed, shape, nonlinear=False, dtype=np.float32):
+    """Config 4 request: (fixed, moving, truth_param) of `shape` and `dtype`
+    (float32 or int16): the tissue phantom, and as moving its monotone remap,
+    for affine+CDM requests warped by an RBF field (max 4 voxels), then moved
+    by a random affine."""
+    rng = np.random.default_rng(seed)
+    fixed = tissue(shape, seed)
+    moving = monotone_remap(fixed, rng)
+    if nonlinear:
+        moving = warp_field(moving, rbf_field(shape, 30, 4.0, seed + 1, step=4))
+    truth = random_affine(rng)
+    moving = warp_affine(moving, truth)
+    if np.dtype(dtype) == np.int16:
+        fixed = np.clip(np.rint(fixed), -32768, 32767).astype(np.int16)
+        moving = np.clip(np.rint(moving), -32768, 32767).astype(np.int16)
+    return fixed.astype(dtype), moving.astype(dtype), truth
 
 
 def quantize(v: np.ndarray, upper: float = 255.99) -> np.ndarray:
