@@ -153,6 +153,51 @@ def serving_request(i):
     return fixed, moving, nonlinear
 
 
+def batch_pair(i):
+    """Pair i of the multi-pair workload: a cube with side uniform in [64, 192],
+    the tissue phantom and its remapped, affinely moved copy (seed 3000 + i)."""
+    from tipl_amd import phantoms
+    rng = np.random.default_rng(3000 + i)
+    side = int(rng.integers(64, 193))
+    fixed, moving, _ = phantoms.serving_pair(3000 + i, (side, side, side), False, np.float32)
+    return fixed, moving
+
+
+def make_batch_pairs(n):
+    import multiprocessing as mp
+    with mp.get_context("fork").Pool(min(8, n)) as pool:
+        return pool.map(batch_pair, range(n))
+
+
+def multi_pair_bench(ctx, pairs, K=64, reps=5):
+    """32 pairs of uneven cubes, K candidates each: one multi-pair launch per
+    step (tiplhip_cost_batch_pairs) against one launch per pair, MI 64 bins."""
+    import tipl_amd
+    from tipl_amd import phantoms
+    rng = np.random.default_rng(3100)
+    vols, voxels = [], 0
+    for f, m in pairs:
+        vf, vm = ctx.volume(f, 255.99), ctx.volume(m, 255.99)
+        P = phantoms.candidate_params(rng, K)
+        Ts = np.stack([tipl_amd.affine_matrix(p, f.shape, (1, 1, 1), m.shape, (1, 1, 1)) for p in P])
+        vols.append((vf, vm, Ts))
+        voxels += f.size * K
+    batched = ctx.cost_batch_pairs(vols, "mi", 64)
+    separate = [ctx.cost_batch(vf, vm, Ts, "mi", 64) for vf, vm, Ts in vols]
+    equal = all(np.array_equal(a, b) for a, b in zip(batched, separate))
+    t_b = time_steps(ctx, lambda: ctx.cost_batch_pairs(vols, "mi", 64), reps, 1) / reps
+    t_s = time_steps(ctx, lambda: [ctx.cost_batch(vf, vm, Ts, "mi", 64) for vf, vm, Ts in vols],
+                     reps, 1) / reps
+    for vf, vm, _ in vols:
+        vf.free()
+        vm.free()
+    return {"pairs": len(pairs), "candidates_per_pair": K, "sides": [int(f.shape[0]) for f, _ in pairs],
+            "batched_ms": round(t_b * 1e3, 3), "separate_ms": round(t_s * 1e3, 3),
+            "batched_gvox_s": round(voxels / t_b / 1e9, 2),
+            "separate_gvox_s": round(voxels / t_s / 1e9, 2),
+            "batched_over_separate": round(t_s / t_b, 2), "results_equal": bool(equal)}
+
+
 def make_serving_requests(n):
     """Generated in worker processes before the GPU is touched (fork start)."""
     import multiprocessing as mp
@@ -162,8 +207,11 @@ def make_serving_requests(n):
 
 def serving_bench(reqs, slots, rate_factor, seed=4000):
     """The plain single-pair path (one registration at a time, tiplhip_register)
-    against the serving loop fed by a Poisson stream at rate_factor x the
-    plain path's rate; every served result is compared with the plain one."""
+    against the serving loop on the same requests: (1) saturation -- every
+    request submitted at once, throughput = completed / (last completion -
+    first submit); (2) a Poisson stream offered at rate_factor x the plain
+    path's rate, with latency percentiles (submit to result).  Every served
+    result is compared with the plain one, bit for bit."""
     import tipl_amd
     ctx = tipl_amd.Context(0)
     tipl_amd.register(ctx, reqs[0][0], reqs[0][1])  # warm-up (code objects, pools)
@@ -174,36 +222,86 @@ def serving_bench(reqs, slots, rate_factor, seed=4000):
     plain_dt = time.perf_counter() - t0
     plain_rate = len(reqs) / plain_dt
     ctx.close()
+
+    def run(arrivals):
+        with tipl_amd.Server(0, slots=slots, hbm_budget=192 << 30) as srv:
+            start = time.perf_counter()
+            tickets = []
+            for (f, m, nl), at in zip(reqs, arrivals):
+                delay = start + at - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                tickets.append(srv.submit(f, m, nonlinear=nl, want_field=False))
+            served = [srv.wait(t) for t in tickets]
+            st = srv.stats()
+        same = all(np.array_equal(a["T"].view(np.uint32), b["T"].view(np.uint32)) and
+                   a["evaluations"] == b["evaluations"] and a["cdm_iterations"] == b["cdm_iterations"]
+                   for a, b in zip(served, plain))
+        return st, same
+
+    sat, same1 = run(np.zeros(len(reqs)))
     rate = rate_factor * plain_rate
     rng = np.random.default_rng(seed)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
-    with tipl_amd.Server(0, slots=slots, hbm_budget=192 << 30) as srv:
-        start = time.perf_counter()
-        tickets = []
-        for (f, m, nl), at in zip(reqs, arrivals):
-            delay = start + at - time.perf_counter()
-            if delay > 0:
-                time.sleep(delay)
-            tickets.append(srv.submit(f, m, nonlinear=nl, want_field=False))
-        served = [srv.wait(t) for t in tickets]
-        st = srv.stats()
-    same = all(np.array_equal(a["T"].view(np.uint32), b["T"].view(np.uint32)) and
-               a["evaluations"] == b["evaluations"] and a["cdm_iterations"] == b["cdm_iterations"]
-               for a, b in zip(served, plain))
+    stream, same2 = run(np.cumsum(rng.exponential(1.0 / rate, len(reqs))))
+    split = {k: round(float(np.mean([p[k] for p in plain])), 2) for k in ("prep_ms", "affine_ms", "cdm_ms")}
     return {
-        "served_regs_per_s": round(st["regs_per_s"], 3),
-        "latency_p50_ms": round(st["latency_p50_ms"], 1),
-        "latency_p95_ms": round(st["latency_p95_ms"], 1),
-        "latency_p99_ms": round(st["latency_p99_ms"], 1),
+        "served_regs_per_s": round(sat["regs_per_s"], 3),
         "plain_regs_per_s": round(plain_rate, 3),
-        "served_over_plain": round(st["regs_per_s"] / plain_rate, 2),
+        "served_over_plain": round(sat["regs_per_s"] / plain_rate, 2),
+        "saturation_latency_p50_ms": round(sat["latency_p50_ms"], 1),
+        "saturation_latency_p99_ms": round(sat["latency_p99_ms"], 1),
         "offered_rate_per_s": round(rate, 3),
+        "latency_p50_ms": round(stream["latency_p50_ms"], 1),
+        "latency_p95_ms": round(stream["latency_p95_ms"], 1),
+        "latency_p99_ms": round(stream["latency_p99_ms"], 1),
+        "stream_regs_per_s": round(stream["regs_per_s"], 3),
         "requests": len(reqs), "slots": slots,
         "nonlinear_requests": int(sum(nl for _, _, nl in reqs)),
-        "batch_launches": int(st["batch_launches"]), "batched_pairs": int(st["batched_pairs"]),
-        "hbm_peak_gib": round(st["hbm_peak"] / 2 ** 30, 2),
-        "results_equal_plain": bool(same),
+        "batch_launches": int(sat["batch_launches"]), "batched_pairs": int(sat["batched_pairs"]),
+        "hbm_peak_gib": round(sat["hbm_peak"] / 2 ** 30, 2),
+        "results_equal_plain": bool(same1 and same2),
+        "plain_mean_split_ms": split,
     }
+
+
+def cdm_config2(ctx):
+    """CDM at 160x192x160: the tissue phantom warped by 30 RBF displacements
+    (max 6 voxels, Jacobian > 0); fixed schedule of 3 levels x 48 iterations
+    with the field smoothed by a sigma-2 binomial gaussian each iteration.
+    Reports iterations/s over the whole run and the recovered field's error:
+    moving(x) = fixed(x + d(x)), and the CDM field f should satisfy
+    f(x) + d(x + f(x)) = 0."""
+    from scipy import ndimage
+    from tipl_amd import phantoms
+    fixed, moving, d = phantoms.warped_pair(2000, (160, 192, 160))
+    vt, vs = ctx.volume(fixed, 255.99), ctx.volume(moving, 255.99)
+    p = ctx.cdm_param(levels=3, level_iterations=48, field_bandwidth=8)
+    ctx.cdm(vt, vs, p)  # warm-up
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    rc, f, n = ctx.cdm(vt, vs, p)
+    dt = time.perf_counter() - t0
+    # error on every 2nd voxel inside the phantom
+    sl = tuple(slice(0, s, 2) for s in fixed.shape)
+    zz, yy, xx = np.meshgrid(*[np.arange(0, s, 2, dtype=np.float64) for s in fixed.shape], indexing="ij")
+    fs = f[sl].astype(np.float64)
+    coords = [zz + fs[..., 2], yy + fs[..., 1], xx + fs[..., 0]]
+    dd = np.stack([ndimage.map_coordinates(d[..., k], coords, order=1, mode="nearest")
+                   for k in range(3)], -1)
+    It8, Is8 = vt.read().astype(np.float64), vs.read()
+    Js8 = ctx.compose(Is8, f).astype(np.float64)
+    corr_before = float(np.corrcoef(It8.reshape(-1), Is8.astype(np.float64).reshape(-1))[0, 1])
+    corr_after = float(np.corrcoef(It8.reshape(-1), Js8.reshape(-1))[0, 1])
+    inside = fixed[sl] > 150.0
+    err = np.sqrt(((fs + dd) ** 2).sum(-1))[inside]
+    mag = np.sqrt((d[sl] ** 2).sum(-1))[inside]
+    return {"shape": [160, 192, 160], "levels": 3, "iterations_per_level": 48,
+            "field_sigma_voxels": 2.0, "status": int(rc), "iterations": int(n),
+            "seconds": round(dt, 4), "iter_per_s": round(n / dt, 2),
+            "field_rms_error_voxels": round(float(np.sqrt((err ** 2).mean())), 4),
+            "true_displacement_rms_voxels": round(float(np.sqrt((mag ** 2).mean())), 4),
+            "image_correlation_before": round(corr_before, 4),
+            "image_correlation_after": round(corr_after, 4)}
 
 
 def main():
@@ -213,11 +311,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cdm-iters", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--serve-requests", type=int, default=24,
+    ap.add_argument("--serve-requests", type=int, default=48,
                     help="requests in the serving leg (0 skips it)")
-    ap.add_argument("--serve-slots", type=int, default=8)
-    ap.add_argument("--serve-rate", type=float, default=4.0,
-                    help="Poisson arrival rate as a multiple of the plain path's rate")
+    ap.add_argument("--serve-slots", type=int, default=16)
+    ap.add_argument("--batch-pairs", type=int, default=32,
+                    help="pairs in the multi-pair launch leg (0 skips it)")
+    ap.add_argument("--serve-rate", type=float, default=1.0,
+                    help="Poisson arrival rate of the latency run, as a multiple of the plain "
+                         "path's rate")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (each pair or "
@@ -228,6 +329,10 @@ def main():
     if args.serve_requests > 0:
         log(f"generating {args.serve_requests} serving requests")
         reqs = make_serving_requests(args.serve_requests)
+    bpairs = []
+    if args.batch_pairs > 0:
+        log(f"generating {args.batch_pairs} uneven pairs")
+        bpairs = make_batch_pairs(args.batch_pairs)
 
     import tipl_amd
     from tipl_amd import phantoms
@@ -242,6 +347,14 @@ def main():
     P, Ts = make_candidates(rng, CANDIDATES, shape, shape)
 
     res = {}
+    # --- launch shapes from the autotuner (results do not depend on them) -----
+    shapes = {}
+    for cost, bins in (("mi", 64), ("mi", 256), ("corr", 256)):
+        for K in (64, 1):
+            zc, pc, _ = ctx.autotune(vf, vm, Ts[:K], cost, bins, reps=3)
+            shapes[f"{cost}{bins if cost == 'mi' else ''}_k{K}"] = [zc, pc]
+    log(f"autotuned launch shapes (planes per item, workgroups per CU): {shapes}")
+
     # --- headline: K=64 MI candidates per launch, 64x64 bins ------------------
     ctx.set_kernel_timing(True)
     dt = time_steps(ctx, lambda: ctx.cost_batch(vf, vm, Ts, "mi", 64), args.steps, args.warmup)
@@ -294,6 +407,16 @@ def main():
                 "algorithmic_bytes": alg_bytes,
                 "traffic_source": prof["source"] if prof else None}
 
+    mp_res = None
+    if bpairs:
+        log("multi-pair launches")
+        mp_res = multi_pair_bench(ctx, bpairs)
+        log(f"multi-pair {mp_res}")
+
+    log("CDM fixed schedule at 160x192x160")
+    cdm2 = cdm_config2(ctx)
+    log(f"cdm config 2 {cdm2}")
+
     serve = None
     if reqs:
         log("serving leg")
@@ -327,7 +450,7 @@ def main():
         "data": "synthetic (seeded 4-class tissue phantom, monotone remap + random affine)",
         "config": {"workload": "affine MI cost, 64 candidates per launch, 64x64-bin joint "
                                "histogram, 182x218x182 uint8", "shape": list(shape),
-                   "candidates_per_step": CANDIDATES, "bins": 64},
+                   "candidates_per_step": CANDIDATES, "bins": 64, "launch_shapes": shapes},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "cdm_iter_per_s": round(cdm_ips, 3),
@@ -335,6 +458,8 @@ def main():
         "cdm_cpu_baseline": cpu_cdm,
         "corr_cpu_baseline": cpu_corr,
         "hbm_copy_gbps": round(copy_gbps, 1),
+        "cdm_fixed_schedule": cdm2,
+        "multi_pair": mp_res,
         "serving": serve,
         **{k: round(v, 4) for k, v in res.items()},
     }

@@ -30,6 +30,12 @@ struct cdm_param {
     std::string gradient_type;                   // per modality 'r' (default) or 'd'
     std::vector<std::pair<vector<3>, vector<3>>> anchor;  // not used by the loop
     int max_iterations = 200;                    // cdm() hard-codes 200 (reg/cdm.hpp:418)
+    // fixed schedule, an extension (off by default): exactly `levels` pyramid
+    // levels of `level_iterations` iterations each, the field smoothed by
+    // filter::gaussian(field_bandwidth) when that is > 0
+    int levels = 0;
+    int level_iterations = 0;
+    int field_bandwidth = 0;
 };
 
 namespace detail {
@@ -44,6 +50,9 @@ inline tiplhip_cdm_param to_c(const cdm_param& p)
     c.min_dimension = int(p.min_dimension);
     c.max_iterations = p.max_iterations;
     for (size_t i = 0; i < 8; ++i) c.gradient_type[i] = i < p.gradient_type.size() ? p.gradient_type[i] : 'r';
+    c.levels = p.levels;
+    c.level_iterations = p.level_iterations;
+    c.field_bandwidth = p.field_bandwidth;
     return c;
 }
 

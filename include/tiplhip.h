@@ -109,6 +109,14 @@ int tiplhip_volume_create(tiplhip_ctx* ctx, const void* host, int dtype, const i
 /* Upload from a device pointer (no host copy). */
 int tiplhip_volume_create_device(tiplhip_ctx* ctx, const void* device_ptr, int dtype,
                                  const int dim[3], float upper_limit, tiplhip_volume** out);
+/* reg::subject_image_pre (reg/mm_reg.hpp:14-20) on the device: the image
+ * (uint8, int16 or float32, converted to float) smoothed by
+ * filter::gaussian (bandwidth 1), normalised by
+ * segmentation::normalize_otsu_median(I, 255.99f) (numerical/otsu.hpp:96-136)
+ * and truncated to uint8 -- bit-exact.  (Template images use
+ * tiplhip_volume_create with upper_limit 255.999, reg/mm_reg.hpp:28-34.) */
+int tiplhip_volume_create_subject(tiplhip_ctx* ctx, const void* host, int dtype, const int dim[3],
+                                  tiplhip_volume** out);
 int tiplhip_volume_destroy(tiplhip_volume* vol);
 int tiplhip_volume_dims(const tiplhip_volume* vol, int dim[3]);
 int tiplhip_volume_read(tiplhip_ctx* ctx, const tiplhip_volume* vol, uint8_t* host);
@@ -156,6 +164,17 @@ int tiplhip_mi_hist(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_
 int tiplhip_cost_batch(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_volume* to,
                        const float* T, int K, int cost, int bins, double* costs);
 
+/* Autotuner of the cost kernels' launch shape (no reference counterpart):
+ * times the histogram (MI) or sums (correlation) kernel for these K
+ * transforms over a grid of work-item depths (8, 16, 32 planes) and
+ * workgroups per CU, `reps` launches each, and keeps the fastest for every
+ * later single-pair launch on ctx with the same shapes, K, cost and bins.
+ * Results do not depend on the shape (integer accumulation).  Outputs may be
+ * NULL. */
+int tiplhip_autotune(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_volume* to,
+                     const float* T, int K, int cost, int bins, int reps, int* zc, int* per_cu,
+                     double* best_ms);
+
 /* One image pair's candidates in a multi-pair launch. */
 typedef struct tiplhip_pair_costs {
     const tiplhip_volume* from;
@@ -194,14 +213,24 @@ int tiplhip_matrix_inverse(float T[12]);
 /* ---- CDM ------------------------------------------------------------------ */
 
 /* cdm_param (reg/cdm.hpp:36-44) plus the iteration cap that cdm() hard-codes
- * as 200 (reg/cdm.hpp:418). */
+ * as 200 (reg/cdm.hpp:418), plus a fixed schedule (an extension with no
+ * reference counterpart, off by default): with levels > 0 the pyramid has
+ * exactly `levels` levels, every level runs exactly `level_iterations`
+ * iterations of the loop body with no early exit and keeps its running
+ * field, and the field is smoothed each iteration by
+ * filter::gaussian(field_bandwidth) instead of cdm_smooth when
+ * field_bandwidth > 0 (bandwidth b has variance b/2: 8 is sigma 2).
+ * resolution, min_dimension and max_iterations do not apply then. */
 typedef struct {
     float resolution;       /* 2.0 */
     float speed;            /* 0.3 */
-    float smoothing;        /* 0.05 */
+    float smoothing;        /* 0.05 (in the fixed schedule: cdm_smooth's weight) */
     int min_dimension;      /* 16 */
     int max_iterations;     /* 200 */
     char gradient_type[8];  /* per modality: 'r' (default) or 'd' */
+    int levels;             /* 0: the reference's pyramid rule */
+    int level_iterations;   /* iterations per level in the fixed schedule */
+    int field_bandwidth;    /* 0: cdm_smooth; 1..16: gaussian smoothing of the field */
 } tiplhip_cdm_param;
 
 void tiplhip_cdm_param_default(tiplhip_cdm_param* p);
@@ -276,6 +305,10 @@ typedef struct tiplhip_reg_request {
     int nonlinear;            /* 0: affine only; 1: affine then CDM */
     tiplhip_cdm_param cdm;
     float* field;             /* host, vox(fixed_dim) x 3 floats, or NULL */
+    int preprocess;           /* 0: both images by normalize_upper_lower2(255.99), the
+                                 linear cost functor's rule (reg/linear.hpp:65-71);
+                                 1: mm_reg's rule -- fixed by template_image_pre,
+                                 moving by subject_image_pre (reg/mm_reg.hpp:14-34) */
 } tiplhip_reg_request;
 
 typedef struct tiplhip_reg_result {
@@ -287,6 +320,9 @@ typedef struct tiplhip_reg_result {
     int64_t evaluations;      /* affine cost evaluations */
     double queue_ms;          /* submit to start (served requests; 0 on the plain path) */
     double run_ms;            /* start to done */
+    double prep_ms;           /* upload, quantisation, host mask / moment estimate */
+    double affine_ms;         /* the linear() optimisation */
+    double cdm_ms;            /* resample and cdm (nonlinear requests) */
 } tiplhip_reg_result;
 
 void tiplhip_reg_request_default(tiplhip_reg_request* r);

@@ -80,6 +80,13 @@ class Oracle(_Lib):
         self._smooth = self._fn("orc_cdm_smooth", None, _f32p, _f32p, _i32p, C.c_float)
         self._improved = self._fn("orc_cdm_improved", C.c_int, _f32p, P(C.c_int))
         self._gauss = self._fn("orc_gaussian_f32", None, _f32p, _i32p, C.c_int, C.c_uint)
+        self._subj = self._fn("orc_subject_pre", None, _f32p, _i32p, _u8p)
+        self._otsu_median = self._fn("orc_otsu_median", C.c_float, _f32p, C.c_size_t)
+        self._median = self._fn("orc_median", C.c_float, _f32p, C.c_size_t)
+        self._otsu = self._fn("orc_otsu_threshold", C.c_float, _f32p, C.c_size_t)
+        self._cdm_fixed = self._fn("orc_cdm_fixed", None, P(C.c_void_p), P(C.c_void_p), C.c_int,
+                                   _i32p, C.c_int, C.c_int, C.c_float, C.c_float, C.c_uint,
+                                   C.c_char, _f32p, P(C.c_int))
         self._cdm = self._fn("orc_cdm", C.c_int, P(C.c_void_p), P(C.c_void_p), C.c_int, _i32p,
                              C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, C.c_char, _f32p,
                              P(C.c_int))
@@ -221,6 +228,39 @@ class Oracle(_Lib):
         r = self._improved(c, C.byref(n))
         return bool(r), list(map(float, c[: n.value]))
 
+    def subject_pre(self, img) -> np.ndarray:
+        """reg::subject_image_pre of a float32 (z, y, x) image -> uint8."""
+        img = np.ascontiguousarray(img, np.float32)
+        out = np.zeros(img.shape, np.uint8)
+        self._subj(img.reshape(-1), _dims(img), out.reshape(-1))
+        return out
+
+    def otsu_median(self, v) -> float:
+        v = np.ascontiguousarray(v, np.float32).reshape(-1)
+        return float(self._otsu_median(v, v.size))
+
+    def otsu_threshold(self, v) -> float:
+        v = np.ascontiguousarray(v, np.float32).reshape(-1)
+        return float(self._otsu(v, v.size))
+
+    def median(self, v) -> float:
+        v = np.ascontiguousarray(v, np.float32).reshape(-1)
+        return float(self._median(v, v.size))
+
+    def cdm_fixed(self, It, Is, levels=3, level_iterations=48, speed=0.3, smoothing=0.05,
+                  bandwidth=8, gradient_type="r"):
+        """The fixed-schedule CDM extension; returns (field, iterations)."""
+        It = [np.ascontiguousarray(v, np.uint8) for v in (It if isinstance(It, list) else [It])]
+        Is = [np.ascontiguousarray(v, np.uint8) for v in (Is if isinstance(Is, list) else [Is])]
+        n = len(It)
+        pt = (C.c_void_p * n)(*[v.ctypes.data for v in It])
+        ps = (C.c_void_p * n)(*[v.ctypes.data for v in Is])
+        field = np.zeros(It[0].shape + (3,), np.float32)
+        it = C.c_int(0)
+        self._cdm_fixed(pt, ps, n, _dims(It[0]), levels, level_iterations, speed, smoothing,
+                        bandwidth, gradient_type.encode(), field, C.byref(it))
+        return field, it.value
+
     def cdm(self, It, Is, resolution=2.0, speed=0.3, smoothing=0.05, min_dimension=16,
             max_iterations=200, gradient_type="r"):
         if isinstance(It, np.ndarray):
@@ -269,6 +309,11 @@ class Reference(_Lib):
         self._linear = self._fn("tref_linear", C.c_float, _u8p, _i32p, _f32p, _u8p, _i32p, _f32p,
                                 C.c_int, C.c_int, _f32p)
         self._gauss = self._fn("tref_gaussian_f32", None, _f32p, _i32p, C.c_int, C.c_uint)
+        self._cdm_fixed = self._fn("tref_cdm_fixed", None, _u8p, _u8p, _i32p, C.c_int, C.c_int,
+                                   C.c_float, C.c_uint, _f32p)
+        self._subj = self._fn("tref_subject_pre", None, _f32p, _i32p, _u8p)
+        self._otsu_median = self._fn("tref_otsu_median", C.c_float, _f32p, C.c_size_t)
+        self._median = self._fn("tref_median", C.c_float, _f32p, C.c_size_t)
         self._cdm_iters = self._fn("tref_cdm_iterations", C.c_double, _u8p, _u8p, _i32p, C.c_int,
                                    C.c_void_p)
 
@@ -389,6 +434,27 @@ class Reference(_Lib):
         dd = _f3(dd).copy()
         self._smooth(d, dd, np.array(d.shape[:3][::-1], np.int32), s)
         return dd
+
+    def subject_pre(self, img):
+        img = np.ascontiguousarray(img, np.float32)
+        out = np.zeros(img.shape, np.uint8)
+        self._subj(img.reshape(-1), _dims(img), out.reshape(-1))
+        return out
+
+    def otsu_median(self, v):
+        v = np.ascontiguousarray(v, np.float32).reshape(-1)
+        return float(self._otsu_median(v, v.size))
+
+    def median(self, v):
+        v = np.ascontiguousarray(v, np.float32).reshape(-1)
+        return float(self._median(v, v.size))
+
+    def cdm_fixed(self, It, Is, levels=3, level_iterations=48, smoothing=0.8, bandwidth=8):
+        It = np.ascontiguousarray(It, np.uint8)
+        Is = np.ascontiguousarray(Is, np.uint8)
+        field = np.zeros(It.shape + (3,), np.float32)
+        self._cdm_fixed(It, Is, _dims(It), levels, level_iterations, smoothing, bandwidth, field)
+        return field
 
     def gaussian(self, img, bandwidth=1):
         out = np.array(img, np.float32, order="C")

@@ -1005,3 +1005,220 @@ ORC_API void orc_gaussian_f32(float* img, const int* dim, int ch, unsigned bandw
     free(a);
     free(b);
 }
+
+/* A fixed-schedule CDM (an extension; the reference has no such mode):
+ * `levels` pyramid levels, each the downsample_with_padding of the one
+ * above, all run, coarsest first, for exactly `level_iterations` iterations
+ * of the reference loop body (reg/cdm.hpp:418-472: compose, gradient,
+ * 12-sweep Poisson, theta on a level's first iteration, scale,
+ * accumulate), keeping the running field (no cost-driven control).  Each
+ * iteration smooths the field with filter::gaussian(bandwidth) of new_d, or
+ * with cdm_smooth(new_d, smoothing) when bandwidth is 0.  A level's result
+ * is doubled and upsample_with_padding'ed into the next.  `field` (vox(dim)
+ * float3) receives the finest level's field. */
+static void orc_cdm_fixed_rec(const uint8_t* const* It, const uint8_t* const* Is, int nmod,
+                              const int* dim, float* field, int levels, int level_iterations,
+                              float speed, float smoothing, unsigned bandwidth, char gradient_type,
+                              int* iterations)
+{
+    size_t n = vox(dim);
+    memset(field, 0, sizeof(float) * 3 * n);
+    if (levels > 1) {
+        int rdim[3] = {(dim[0] + 1) / 2, (dim[1] + 1) / 2, (dim[2] + 1) / 2};
+        size_t rn = vox(rdim);
+        uint8_t* buf = (uint8_t*)malloc(2 * (size_t)nmod * rn);
+        float* rd = (float*)calloc(3 * rn, sizeof(float));
+        const uint8_t* rIt[16];
+        const uint8_t* rIs[16];
+        for (int i = 0; i < nmod && i < 16; ++i) {
+            int od[3];
+            orc_downsample_u8(It[i], dim, buf + (size_t)(2 * i) * rn, od);
+            orc_downsample_u8(Is[i], dim, buf + (size_t)(2 * i + 1) * rn, od);
+            rIt[i] = buf + (size_t)(2 * i) * rn;
+            rIs[i] = buf + (size_t)(2 * i + 1) * rn;
+        }
+        orc_cdm_fixed_rec(rIt, rIs, nmod, rdim, rd, levels - 1, level_iterations, speed, smoothing,
+                          bandwidth, gradient_type, iterations);
+        free(buf);
+        for (size_t k = 0; k < 3 * rn; ++k) rd[k] *= 2.0f;
+        orc_upsample_f3(rd, rdim, field, dim);
+        free(rd);
+    }
+    float* nd = (float*)malloc(sizeof(float) * 3 * n);
+    float* dd = (float*)malloc(sizeof(float) * 3 * n);
+    uint8_t* Js = (uint8_t*)malloc(n);
+    float theta = 0.0f;
+    for (int index = 0; index < level_iterations; ++index) {
+        ++*iterations;
+        for (int i = 0; i < nmod; ++i) {
+            orc_compose_displacement(Is[i], dim, field, Js);
+            (void)orc_cdm_gradient(Js, It[i], dim, gradient_type, i == 0 ? nd : dd);
+            if (i)
+                for (size_t k = 0; k < 3 * n; ++k) nd[k] += dd[k];
+        }
+        orc_cdm_poisson(nd, dim, 12);
+        if (theta == 0.0f)
+            theta = orc_cdm_max_length(nd, n);
+        if (theta == 0.0f)
+            break;
+        float k = speed / theta;
+        for (size_t j = 0; j < 3 * n; ++j) nd[j] *= k;
+        orc_accumulate_displacement(field, nd, dim);
+        if (bandwidth) {
+            orc_gaussian_f32(nd, dim, 3, bandwidth);
+            memcpy(field, nd, sizeof(float) * 3 * n);
+        } else {
+            orc_cdm_smooth(nd, field, dim, smoothing);
+        }
+    }
+    free(nd); free(dd); free(Js);
+}
+
+ORC_API void orc_cdm_fixed(const uint8_t* const* It, const uint8_t* const* Is, int nmod,
+                           const int* dim, int levels, int level_iterations, float speed,
+                           float smoothing, unsigned bandwidth, char gradient_type, float* field,
+                           int* iterations)
+{
+    int it = 0;
+    orc_cdm_fixed_rec(It, Is, nmod, dim, field, levels, level_iterations, speed, smoothing,
+                      bandwidth, gradient_type, &it);
+    if (iterations) *iterations = it;
+}
+
+/* --- subject preprocessing ------------------------------------------------
+ * reg::subject_image_pre (reg/mm_reg.hpp:14-20): filter::gaussian(I) with
+ * bandwidth 1, segmentation::normalize_otsu_median(I, 255.99f)
+ * (numerical/otsu.hpp:96-136), then image<3,uint8>(I) (std::copy, so a
+ * truncating conversion). */
+
+static int orc_cmp_float(const void* a, const void* b)
+{
+    float x = *(const float*)a, y = *(const float*)b;
+    return (x > y) - (x < y);
+}
+
+/* The value of rank k (0-based) of v[0..n): std::nth_element's element. */
+static float orc_rank(const float* v, size_t n, size_t k)
+{
+    float* t = (float*)malloc(sizeof(float) * n);
+    memcpy(t, v, sizeof(float) * n);
+    qsort(t, n, sizeof(float), orc_cmp_float);
+    float r = t[k];
+    free(t);
+    return r;
+}
+
+/* tipl::median (numerical/statistics.hpp:327-363): below 2048 values the
+ * element of rank n/2; otherwise the rank-128 element of the 256 medians of
+ * contiguous chunks (chunk i: i*(n/256) + min(i, n%256), one longer for
+ * i < n%256), each the element of rank len/2 of its chunk. */
+ORC_API float orc_median(const float* v, size_t n)
+{
+    if (n < 2048)
+        return orc_rank(v, n, n / 2);
+    float chunk[256];
+    size_t cs = n / 256, rem = n % 256;
+    for (size_t i = 0; i < 256; ++i) {
+        size_t start = i * cs + (i < rem ? i : rem);
+        size_t len = cs + (i < rem ? 1 : 0);
+        chunk[i] = orc_rank(v + start, len, len / 2);
+    }
+    return orc_rank(chunk, 256, 128);
+}
+
+/* otsu_threshold (numerical/otsu.hpp:15-81) of a float image: 256-bin
+ * histogram (numerical/basic_op.hpp:1294-1311) between min and max, bin 0
+ * cleared, then the between-class variance scan in float. */
+ORC_API float orc_otsu_threshold(const float* v, size_t n)
+{
+    if (n == 0) return 0.0f;
+    float mn = v[0], mx = v[0];
+    for (size_t i = 1; i < n; ++i) {
+        if (v[i] > mx) mx = v[i];
+        else if (v[i] < mn) mn = v[i];
+    }
+    if (mn >= mx) return mn;  /* histogram() leaves the histogram empty */
+    unsigned hist[256] = {0};
+    float range = mx - mn;
+    if (range == 0.0f) range = 1.0f;
+    range = 256.0f / range;
+    for (size_t i = 0; i < n; ++i) {
+        float value = (v[i] - mn) * range;
+        int b = (int)floorf(value);
+        b = b < 0 ? 0 : (b > 255 ? 255 : b);
+        ++hist[b];
+    }
+    hist[0] = 0;
+    float total_w = 0.0f, total_sum = 0.0f;
+    for (int i = 0; i < 256; ++i) {
+        float val = (float)hist[i];
+        total_w += val;
+        total_sum += val * (float)i;
+    }
+    if (total_w == 0.0f) return mn;
+    float max_sig_b = 0.0f, w_b = 0.0f, sum_b = 0.0f;
+    int opt = 0;
+    for (int i = 0; i < 255; ++i) {
+        w_b += (float)hist[i];
+        if (w_b == 0.0f) continue;
+        float w_f = total_w - w_b;
+        if (w_f == 0.0f) break;
+        sum_b += (float)hist[i] * (float)i;
+        float d = (sum_b * total_w) - (total_sum * w_b);
+        float sig_b = (d * d) / (w_b * w_f);
+        if (sig_b > max_sig_b) {
+            max_sig_b = sig_b;
+            opt = i;
+        }
+    }
+    return mn + (float)opt * (mx - mn) / 256.0f;
+}
+
+/* otsu_median (numerical/otsu.hpp:96-108). */
+ORC_API float orc_otsu_median(const float* v, size_t n)
+{
+    float ot = orc_otsu_threshold(v, n);
+    float* buf = (float*)malloc(sizeof(float) * (n ? n : 1));
+    size_t m = 0;
+    for (size_t i = 0; i < n; ++i)
+        if (v[i] > ot) buf[m++] = v[i];
+    float r = m ? orc_median(buf, m) : ot;
+    free(buf);
+    return r;
+}
+
+/* The float -> uint8 conversion of std::copy on the reference's x86 build:
+ * truncation to int32, low byte kept (values are in [0, 255.99] after the
+ * normalisation; other values only arise when the median is 0). */
+static uint8_t orc_to_u8(float v)
+{
+    return (uint8_t)(uint32_t)(int32_t)v;
+}
+
+ORC_API void orc_subject_pre(const float* in, const int* dim, uint8_t* out)
+{
+    size_t n = vox(dim);
+    float* I = (float*)malloc(sizeof(float) * (n ? n : 1));
+    memcpy(I, in, sizeof(float) * n);
+    orc_gaussian_f32(I, dim, 1, 1);
+    const float upper = 255.99f;
+    float ot = orc_otsu_median(I, n);
+    if (ot != 0.0f) {
+        float c = upper * 0.5f / ot;
+        size_t pos = 0;
+        for (size_t i = 0; i < n; ++i) {
+            I[i] *= c;
+            if (I[i] > 0.0f) ++pos;
+        }
+        if ((double)pos > 0.5 * (double)n) {
+            float s = upper * 0.1f;
+            for (size_t i = 0; i < n; ++i) I[i] -= s;
+        }
+        for (size_t i = 0; i < n; ++i) {
+            float v = I[i];
+            I[i] = v > upper ? upper : (v < 0.0f ? 0.0f : v);
+        }
+    }
+    for (size_t i = 0; i < n; ++i) out[i] = orc_to_u8(I[i]);
+    free(I);
+}

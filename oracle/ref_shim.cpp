@@ -14,7 +14,9 @@
 #include <vector>
 
 #include "filter/gaussian.hpp"
+#include "numerical/otsu.hpp"
 #include "reg/cdm.hpp"
+#include "reg/mm_reg.hpp"
 #include "reg/linear.hpp"
 
 namespace {
@@ -267,6 +269,79 @@ double tref_cdm_iterations(const uint8_t* It, const uint8_t* Is, const int* dim,
     auto t1 = std::chrono::steady_clock::now();
     if (field) from_field(cur, field);
     return std::chrono::duration<double>(t1 - t0).count();
+}
+
+// The fixed-schedule CDM extension composed from the reference's own
+// routines: its pyramid (downsample_with_padding, best_d *= 2,
+// upsample_with_padding, reg/cdm.hpp:385-405), its loop body as above for
+// exactly `iters` iterations per level, and filter::gaussian(new_d, bw) in
+// place of cdm_smooth when bw > 0.  Checks the oracle's restatement.
+static void ref_cdm_fixed(const tipl::const_pointer_image<3, uint8_t>& T,
+                          const tipl::const_pointer_image<3, uint8_t>& S, field_t& cur, int levels,
+                          int iters, float smoothing, unsigned bw)
+{
+    cur.resize(T.shape());
+    std::fill(cur.begin(), cur.end(), tipl::vector<3>(0.0f, 0.0f, 0.0f));
+    if (levels > 1) {
+        tipl::image<3, uint8_t> rT, rS;
+        tipl::downsample_with_padding(T, rT);
+        tipl::downsample_with_padding(S, rS);
+        field_t rd;
+        ref_cdm_fixed(tipl::const_pointer_image<3, uint8_t>(rT.data(), rT.shape()),
+                      tipl::const_pointer_image<3, uint8_t>(rS.data(), rS.shape()), rd, levels - 1,
+                      iters, smoothing, bw);
+        rd *= 2.0f;
+        tipl::upsample_with_padding(rd, T.shape());
+        cur = rd;
+    }
+    float theta = 0.0f;
+    bool terminated = false;
+    for (int i = 0; i < iters; ++i) {
+        field_t new_d;
+        (void)tipl::reg::cdm_get_gradient(tipl::compose_displacement(S, cur), T, new_d, 'r');
+        tipl::reg::cdm_solve_poisson(new_d, terminated);
+        if (theta == 0.0f) theta = tipl::reg::cdm_max_displacement_length(new_d);
+        if (theta == 0.0f) break;
+        new_d *= 0.3f / theta;
+        tipl::accumulate_displacement(cur, new_d);
+        if (bw) {
+            tipl::filter::gaussian(new_d, bw);
+            cur = new_d;
+        } else {
+            tipl::reg::cdm_smooth(new_d, cur, smoothing);
+        }
+    }
+}
+
+void tref_cdm_fixed(const uint8_t* It, const uint8_t* Is, const int* dim, int levels, int iters,
+                    float smoothing, unsigned bw, float* field)
+{
+    tipl::const_pointer_image<3, uint8_t> T(It, shp(dim)), S(Is, shp(dim));
+    field_t cur;
+    ref_cdm_fixed(T, S, cur, levels, iters, smoothing, bw);
+    from_field(cur, field);
+}
+
+// reg::subject_image_pre (gaussian, normalize_otsu_median, uint8 conversion).
+void tref_subject_pre(const float* in, const int* dim, uint8_t* out)
+{
+    tipl::image<3> I(shp(dim));
+    std::memcpy(&I[0], in, sizeof(float) * I.size());
+    auto J = tipl::reg::subject_image_pre(std::move(I));
+    std::memcpy(out, &J[0], J.size());
+}
+
+float tref_otsu_median(const float* in, size_t n)
+{
+    tipl::image<3> I(tipl::shape<3>(unsigned(n), 1u, 1u));
+    std::memcpy(&I[0], in, sizeof(float) * n);
+    return tipl::segmentation::otsu_median(I);
+}
+
+float tref_median(const float* in, size_t n)
+{
+    std::vector<float> v(in, in + n);
+    return tipl::median(v);
 }
 
 // tipl::reg::linear on the CPU; arg (12 floats) is in/out.

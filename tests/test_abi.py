@@ -1,6 +1,7 @@
 """CPU checks of the drop-in boundary: the C-ABI library loads, exports every
 symbol include/tiplhip.h declares, and its host geometry matches the
 oracle bit for bit.  No GPU work is launched here."""
+import ctypes
 import os
 import re
 
@@ -61,3 +62,15 @@ def test_errors_are_reported_without_gpu():
     rc = lib.tiplhip_ctx_create(-1, ctypes.byref(out))
     assert rc != tipl_amd.OK
     assert lib.tiplhip_last_error()
+
+
+def test_cdm_param_defaults():
+    """cdm_param (reg/cdm.hpp:36-44) defaults, the 200-iteration cap, and the
+    fixed-schedule extension off."""
+    import tipl_amd
+    p = tipl_amd.CdmParam()
+    tipl_amd.load_library().tiplhip_cdm_param_default(ctypes.byref(p))
+    assert (p.resolution, p.speed, p.smoothing, p.min_dimension, p.max_iterations) == \
+        (2.0, np.float32(0.3), np.float32(0.05), 16, 200)
+    assert p.gradient_type == b"rrrrrrrr"
+    assert (p.levels, p.level_iterations, p.field_bandwidth) == (0, 0, 0)

@@ -109,3 +109,20 @@ def test_pairs_rejects_bad_arguments(ctx):
         ctx.cost_batch_pairs([], "mi")
     with pytest.raises(tipl_amd.TiplHipError):
         ctx.cost_batch_pairs([(vf, vf, T[:3])], "mi", 128)
+
+
+@pytest.mark.parametrize("cost,bins", [("mi", 64), ("mi", 256), ("corr", 256)])
+def test_autotune_keeps_results(ctx, cost, bins):
+    """The tuned launch shape is used for later launches of the same key and
+    changes no result."""
+    rng = np.random.default_rng(31)
+    F = smooth_volume((40, 44, 36), 41)
+    M = smooth_volume((44, 40, 38), 42)
+    vf, vm = ctx.volume(F), ctx.volume(M)
+    Ts = np.stack([_T(p, F.shape, M.shape) for p in random_params(rng, 12)])
+    before = ctx.cost_batch(vf, vm, Ts, cost, bins)
+    zc, pc, ms = ctx.autotune(vf, vm, Ts, cost, bins, reps=2)
+    print(f"autotune {cost}{bins}: planes {zc}, workgroups/CU {pc}, {ms:.4f} ms")
+    assert zc in (8, 16, 32) and pc in (1, 2, 4, 8, 16) and ms > 0
+    after = ctx.cost_batch(vf, vm, Ts, cost, bins)
+    assert np.array_equal(before, after)

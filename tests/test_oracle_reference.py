@@ -144,3 +144,17 @@ def test_gaussian(oracle, ref, shape, bandwidth):
     assert np.array_equal(_bits(oracle.gaussian(img, bandwidth)), _bits(ref.gaussian(img, bandwidth)))
     assert np.array_equal(_bits(oracle.gaussian(field, bandwidth)),
                           _bits(ref.gaussian(field, bandwidth)))
+
+
+@pytest.mark.parametrize("levels,iters,smoothing,bandwidth",
+                         [(1, 5, 0.8, 0), (2, 4, 0.8, 0), (3, 3, 0.5, 2), (2, 3, 0.8, 8)])
+def test_cdm_fixed_schedule(oracle, ref, levels, iters, smoothing, bandwidth):
+    """The fixed-schedule CDM extension against the same schedule composed
+    from the reference's own pyramid, loop-body and gaussian routines."""
+    from scipy.ndimage import shift
+    It = smooth_volume((26, 30, 22), 21, sigma=2.5)
+    Is = shift(It.astype(np.float32), (1.0, -1.5, 2.0), order=1).astype(np.uint8)
+    fo, n = oracle.cdm_fixed(It, Is, levels, iters, 0.3, smoothing, bandwidth)
+    fr = ref.cdm_fixed(It, Is, levels, iters, smoothing, bandwidth)
+    assert np.array_equal(_bits(fo), _bits(fr))
+    assert n == levels * iters

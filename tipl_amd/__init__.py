@@ -36,7 +36,8 @@ EXPORTS = (
     "tiplhip_gaussian", "tiplhip_cost_batch_pairs", "tiplhip_volume_resample",
     "tiplhip_reg_request_default", "tiplhip_register", "tiplhip_server_create",
     "tiplhip_request_bytes", "tiplhip_server_submit", "tiplhip_server_wait",
-    "tiplhip_server_stats_get", "tiplhip_server_destroy",
+    "tiplhip_server_stats_get", "tiplhip_server_destroy", "tiplhip_volume_create_subject",
+    "tiplhip_autotune",
 )
 
 
@@ -55,7 +56,8 @@ class CdmParam(C.Structure):
     """tiplhip_cdm_param (cdm_param, reg/cdm.hpp:36-44, plus the iteration cap)."""
     _fields_ = [("resolution", C.c_float), ("speed", C.c_float), ("smoothing", C.c_float),
                 ("min_dimension", C.c_int), ("max_iterations", C.c_int),
-                ("gradient_type", C.c_char * 8)]
+                ("gradient_type", C.c_char * 8), ("levels", C.c_int), ("level_iterations", C.c_int),
+                ("field_bandwidth", C.c_int)]
 
 
 class RegRequest(C.Structure):
@@ -63,13 +65,15 @@ class RegRequest(C.Structure):
     _fields_ = [("fixed", C.c_void_p), ("fixed_dtype", C.c_int), ("fixed_dim", C.c_int * 3),
                 ("fixed_vs", C.c_float * 3), ("moving", C.c_void_p), ("moving_dtype", C.c_int),
                 ("moving_dim", C.c_int * 3), ("moving_vs", C.c_float * 3), ("cost", C.c_int),
-                ("bins", C.c_int), ("nonlinear", C.c_int), ("cdm", CdmParam), ("field", C.c_void_p)]
+                ("bins", C.c_int), ("nonlinear", C.c_int), ("cdm", CdmParam), ("field", C.c_void_p),
+                ("preprocess", C.c_int)]
 
 
 class RegResult(C.Structure):
     _fields_ = [("status", C.c_int), ("affine", C.c_float * 12), ("T", C.c_float * 12),
                 ("cost", C.c_float), ("cdm_iterations", C.c_int), ("evaluations", C.c_int64),
-                ("queue_ms", C.c_double), ("run_ms", C.c_double)]
+                ("queue_ms", C.c_double), ("run_ms", C.c_double), ("prep_ms", C.c_double),
+                ("affine_ms", C.c_double), ("cdm_ms", C.c_double)]
 
 
 class ServerStats(C.Structure):
@@ -139,6 +143,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "tiplhip_server_wait": (ip, [vp, C.c_uint64, P(RegResult)]),
         "tiplhip_server_stats_get": (ip, [vp, P(ServerStats)]),
         "tiplhip_server_destroy": (ip, [vp]),
+        "tiplhip_volume_create_subject": (ip, [vp, vp, ip, P(ip), P(vp)]),
+        "tiplhip_autotune": (ip, [vp, vp, vp, vp, ip, ip, ip, ip, P(ip), P(ip), P(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -288,6 +294,15 @@ class Context:
                                                     C.c_float(upper_limit), C.byref(h)))
         return Volume(self, h.value, data.shape)
 
+    def volume_subject(self, data: np.ndarray) -> Volume:
+        """reg::subject_image_pre on the device (gaussian, Otsu-median normalisation, uint8)."""
+        dt = {np.dtype(np.uint8): U8, np.dtype(np.int16): I16}.get(data.dtype, F32)
+        data = np.ascontiguousarray(data if dt != F32 else data.astype(np.float32, copy=False))
+        h = C.c_void_p()
+        _check(load_library().tiplhip_volume_create_subject(self.handle, _ptr(data), dt,
+                                                            _dims(data.shape), C.byref(h)))
+        return Volume(self, h.value, data.shape)
+
     # -- affine --------------------------------------------------------------
     def resample(self, src: Volume, out_shape, T, crop: bool = True) -> np.ndarray:
         out = np.empty(tuple(out_shape), np.uint8)
@@ -317,6 +332,16 @@ class Context:
                                                  Ts.shape[0], COST_MI if cost == "mi" else COST_CORR,
                                                  bins, _ptr(out)))
         return out
+
+    def autotune(self, frm: Volume, to: Volume, Ts, cost: str = "mi", bins: int = 256, reps: int = 3):
+        """Picks the cost kernel's launch shape for this pair, K and cost (kept on
+        the context); returns (planes per item, workgroups per CU, best kernel ms)."""
+        Ts = np.ascontiguousarray(Ts, np.float32).reshape(-1, 12)
+        zc, pc, ms = C.c_int(0), C.c_int(0), C.c_double(0)
+        _check(load_library().tiplhip_autotune(self.handle, frm.handle, to.handle, _ptr(Ts),
+                                               Ts.shape[0], COST_MI if cost == "mi" else COST_CORR,
+                                               bins, reps, C.byref(zc), C.byref(pc), C.byref(ms)))
+        return zc.value, pc.value, ms.value
 
     def cost_batch_pairs(self, pairs, cost: str = "mi", bins: int = 256) -> list:
         """One launch over several image pairs: pairs = [(from, to, Ts (K x 12)), ...];
@@ -441,8 +466,10 @@ _DTYPES = {np.dtype(np.uint8): U8, np.dtype(np.int16): I16, np.dtype(np.float32)
 
 
 def make_request(fixed, moving, fixed_vs=(1, 1, 1), moving_vs=(1, 1, 1), cost="mi", bins=256,
-                 nonlinear=False, want_field=True, **cdm):
+                 nonlinear=False, want_field=True, preprocess="linear", **cdm):
     """A RegRequest over numpy (z, y, x) volumes of uint8 / int16 / float32.
+    preprocess: 'linear' quantises both with normalize_upper_lower2 (255.99);
+    'mm_reg' uses template_image_pre for fixed and subject_image_pre for moving.
     Returns (request, keep) where keep holds the arrays the request points to
     (and the field output, when nonlinear and want_field)."""
     fixed = np.ascontiguousarray(fixed)
@@ -460,6 +487,9 @@ def make_request(fixed, moving, fixed_vs=(1, 1, 1), moving_vs=(1, 1, 1), cost="m
     r.cost = COST_MI if cost == "mi" else COST_CORR
     r.bins = int(bins)
     r.nonlinear = int(bool(nonlinear))
+    if preprocess not in ("linear", "mm_reg"):
+        raise ValueError("preprocess is 'linear' or 'mm_reg'")
+    r.preprocess = 1 if preprocess == "mm_reg" else 0
     gt = cdm.pop("gradient_type", None)
     for k, v in cdm.items():
         setattr(r.cdm, k, v)
@@ -476,7 +506,9 @@ def _result(res: RegResult, field):
     return {"status": res.status, "affine": np.array(res.affine[:], np.float32),
             "T": np.array(res.T[:], np.float32), "cost": float(res.cost),
             "cdm_iterations": int(res.cdm_iterations), "evaluations": int(res.evaluations),
-            "queue_ms": float(res.queue_ms), "run_ms": float(res.run_ms), "field": field}
+            "queue_ms": float(res.queue_ms), "run_ms": float(res.run_ms),
+            "prep_ms": float(res.prep_ms), "affine_ms": float(res.affine_ms),
+            "cdm_ms": float(res.cdm_ms), "field": field}
 
 
 def register(ctx: "Context", fixed, moving, **kw) -> dict:

@@ -56,8 +56,9 @@ VolGeo make_geo(const int* dim)
 
 constexpr uint32_t kTileX = 16, kTileY = 4;  // one wave = 16 x 4 voxels of one plane
 
-// Item geometry of one pair for K candidates (blocks chosen by batch_blocks).
-BatchGeo batch_geometry(const tiplhip_volume* from, const tiplhip_volume* to, int K)
+// Item geometry of one pair for K candidates, items of up to zc planes
+// (blocks chosen by batch_blocks).
+BatchGeo batch_geometry(const tiplhip_volume* from, const tiplhip_volume* to, int K, uint32_t zc)
 {
     BatchGeo b;
     b.fw = from->dim[0];
@@ -65,7 +66,7 @@ BatchGeo batch_geometry(const tiplhip_volume* from, const tiplhip_volume* to, in
     b.fd = from->dim[2];
     b.ntx = (uint32_t(b.fw) + kTileX - 1) / kTileX;
     b.ncols = b.ntx * ((uint32_t(b.fh) + kTileY - 1) / kTileY);
-    b.zc = std::min<uint32_t>(16u, uint32_t(b.fd));
+    b.zc = std::max<uint32_t>(1u, std::min<uint32_t>(zc, uint32_t(b.fd)));
     b.nzc = (uint32_t(b.fd) + b.zc - 1) / b.zc;
     b.items = b.ncols * b.nzc;
     b.K = uint32_t(K);
@@ -88,9 +89,6 @@ void batch_blocks(BatchGeo& b, uint32_t max_voxels_per_block, uint32_t target_bl
     uint64_t want = (uint64_t(b.items) * uint64_t(b.K) + target_blocks - 1) / target_blocks;
     uint32_t ipb = uint32_t(std::min<uint64_t>(want, max_items));
     ipb = std::max<uint32_t>(ipb, std::min<uint32_t>(4u, max_items));
-    // at most one plane of tile columns per workgroup, so that its items
-    // span at most two z runs (the kernels tabulate z products per run)
-    ipb = std::min<uint32_t>(ipb, std::max<uint32_t>(1u, b.ncols));
     b.items_per_block = ipb;
     b.total_blocks = ((b.items + ipb - 1) / ipb) * b.K;
 }
@@ -182,23 +180,6 @@ __device__ inline Pos apply(const float* T, float fx, float fy, float fz)
     return p;
 }
 
-// z*r2, z*r5, z*r8 (the z terms of vector_transformation) for the planes a
-// workgroup visits: its items [it0, it1) span at most two z runs, so at most
-// 2 x 16 planes starting at plane zbase.  Returns zbase; needs a barrier
-// before use.
-constexpr uint32_t kZTable = 32;
-__device__ inline uint32_t build_z_table(const BatchGeo& g, const float* T, uint32_t it0,
-                                         uint32_t it1, float4* zt)
-{
-    const uint32_t zbase = (it0 / g.ncols) * g.zc;
-    const uint32_t ztop = min(((it1 - 1) / g.ncols + 1) * g.zc, uint32_t(g.fd));
-    for (uint32_t i = threadIdx.x; i < ztop - zbase; i += blockDim.x) {
-        const float fz = float(zbase + i);
-        zt[i] = make_float4(fz * T[2], fz * T[5], fz * T[8], 0.0f);
-    }
-    return zbase;
-}
-
 // ---------------------------------------------------------------------------
 // Correlation: per candidate the exact sums Sy, Syy, Sxy (Sx, Sxx of `from`
 // do not depend on the transform and are kept with the volume).
@@ -230,9 +211,6 @@ __global__ __launch_bounds__(256) void corr_sums_kernel(const Segment* __restric
     const uint32_t it0 = chunk * g.items_per_block;
     const uint32_t it1 = min(it0 + g.items_per_block, g.items);
     const uint32_t fwh = uint32_t(g.fw) * uint32_t(g.fh);
-    __shared__ float4 zt[kZTable];
-    const uint32_t zbase = build_z_table(g, T, it0, it1, zt);
-    __syncthreads();
     uint32_t sy = 0, syy = 0, sxy = 0;
     for (uint32_t it = it0 + wave; it < it1; it += nwaves) {
         const Item m = decode(g, it, lane);
@@ -250,10 +228,10 @@ __global__ __launch_bounds__(256) void corr_sums_kernel(const Segment* __restric
                                           uint32_t(sx), 0)
                         : 0u;
             } else {
-                const float4 zr = zt[z - zbase];
-                const float px = (cs.hx + zr.x) + T[9];
-                const float py = (cs.hy + zr.y) + T[10];
-                const float pz = (cs.hz + zr.z) + T[11];
+                const float fz = float(z);
+                const float px = (cs.hx + fz * T[2]) + T[9];
+                const float py = (cs.hy + fz * T[5]) + T[10];
+                const float pz = (cs.hz + fz * T[8]) + T[11];
                 yv = sample_u8_packed(r8, g.to, px, py, pz);
             }
             sy += yv;
@@ -318,15 +296,19 @@ __global__ void corr_finalize_kernel(const Segment* __restrict__ segs, uint32_t 
 //               so a workgroup may add at most 65535 voxels (items_per_block
 //               <= 1023 segments of 64);
 //   BINS = 64 : 4096 uint32 words (16 KB).
-// Adds are wave-aggregated: the lanes whose bin equals the first active
-// lane's bin (the background bin, mostly) are counted with one ballot and
-// one LDS atomic; the other lanes add individually.  At the end the
-// non-zero words are flushed into the candidate's global int32 histogram.
+// With 64 bins adds are wave-aggregated: the lanes whose bin equals the
+// first active lane's bin (the background bin, mostly) are counted with one
+// ballot and one LDS atomic; the other lanes add individually.  With 256
+// bins values spread over many more bins and plain LDS atomics are faster
+// (measured: 1.08 ms against 1.22 ms for 64 candidates at 182x218x182).  At
+// the end the non-zero words are flushed into the candidate's global int32
+// histogram.
 // ---------------------------------------------------------------------------
 template <int BINS>
 struct MiTraits;
 template <>
 struct MiTraits<256> {
+    static constexpr bool kAggregate = false;  // values spread over many bins
     static constexpr int kWords = 32768;
     static constexpr int kBand = 8;
     static constexpr int kShift = 0;
@@ -336,6 +318,7 @@ struct MiTraits<256> {
 };
 template <>
 struct MiTraits<64> {
+    static constexpr bool kAggregate = true;  // 64 x 64 bins: runs of equal bins are common
     static constexpr int kWords = 4096;
     static constexpr int kBand = 6;
     static constexpr int kShift = 2;
@@ -351,8 +334,8 @@ __global__ __launch_bounds__(MiTraits<BINS>::kThreads) void mi_hist_kernel(
 {
     using Tr = MiTraits<BINS>;
     __shared__ uint32_t lds[Tr::kWords];
-    __shared__ float4 zt[kZTable];
     for (uint32_t i = threadIdx.x; i < uint32_t(Tr::kWords); i += blockDim.x) lds[i] = 0u;
+    __syncthreads();
 
     const uint32_t L = xcd_remap(blockIdx.x, total_blocks);
     const Segment& sg = segs[find_segment(segs, nseg, L, ByBlock{})];
@@ -371,8 +354,6 @@ __global__ __launch_bounds__(MiTraits<BINS>::kThreads) void mi_hist_kernel(
     const uint32_t it0 = chunk * g.items_per_block;
     const uint32_t it1 = min(it0 + g.items_per_block, g.items);
     const uint32_t fwh = uint32_t(g.fw) * uint32_t(g.fh);
-    const uint32_t zbase = build_z_table(g, T, it0, it1, zt);
-    __syncthreads();
     for (uint32_t it = it0 + wave; it < it1; it += nwaves) {
         const Item m = decode(g, it, lane);
         if (!m.ok) continue;
@@ -380,19 +361,23 @@ __global__ __launch_bounds__(MiTraits<BINS>::kThreads) void mi_hist_kernel(
         uint32_t fidx = m.z0 * fwh + m.y * uint32_t(g.fw) + m.x;
         for (uint32_t z = m.z0; z < m.z1; ++z, fidx += fwh) {
             const uint32_t fv = load_u8(rf, fidx, 0);
-            const float4 zr = zt[z - zbase];
-            const float px = (cs.hx + zr.x) + T[9];
-            const float py = (cs.hy + zr.y) + T[10];
-            const float pz = (cs.hz + zr.z) + T[11];
+            const float fz = float(z);
+            const float px = (cs.hx + fz * T[2]) + T[9];
+            const float py = (cs.hy + fz * T[5]) + T[10];
+            const float pz = (cs.hz + fz * T[8]) + T[11];
             const uint32_t tv = sample_u8_packed(r8, g.to, px, py, pz);
             const uint32_t bin = ((fv >> Tr::kShift) << Tr::kBand) | (tv >> Tr::kShift);
-            const uint32_t b0 = __builtin_amdgcn_readfirstlane(bin);
-            const bool same = bin == b0;
-            const unsigned long long mk = __ballot(same);
-            if (same) {
-                const uint32_t leader = uint32_t(__ffsll((long long)mk) - 1);
-                if (lane == leader)
-                    atomicAdd(&lds[Tr::word(b0)], Tr::inc(b0) * uint32_t(__popcll(mk)));
+            if constexpr (Tr::kAggregate) {
+                const uint32_t b0 = __builtin_amdgcn_readfirstlane(bin);
+                const bool same = bin == b0;
+                const unsigned long long mk = __ballot(same);
+                if (same) {
+                    const uint32_t leader = uint32_t(__ffsll((long long)mk) - 1);
+                    if (lane == leader)
+                        atomicAdd(&lds[Tr::word(b0)], Tr::inc(b0) * uint32_t(__popcll(mk)));
+                } else {
+                    atomicAdd(&lds[Tr::word(bin)], Tr::inc(bin));
+                }
             } else {
                 atomicAdd(&lds[Tr::word(bin)], Tr::inc(bin));
             }
@@ -415,28 +400,40 @@ __global__ __launch_bounds__(MiTraits<BINS>::kThreads) void mi_hist_kernel(
 // get_mutual_info_mean (reg/linear.hpp:87-104): to_hist = column sums,
 // sum of mu*log(mu/from_h[row]/to_h[col]) in FP64 over non-zero bins,
 // divided by float(BINS*BINS) and negated (reg/linear.hpp:201).  One
-// workgroup per candidate; the FP64 sum is a fixed-shape tree, so it is
-// deterministic (the reference's sum is sequential in row-major order; the
-// two agree to rounding).
+// 1024-thread workgroup per candidate: column sums in 1024/BINS row slices
+// (exact integers), then 16 waves of FP64 terms combined by a fixed tree,
+// so the result is deterministic (the reference's sum is sequential in
+// row-major order; the two agree to rounding).
+constexpr int kEntropyThreads = 1024;
 template <int BINS>
-__global__ __launch_bounds__(256) void mi_entropy_kernel(const Segment* __restrict__ segs,
-                                                         uint32_t nseg,
-                                                         const int* __restrict__ hist,
-                                                         double* __restrict__ costs)
+__global__ __launch_bounds__(kEntropyThreads) void mi_entropy_kernel(
+    const Segment* __restrict__ segs, uint32_t nseg, const int* __restrict__ hist,
+    double* __restrict__ costs)
 {
+    constexpr int kSlices = kEntropyThreads / BINS;
+    constexpr int kRows = BINS / kSlices;
+    constexpr int kWaves = kEntropyThreads / 64;
     const int* h = hist + size_t(blockIdx.x) * BINS * BINS;
     const uint32_t* __restrict__ from_hist =
         segs[find_segment(segs, nseg, blockIdx.x, ByCandidate{})].from_hist;
+    __shared__ uint32_t colpart[kSlices][BINS];
     __shared__ uint32_t to_h[BINS];
-    __shared__ double part[4];
-    for (int c = threadIdx.x; c < BINS; c += blockDim.x) {
+    __shared__ double part[kWaves];
+    {
+        const int c = threadIdx.x % BINS, sl = threadIdx.x / BINS;
         uint32_t s = 0;
-        for (int r = 0; r < BINS; ++r) s += uint32_t(h[r * BINS + c]);
-        to_h[c] = s;
+        for (int r = sl * kRows; r < (sl + 1) * kRows; ++r) s += uint32_t(h[r * BINS + c]);
+        colpart[sl][c] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < BINS) {
+        uint32_t s = 0;
+        for (int sl = 0; sl < kSlices; ++sl) s += colpart[sl][threadIdx.x];
+        to_h[threadIdx.x] = s;
     }
     __syncthreads();
     double acc = 0.0;
-    for (int i = threadIdx.x; i < BINS * BINS; i += blockDim.x) {
+    for (int i = threadIdx.x; i < BINS * BINS; i += kEntropyThreads) {
         const int mu = h[i];
         if (mu == 0) continue;
         const double m = double(mu);
@@ -446,8 +443,11 @@ __global__ __launch_bounds__(256) void mi_entropy_kernel(const Segment* __restri
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
     __syncthreads();
     if (threadIdx.x == 0) {
-        double s = (part[0] + part[1]) + (part[2] + part[3]);
-        costs[blockIdx.x] = -(s / double(float(BINS * BINS)));
+        double q[kWaves];
+        for (int i = 0; i < kWaves; ++i) q[i] = part[i];
+        for (int w = kWaves / 2; w > 0; w /= 2)
+            for (int i = 0; i < w; ++i) q[i] = q[2 * i] + q[2 * i + 1];
+        costs[blockIdx.x] = -(q[0] / double(float(BINS * BINS)));
     }
 }
 
@@ -568,28 +568,53 @@ struct Staged {
 
 bool hist256(int cost, int bins) { return cost == TIPLHIP_COST_MI && bins == 256; }
 
+// Work items (16 x 4 voxel columns of up to zc planes) per workgroup above
+// which a launch gets more workgroups: about 256 items, 256K samples at zc = 16.
+constexpr double kMaxItemsPerBlock = 256.0;
+
+// The launch shape of one pair: planes per work item and workgroups per CU.
+LaunchShape default_shape(int cost, int bins) { return LaunchShape{16u, hist256(cost, bins) ? 4u : 8u}; }
+
+TuneKey tune_key(const tiplhip_volume* from, const tiplhip_volume* to, int K, int cost, int bins)
+{
+    return TuneKey{from->dim[0], from->dim[1], from->dim[2], to->dim[0], to->dim[1], to->dim[2], K,
+                   cost, cost == TIPLHIP_COST_MI ? bins : 0};
+}
+
 // Builds the segment table of `jobs` (T holds their transforms, job after
 // job; crop, when not null, one entry per candidate) and uploads all of it
 // in one copy.  Each pair gets a share of about cu_count x 8 workgroups (x 4
 // for the 1024-thread 256-bin histogram) in proportion to its voxel-candidate
 // work.
 Staged stage_jobs(tiplhip_ctx* ctx, const std::vector<PairJob>& jobs, const float* T,
-                  const int4* crop, int cost, int bins)
+                  const int4* crop, int cost, int bins, const LaunchShape* force = nullptr)
 {
     Staged st;
     std::vector<Segment> segs(jobs.size());
     const uint32_t max_vox = hist256(cost, bins) ? 65535u : (1u << 23);
-    const uint32_t target = uint32_t(ctx->cu_count) * (hist256(cost, bins) ? 4u : 8u);
+    // a single pair uses the autotuned shape for its key when there is one
+    LaunchShape shape = default_shape(cost, bins);
+    if (force) {
+        shape = *force;
+    } else if (jobs.size() == 1) {
+        auto it = ctx->tuned.find(tune_key(jobs[0].from, jobs[0].to, jobs[0].K, cost, bins));
+        if (it != ctx->tuned.end()) shape = it->second;
+    }
     double work = 0.0;
     for (size_t j = 0; j < jobs.size(); ++j) {
-        segs[j].g = batch_geometry(jobs[j].from, jobs[j].to, jobs[j].K);
+        segs[j].g = batch_geometry(jobs[j].from, jobs[j].to, jobs[j].K, shape.zc);
         work += double(segs[j].g.items) * double(jobs[j].K);
     }
+    // at least per_cu workgroups per CU, and no workgroup with more than
+    // kMaxItemsPerBlock items (large batches get more workgroups, not longer ones:
+    // concurrent workgroups then share z slabs of one pair in L2)
+    const double target = std::max(double(ctx->cu_count) * double(shape.per_cu),
+                                    std::ceil(work / double(kMaxItemsPerBlock)));
     uint64_t blocks = 0;
     for (size_t j = 0; j < jobs.size(); ++j) {
         Segment& sg = segs[j];
         const double share = double(sg.g.items) * double(jobs[j].K) / work;
-        batch_blocks(sg.g, max_vox, uint32_t(std::ceil(share * double(target))));
+        batch_blocks(sg.g, max_vox, uint32_t(std::min(std::ceil(share * target), 2147483647.0)));
         sg.from = jobs[j].from->d;
         sg.to = jobs[j].to->d;
         sg.to8 = jobs[j].to->d_pack;
@@ -672,11 +697,11 @@ void launch_mi_entropy(tiplhip_ctx* ctx, const Staged& st, int bins)
 {
     ctx->d_costs.reserve(sizeof(double) * size_t(st.K));
     if (bins == 256)
-        hipLaunchKernelGGL(mi_entropy_kernel<256>, dim3(st.K), dim3(256), 0, ctx->stream, st.segs,
-                           st.nseg, ctx->d_hist.as<int>(), ctx->d_costs.as<double>());
+        hipLaunchKernelGGL(mi_entropy_kernel<256>, dim3(st.K), dim3(kEntropyThreads), 0, ctx->stream,
+                           st.segs, st.nseg, ctx->d_hist.as<int>(), ctx->d_costs.as<double>());
     else
-        hipLaunchKernelGGL(mi_entropy_kernel<64>, dim3(st.K), dim3(256), 0, ctx->stream, st.segs,
-                           st.nseg, ctx->d_hist.as<int>(), ctx->d_costs.as<double>());
+        hipLaunchKernelGGL(mi_entropy_kernel<64>, dim3(st.K), dim3(kEntropyThreads), 0, ctx->stream,
+                           st.segs, st.nseg, ctx->d_hist.as<int>(), ctx->d_costs.as<double>());
     TIPLHIP_CHECK(hipGetLastError());
 }
 
@@ -847,6 +872,79 @@ int tiplhip_mi_hist(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_
         const Staged st = stage_jobs(ctx, {PairJob{from, to, K}}, T, nullptr, TIPLHIP_COST_MI, bins);
         launch_mi_hist(ctx, st, bins);
         read_back(ctx, ctx->d_hist.p, hist, sizeof(int) * size_t(bins) * bins * size_t(K));
+    });
+}
+
+int tiplhip_autotune(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_volume* to,
+                     const float* T, int K, int cost, int bins, int reps, int* zc, int* per_cu,
+                     double* best_ms)
+{
+    return guarded([&] {
+        TIPLHIP_REQUIRE(ctx != nullptr && T != nullptr, "null argument");
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        check_volume(ctx, from, "from");
+        check_volume(ctx, to, "to");
+        TIPLHIP_REQUIRE(K >= 1 && K <= kMaxK, "K must be in [1, 4096]");
+        TIPLHIP_REQUIRE(cost == TIPLHIP_COST_CORR || cost == TIPLHIP_COST_MI, "unknown cost type");
+        if (cost == TIPLHIP_COST_MI) TIPLHIP_REQUIRE(bins == 256 || bins == 64, "bins must be 256 or 64");
+        TIPLHIP_REQUIRE(reps >= 1 && reps <= 100, "reps must be in [1, 100]");
+        DeviceGuard dg(ctx->device);
+        // the correlation path evaluates with the roles swapped when `from`
+        // is larger (reg/linear.hpp:28-33); tune what will run
+        const tiplhip_volume *f = from, *t = to;
+        std::vector<float> Tk(T, T + 12 * size_t(K));
+        std::vector<int4> crop;
+        if (cost == TIPLHIP_COST_CORR) {
+            if (f->n > t->n) {
+                std::swap(f, t);
+                for (int k = 0; k < K; ++k) (void)matrix_inverse(&Tk[12 * size_t(k)]);
+            }
+            crop.resize(size_t(K));
+            for (int k = 0; k < K; ++k) {
+                int pos[3];
+                const bool c = is_integer_shift(&Tk[12 * size_t(k)], pos);
+                crop[size_t(k)] = make_int4(c ? 1 : 0, pos[0], pos[1], pos[2]);
+            }
+        }
+        volume_ensure_packed(ctx, const_cast<tiplhip_volume*>(t));
+        const bool wide = hist256(cost, bins);  // 1024-thread workgroups
+        const uint32_t zcs[] = {8u, 16u, 32u};
+        const uint32_t pcs_narrow[] = {2u, 4u, 8u, 16u}, pcs_wide[] = {1u, 2u, 4u, 8u};
+        const uint32_t* pcs = wide ? pcs_wide : pcs_narrow;
+        LaunchShape best = default_shape(cost, bins);
+        double best_t = 1e30;
+        const bool keep_timing = ctx->kernel_timing;
+        const double keep_ms = ctx->kernel_ms;
+        const int64_t keep_n = ctx->kernel_launches;
+        ctx->kernel_timing = true;
+        for (uint32_t z : zcs)
+            for (int p = 0; p < 4; ++p) {
+                const LaunchShape shape{z, pcs[p]};
+                const Staged st = stage_jobs(ctx, {PairJob{f, t, K}}, Tk.data(),
+                                             crop.empty() ? nullptr : crop.data(), cost, bins, &shape);
+                ctx->kernel_ms = 0.0;
+                ctx->kernel_launches = 0;
+                for (int r = 0; r <= reps; ++r) {  // the first launch warms up
+                    if (r == 1) {
+                        ctx->kernel_ms = 0.0;
+                        ctx->kernel_launches = 0;
+                    }
+                    if (cost == TIPLHIP_COST_CORR) launch_corr_sums(ctx, st);
+                    else launch_mi_hist(ctx, st, bins);
+                }
+                const double avg = ctx->kernel_ms / double(std::max<int64_t>(1, ctx->kernel_launches));
+                if (avg < best_t) {
+                    best_t = avg;
+                    best = shape;
+                }
+            }
+        ctx->kernel_timing = keep_timing;
+        ctx->kernel_ms = keep_ms;
+        ctx->kernel_launches = keep_n;
+        ctx->tuned[tune_key(f, t, K, cost, bins)] = best;  // the order stage_jobs sees
+        if (zc) *zc = int(best.zc);
+        if (per_cu) *per_cu = int(best.per_cu);
+        if (best_ms) *best_ms = best_t;
     });
 }
 

@@ -600,6 +600,128 @@ __global__ __launch_bounds__(kTX * kPTY) void poisson_zm_kernel(const float* __r
 }
 
 // ---------------------------------------------------------------------------
+// Two Jacobi sweeps per launch (s_t -> s_t+1 -> s_t+2), marching z.
+//
+// Tiles are laid out in flat-index space: the voxel at offset (i, j) of a
+// tile of plane m is flat index P0(m) + j*w + i, P0(m) the flat index of the
+// tile's first voxel.  The reference's neighbours p-1, p+1, p-w, p+w are then
+// offsets (i-1, j), (i+1, j), (i, j-1), (i, j+1) -- row and plane wrap
+// included -- and p-+wh is the same offset in planes m-+1.  A neighbour
+// outside [0, n) is dropped by the reference; here it holds +0.0, and
+// r - (+0.0) == r for every float r, so the results are bit-identical.
+//
+// Per plane m the block loads s_t on its tile grown by 2 (S ring, 3 planes),
+// computes s_t+1 on the tile grown by 1 (T ring, 3 planes), then s_t+2 of
+// plane m-1 on the tile itself.  HBM traffic per two sweeps: new_d and s_t
+// read once, s_t+2 written once (36 B per voxel instead of 72).
+// ---------------------------------------------------------------------------
+constexpr int kQX = 64, kQY = 8, kQZ = 16;
+constexpr int kS2X = kQX + 4, kS2Y = kQY + 4;  // tile grown by 2
+constexpr int kS1X = kQX + 2, kS1Y = kQY + 2;  // tile grown by 1
+
+dim3 poisson2_grid(const FieldGeo& g)
+{
+    return dim3(uint32_t((g.w + kQX - 1) / kQX), uint32_t((g.h + kQY - 1) / kQY),
+                uint32_t((g.d + kQZ - 1) / kQZ));
+}
+
+template <bool FIRST, bool MAXLEN>
+__global__ __launch_bounds__(kQX * kQY) void poisson2_kernel(const float* __restrict__ nd,
+                                                             const float* __restrict__ a,
+                                                             FieldGeo g, float* __restrict__ b,
+                                                             float* __restrict__ maxparts)
+{
+    __shared__ float3 S[3][kS2Y][kS2X];
+    __shared__ float3 T[3][kS1Y][kS1X];
+    const int tx = int(threadIdx.x), ty = int(threadIdx.y), tid = ty * kQX + tx;
+    const int x0 = int(blockIdx.x) * kQX, y0 = int(blockIdx.y) * kQY;
+    const int z0 = int(blockIdx.z) * kQZ, z1 = min(z0 + kQZ, g.d);
+    const long long n = g.n, w = g.w, wh = g.wh;
+    const long long base = (long long)y0 * w + x0;
+    // s_t at flat q, +0 outside the volume (FIRST: s_0 = new_d * (-1/6))
+    auto load_s = [&](long long q) -> float3 {
+        if (q < 0 || q >= n) return make_float3(0.0f, 0.0f, 0.0f);
+        const float3 t = ld3(FIRST ? nd : a, uint32_t(q));
+        if (FIRST) return make_float3(t.x * kInvD2, t.y * kInvD2, t.z * kInvD2);
+        return t;
+    };
+    auto load_plane = [&](int m) {
+        const long long p0 = (long long)m * wh + base;
+        float3(*dst)[kS2X] = S[(m + 3) % 3];
+        for (int e = tid; e < kS2Y * kS2X; e += kQX * kQY) {
+            const int jj = e / kS2X, ii = e - jj * kS2X;
+            dst[jj][ii] = load_s(p0 + (long long)(jj - 2) * w + (ii - 2));
+        }
+    };
+    // one Jacobi update in the reference's order from the six neighbours
+    auto update = [](float3 r, float3 xm, float3 xp, float3 ym, float3 yp, float3 zm, float3 zp) {
+        r.x = r.x - xm.x; r.y = r.y - xm.y; r.z = r.z - xm.z;
+        r.x = r.x - xp.x; r.y = r.y - xp.y; r.z = r.z - xp.z;
+        r.x = r.x - ym.x; r.y = r.y - ym.y; r.z = r.z - ym.z;
+        r.x = r.x - yp.x; r.y = r.y - yp.y; r.z = r.z - yp.z;
+        r.x = r.x - zm.x; r.y = r.y - zm.y; r.z = r.z - zm.z;
+        r.x = r.x - zp.x; r.y = r.y - zp.y; r.z = r.z - zp.z;
+        return make_float3(r.x * kInvD2, r.y * kInvD2, r.z * kInvD2);
+    };
+
+    load_plane(z0 - 2);
+    load_plane(z0 - 1);
+    float len_max = 0.0f;
+    for (int m = z0 - 1; m <= z1; ++m) {
+        load_plane(m + 1);
+        __syncthreads();
+        // s_t+1 of plane m on the tile grown by 1
+        {
+            const long long p0 = (long long)m * wh + base;
+            float3(*sm)[kS2X] = S[(m + 2) % 3];
+            float3(*sc)[kS2X] = S[(m + 3) % 3];
+            float3(*sp)[kS2X] = S[(m + 4) % 3];
+            float3(*dst)[kS1X] = T[(m + 3) % 3];
+            for (int e = tid; e < kS1Y * kS1X; e += kQX * kQY) {
+                const int jj = e / kS1X, ii = e - jj * kS1X;  // offset (ii-1, jj-1)
+                const long long q = p0 + (long long)(jj - 1) * w + (ii - 1);
+                float3 v = make_float3(0.0f, 0.0f, 0.0f);
+                if (q >= 0 && q < n) {
+                    const int cj = jj + 1, ci = ii + 1;  // the same voxel in S coordinates
+                    v = update(ld3(nd, uint32_t(q)), sc[cj][ci - 1], sc[cj][ci + 1], sc[cj - 1][ci],
+                               sc[cj + 1][ci], sm[cj][ci], sp[cj][ci]);
+                }
+                dst[jj][ii] = v;
+            }
+        }
+        __syncthreads();
+        // s_t+2 of plane m-1 on the tile
+        const int z = m - 1;
+        if (z >= z0 && z < z1 && x0 + tx < g.w && y0 + ty < g.h) {
+            const long long p = (long long)z * wh + base + (long long)ty * w + tx;
+            float3(*tm)[kS1X] = T[(z + 2) % 3];
+            float3(*tc)[kS1X] = T[(z + 3) % 3];
+            float3(*tp)[kS1X] = T[(z + 4) % 3];
+            const int cj = ty + 1, ci = tx + 1;
+            const float3 r = update(ld3(nd, uint32_t(p)), tc[cj][ci - 1], tc[cj][ci + 1],
+                                    tc[cj - 1][ci], tc[cj + 1][ci], tm[cj][ci], tp[cj][ci]);
+            st3(b, uint32_t(p), r);
+            if (MAXLEN) {
+                float s2 = r.x * r.x + r.y * r.y;
+                s2 = s2 + r.z * r.z;
+                len_max = fmaxf(len_max, float(sqrt(double(s2))));
+            }
+        }
+    }
+    if (MAXLEN) {
+        __shared__ float part[kQY];
+        const float mx = wave_max_f32(len_max);
+        if ((tid & 63) == 0) part[tid >> 6] = mx;
+        __syncthreads();
+        if (tid == 0) {
+            float r = part[0];
+            for (int i = 1; i < kQY; ++i) r = fmaxf(r, part[i]);
+            maxparts[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = r;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // new_d *= k, then accumulate_displacement(cur, new_d) in place:
 // new(x) = m(x + new(x)) - x with m(y) = cur(y) + y by trilinear
 // interpolation; cur(x) where new(x) == 0 or the sample is outside.
@@ -767,32 +889,60 @@ void launch_gradient(tiplhip_ctx* ctx, const uint8_t* Js, const uint8_t* It, con
 // With maxlen != nullptr the last sweep also forms per-block maxima of the
 // vector length in maxparts (block_count(g) floats) and reduces them into
 // *maxlen (float bits).
+// `sweeps` Jacobi sweeps, two per launch (poisson2_kernel) and a single
+// z-marching sweep for an odd remainder; returns the buffer holding the
+// result.  With maxlen, the last launch also forms per-block maxima of the
+// displacement length and max_reduce_kernel writes their maximum.
 float* launch_poisson(tiplhip_ctx* ctx, const float* nd, float* a, float* b, const FieldGeo& g,
                       int sweeps, unsigned int* maxlen, float* maxparts)
 {
     float* src = a;
     float* dst = b;
-    const dim3 grid = poisson_grid(g), block(kTX, kPTY);
-    for (int it = 0; it < sweeps; ++it) {
-        const bool first = it == 0, last = it + 1 == sweeps && maxlen;
-        if (first && last)
-            hipLaunchKernelGGL((poisson_zm_kernel<true, true>), grid, block, 0, ctx->stream, nd, src,
-                               g, dst, maxparts);
-        else if (first)
-            hipLaunchKernelGGL((poisson_zm_kernel<true, false>), grid, block, 0, ctx->stream, nd, src,
-                               g, dst, maxparts);
-        else if (last)
-            hipLaunchKernelGGL((poisson_zm_kernel<false, true>), grid, block, 0, ctx->stream, nd, src,
-                               g, dst, maxparts);
-        else
-            hipLaunchKernelGGL((poisson_zm_kernel<false, false>), grid, block, 0, ctx->stream, nd, src,
-                               g, dst, maxparts);
+    uint32_t last_blocks = 0;
+    int done = 0;
+    while (done < sweeps) {
+        const bool first = done == 0;
+        const bool pair = sweeps - done >= 2;
+        const bool last = done + (pair ? 2 : 1) == sweeps && maxlen;
+        if (pair) {
+            const dim3 grid = poisson2_grid(g), block(kQX, kQY);
+            if (first && last)
+                hipLaunchKernelGGL((poisson2_kernel<true, true>), grid, block, 0, ctx->stream, nd, src,
+                                   g, dst, maxparts);
+            else if (first)
+                hipLaunchKernelGGL((poisson2_kernel<true, false>), grid, block, 0, ctx->stream, nd,
+                                   src, g, dst, maxparts);
+            else if (last)
+                hipLaunchKernelGGL((poisson2_kernel<false, true>), grid, block, 0, ctx->stream, nd,
+                                   src, g, dst, maxparts);
+            else
+                hipLaunchKernelGGL((poisson2_kernel<false, false>), grid, block, 0, ctx->stream, nd,
+                                   src, g, dst, maxparts);
+            last_blocks = grid.x * grid.y * grid.z;
+            done += 2;
+        } else {
+            const dim3 grid = poisson_grid(g), block(kTX, kPTY);
+            if (first && last)
+                hipLaunchKernelGGL((poisson_zm_kernel<true, true>), grid, block, 0, ctx->stream, nd,
+                                   src, g, dst, maxparts);
+            else if (first)
+                hipLaunchKernelGGL((poisson_zm_kernel<true, false>), grid, block, 0, ctx->stream, nd,
+                                   src, g, dst, maxparts);
+            else if (last)
+                hipLaunchKernelGGL((poisson_zm_kernel<false, true>), grid, block, 0, ctx->stream, nd,
+                                   src, g, dst, maxparts);
+            else
+                hipLaunchKernelGGL((poisson_zm_kernel<false, false>), grid, block, 0, ctx->stream, nd,
+                                   src, g, dst, maxparts);
+            last_blocks = grid.x * grid.y * grid.z;
+            done += 1;
+        }
         TIPLHIP_CHECK(hipGetLastError());
         std::swap(src, dst);
     }
     if (maxlen) {
         hipLaunchKernelGGL(max_reduce_kernel, dim3(1), dim3(1024), 0, ctx->stream, maxparts,
-                           grid.x * grid.y * grid.z, maxlen);
+                           last_blocks, maxlen);
         TIPLHIP_CHECK(hipGetLastError());
     }
     return src;
@@ -953,6 +1103,80 @@ bool cdm_level(CdmRun& run, const std::vector<const uint8_t*>& It,
     return true;
 }
 
+// The fixed schedule (an extension; see tiplhip_cdm_param): `levels`
+// pyramid levels, all run, coarsest first, each for exactly
+// param.level_iterations iterations of the loop body with no cost-driven
+// control; the running field is kept and, with field_bandwidth > 0, smoothed
+// by filter::gaussian each iteration in place of cdm_smooth.  `field` is a
+// device field of vox(dim), overwritten.
+void cdm_fixed(CdmRun& run, const std::vector<const uint8_t*>& It,
+               const std::vector<const uint8_t*>& Is, const int* dim, float* field, int levels)
+{
+    tiplhip_ctx* ctx = run.ctx;
+    const FieldGeo g = field_geo(dim);
+    const size_t n = g.n, fb = sizeof(float) * 3 * n;
+    if (levels > 1) {
+        const int rd[3] = {(dim[0] + 1) / 2, (dim[1] + 1) / 2, (dim[2] + 1) / 2};
+        const size_t rn = vox(rd);
+        DevMem rbuf(ctx, 2 * size_t(run.nmod) * rn), rfield(ctx, sizeof(float) * 3 * rn);
+        std::vector<const uint8_t*> rIt, rIs;
+        for (int i = 0; i < run.nmod; ++i) {
+            uint8_t* t = rbuf.as<uint8_t>() + size_t(2 * i) * rn;
+            uint8_t* s = rbuf.as<uint8_t>() + size_t(2 * i + 1) * rn;
+            launch_downsample_u8(ctx, It[size_t(i)], dim, t);
+            launch_downsample_u8(ctx, Is[size_t(i)], dim, s);
+            rIt.push_back(t);
+            rIs.push_back(s);
+        }
+        cdm_fixed(run, rIt, rIs, rd, rfield.as<float>(), levels - 1);
+        zero(ctx, field, fb);  // far-edge slabs of the upsampled field stay 0
+        launch_upsample(ctx, rfield.as<float>(), field_geo(rd), g, field);
+    } else {
+        zero(ctx, field, fb);
+    }
+    if (run.stopped) return;
+    DevMem nd(ctx, fb), pa(ctx, fb), pb(ctx, fb), Js(ctx, n);
+    const uint32_t nblocks = block_count(g);
+    DevMem partials(ctx, sizeof(double) * nblocks), maxparts(ctx, sizeof(float) * nblocks);
+    DevMem scal(ctx, sizeof(float) * 16 + sizeof(unsigned int));
+    float* d_cost = scal.as<float>();
+    unsigned int* d_maxlen = reinterpret_cast<unsigned int*>(scal.as<float>() + 16);
+    float theta = 0.0f;
+    for (int index = 0; index < run.param.level_iterations; ++index) {
+        if (is_terminated(run)) {
+            run.stopped = true;
+            return;
+        }
+        ++run.iterations;
+        for (int i = 0; i < run.nmod; ++i) {
+            launch_compose(ctx, Is[size_t(i)], field, g, Js.as<uint8_t>());
+            const char type = run.param.gradient_type[std::min(i, 7)] == 'd' ? 'd' : 'r';
+            launch_gradient(ctx, Js.as<uint8_t>(), It[size_t(i)], g, type, i > 0 ? 1 : 0,
+                            nd.as<float>(), partials.as<double>(), d_cost + i);
+        }
+        const bool need_theta = theta == 0.0f;
+        float* solved = launch_poisson(ctx, nd.as<float>(), pa.as<float>(), pb.as<float>(), g, 12,
+                                       need_theta ? d_maxlen : nullptr, maxparts.as<float>());
+        if (need_theta) {
+            unsigned int bits = 0;
+            TIPLHIP_CHECK(hipMemcpyAsync(&bits, d_maxlen, sizeof(bits), hipMemcpyDeviceToHost,
+                                         ctx->stream));
+            TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
+            std::memcpy(&theta, &bits, sizeof(theta));
+        }
+        if (theta == 0.0f) break;
+        launch_accumulate(ctx, field, solved, run.param.speed / theta, g);
+        if (run.param.field_bandwidth > 0) {
+            float* spare = solved == pa.as<float>() ? pb.as<float>() : pa.as<float>();
+            launch_gaussian_to(ctx, solved, field, spare, dim, 3, unsigned(run.param.field_bandwidth));
+        } else if (run.param.smoothing != 0.0f) {
+            // cdm_smooth with weight 0 leaves the field untouched, as in the reference
+            launch_smooth(ctx, solved, field, run.param.smoothing, g);
+        }
+    }
+    TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
+}
+
 std::vector<const uint8_t*> volume_ptrs(tiplhip_ctx* ctx, const tiplhip_volume* const* v, int n,
                                         const int* dim)
 {
@@ -988,6 +1212,9 @@ void tiplhip_cdm_param_default(tiplhip_cdm_param* p)
     p->min_dimension = 16;
     p->max_iterations = 200;
     for (char& c : p->gradient_type) c = 'r';
+    p->levels = 0;
+    p->level_iterations = 0;
+    p->field_bandwidth = 0;
 }
 
 int tiplhip_cdm(tiplhip_ctx* ctx, const tiplhip_volume* const* It, const tiplhip_volume* const* Is,
@@ -1009,10 +1236,20 @@ int tiplhip_cdm(tiplhip_ctx* ctx, const tiplhip_volume* const* It, const tiplhip
         else
             tiplhip_cdm_param_default(&run.param);
         TIPLHIP_REQUIRE(run.param.max_iterations >= 0, "negative iteration cap");
+        TIPLHIP_REQUIRE(run.param.levels >= 0 && run.param.levels <= 16, "levels must be in [0, 16]");
+        TIPLHIP_REQUIRE(run.param.field_bandwidth >= 0 &&
+                            run.param.field_bandwidth <= int(kMaxGaussianBandwidth),
+                        "field_bandwidth must be in [0, 16]");
         const size_t fb = sizeof(float) * 3 * vox(dim);
         DevMem best(ctx, fb);
         zero(ctx, best.p, fb);
-        const bool ok = cdm_level(run, vt, vs, dim, best.as<float>(), run.param.resolution);
+        bool ok = true;
+        if (run.param.levels > 0) {
+            TIPLHIP_REQUIRE(run.param.level_iterations >= 0, "negative iteration count");
+            cdm_fixed(run, vt, vs, dim, best.as<float>(), run.param.levels);
+        } else {
+            ok = cdm_level(run, vt, vs, dim, best.as<float>(), run.param.resolution);
+        }
         if (iterations) *iterations = run.iterations;
         if (!ok) throw Error(TIPLHIP_ERR_FAILED, "cdm returned false (zero cost)");
         TIPLHIP_CHECK(hipMemcpyAsync(field, best.p, fb, hipMemcpyDeviceToHost, ctx->stream));

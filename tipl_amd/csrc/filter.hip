@@ -115,31 +115,67 @@ __global__ __launch_bounds__(256) void gaussian_round_kernel(const float* __rest
     }
 }
 
-}  // namespace
+GaussGeo gauss_geo(const int* dim)
+{
+    return GaussGeo{dim[0], dim[1], dim[2], uint32_t(dim[0]) * uint32_t(dim[1])};
+}
 
-float* launch_gaussian(tiplhip_ctx* ctx, float* a, float* tmp, const int* dim, int channels,
-                       unsigned bandwidth)
+void launch_round(tiplhip_ctx* ctx, const float* in, float* out, const GaussGeo& g, int channels,
+                  float scale)
+{
+    const dim3 grid(uint32_t((g.w + kGX - 1) / kGX), uint32_t((g.h + kGY - 1) / kGY),
+                    uint32_t((g.d + kGZ - 1) / kGZ));
+    const dim3 block(kGX, 4);
+    if (channels == 1)
+        hipLaunchKernelGGL(gaussian_round_kernel<1>, grid, block, 0, ctx->stream, in, out, g, scale);
+    else
+        hipLaunchKernelGGL(gaussian_round_kernel<3>, grid, block, 0, ctx->stream, in, out, g, scale);
+    TIPLHIP_CHECK(hipGetLastError());
+}
+
+void check_gaussian(const int* dim, int channels, unsigned bandwidth)
 {
     TIPLHIP_REQUIRE(channels == 1 || channels == 3, "gaussian: channels must be 1 or 3");
     TIPLHIP_REQUIRE(bandwidth <= kMaxGaussianBandwidth, "gaussian: bandwidth above 16");
     TIPLHIP_REQUIRE(vox(dim) * size_t(channels) < (size_t(1) << 32),
                     "gaussian: image exceeds 2^32 floats");
+}
+
+}  // namespace
+
+void launch_gaussian_to(tiplhip_ctx* ctx, const float* src, float* dst, float* tmp,
+                        const int* dim, int channels, unsigned bandwidth)
+{
+    check_gaussian(dim, channels, bandwidth);
+    const size_t bytes = sizeof(float) * size_t(channels) * vox(dim);
+    if (bandwidth == 0) {
+        TIPLHIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        return;
+    }
+    const GaussGeo g = gauss_geo(dim);
+    const float last_scale = std::ldexp(1.0f, -int(6 * bandwidth));
+    // round k writes dst when bandwidth-1-k is even, tmp otherwise, so the
+    // last round lands in dst and src is only read
+    const float* in = src;
+    for (unsigned k = 0; k < bandwidth; ++k) {
+        float* out = ((bandwidth - 1 - k) % 2 == 0) ? dst : tmp;
+        launch_round(ctx, in, out, g, channels, k + 1 == bandwidth ? last_scale : 1.0f);
+        in = out;
+    }
+}
+
+float* launch_gaussian(tiplhip_ctx* ctx, float* a, float* tmp, const int* dim, int channels,
+                       unsigned bandwidth)
+{
+    check_gaussian(dim, channels, bandwidth);
     if (bandwidth == 0 || vox(dim) == 0) return a;
-    GaussGeo g{dim[0], dim[1], dim[2], uint32_t(dim[0]) * uint32_t(dim[1])};
-    const dim3 grid(uint32_t((g.w + kGX - 1) / kGX), uint32_t((g.h + kGY - 1) / kGY),
-                    uint32_t((g.d + kGZ - 1) / kGZ));
-    const dim3 block(kGX, 4);
+    const GaussGeo g = gauss_geo(dim);
     // the scale is a power of two with a normal float value for bandwidth <= 16,
     // so scaling in float equals the reference's float(double(v) * scale)
     const float last_scale = std::ldexp(1.0f, -int(6 * bandwidth));
     float *src = a, *dst = tmp;
     for (unsigned k = 0; k < bandwidth; ++k) {
-        const float s = (k + 1 == bandwidth) ? last_scale : 1.0f;
-        if (channels == 1)
-            hipLaunchKernelGGL(gaussian_round_kernel<1>, grid, block, 0, ctx->stream, src, dst, g, s);
-        else
-            hipLaunchKernelGGL(gaussian_round_kernel<3>, grid, block, 0, ctx->stream, src, dst, g, s);
-        TIPLHIP_CHECK(hipGetLastError());
+        launch_round(ctx, src, dst, g, channels, (k + 1 == bandwidth) ? last_scale : 1.0f);
         std::swap(src, dst);
     }
     return src;

@@ -3,8 +3,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <new>
 #include <stdexcept>
@@ -139,6 +141,15 @@ void affine_matrix(const float* p, const int* fdim, const float* fvs, const int*
 bool matrix_inverse(float* T);
 bool is_integer_shift(const float* T, int* from_pos);
 
+// Launch shape of the affine cost kernels for one image pair: planes per
+// work item and workgroups per CU (chosen by tiplhip_autotune per key).
+struct LaunchShape {
+    uint32_t zc;
+    uint32_t per_cu;
+};
+// from dims, to dims, K, cost, bins
+using TuneKey = std::array<int, 9>;
+
 }  // namespace tiplhip
 
 struct tiplhip_ctx {
@@ -155,6 +166,8 @@ struct tiplhip_ctx {
     tiplhip::DeviceBuffer d_tmp2;
     tiplhip::PinnedBuffer h_T;
     tiplhip::PinnedBuffer h_out;
+    // autotuned launch shapes of single-pair cost launches
+    std::map<tiplhip::TuneKey, tiplhip::LaunchShape> tuned;
     // timing
     hipEvent_t t0 = nullptr, t1 = nullptr, k0 = nullptr, k1 = nullptr;
     bool kernel_timing = false;
@@ -205,5 +218,8 @@ void volume_compute_stats(tiplhip_ctx* ctx, tiplhip_volume* vol);
 constexpr unsigned kMaxGaussianBandwidth = 16;
 float* launch_gaussian(tiplhip_ctx* ctx, float* a, float* tmp, const int* dim, int channels,
                        unsigned bandwidth);
+// The same from src into dst (src is only read; tmp is clobbered).
+void launch_gaussian_to(tiplhip_ctx* ctx, const float* src, float* dst, float* tmp,
+                        const int* dim, int channels, unsigned bandwidth);
 
 }  // namespace tiplhip

@@ -59,6 +59,7 @@ void check_request(const tiplhip_reg_request* r)
         TIPLHIP_REQUIRE(dt == TIPLHIP_U8 || dt == TIPLHIP_I16 || dt == TIPLHIP_F32, "unknown dtype");
     TIPLHIP_REQUIRE(r->cost == TIPLHIP_COST_MI || r->cost == TIPLHIP_COST_CORR, "unknown cost type");
     TIPLHIP_REQUIRE(r->bins == 256 || r->bins == 64, "bins must be 256 or 64");
+    TIPLHIP_REQUIRE(r->preprocess == 0 || r->preprocess == 1, "preprocess must be 0 or 1");
 }
 
 size_t request_bytes(const tiplhip_reg_request& r)
@@ -70,6 +71,7 @@ size_t request_bytes(const tiplhip_reg_request& r)
     auto es = [](int dt) { return dt == TIPLHIP_F32 ? 4.0 : dt == TIPLHIP_I16 ? 2.0 : 0.0; };
     b += std::max(es(r.fixed_dtype) * nf, es(r.moving_dtype) * nm);  // quantisation staging
     if (r.nonlinear) b += nf * 75.0 * pyramid;        // J, best field, 4 work fields, Js
+    if (r.preprocess == 1) b += 12.0 * nm;            // subject preprocessing: 3 float images
     return size_t(b) + (size_t(4) << 20);
 }
 
@@ -77,12 +79,19 @@ size_t request_bytes(const tiplhip_reg_request& r)
 void register_pair(tipl::hip::context& ctx, const tiplhip_reg_request& req,
                    const tipl::reg::detail::cost_evaluator& eval, tiplhip_reg_result& res)
 {
+    const auto t0 = steady::now();
     tiplhip_volume* raw = nullptr;
-    rc_check(tiplhip_volume_create(ctx.get(), req.fixed, req.fixed_dtype, req.fixed_dim, 255.99f, &raw));
+    const bool mm = req.preprocess == 1;
+    rc_check(tiplhip_volume_create(ctx.get(), req.fixed, req.fixed_dtype, req.fixed_dim,
+                                   mm ? 255.999f : 255.99f, &raw));
     auto df = std::make_shared<tipl::hip::volume>(ctx, raw);
     raw = nullptr;
-    rc_check(tiplhip_volume_create(ctx.get(), req.moving, req.moving_dtype, req.moving_dim, 255.99f,
-                                   &raw));
+    if (mm)
+        rc_check(tiplhip_volume_create_subject(ctx.get(), req.moving, req.moving_dtype, req.moving_dim,
+                                               &raw));
+    else
+        rc_check(tiplhip_volume_create(ctx.get(), req.moving, req.moving_dtype, req.moving_dim,
+                                       255.99f, &raw));
     auto dm = std::make_shared<tipl::hip::volume>(ctx, raw);
     // has_mask and the moment estimate read the quantised images on the host
     const tipl::image<3, uint8_t> hf = df->read(), hm = dm->read();
@@ -96,6 +105,7 @@ void register_pair(tipl::hip::context& ctx, const tiplhip_reg_request& req,
     tipl::affine_param<float, 3> arg;
     bool terminated = false;
     size_t evals = 0;
+    const auto t1 = steady::now();
     const float cost = tipl::reg::detail::linear_impl<void>(
         {tipl::const_pointer_image<3, unsigned char>(hf)}, fvs,
         {tipl::const_pointer_image<3, unsigned char>(hm)}, mvs, {df}, {dm}, arg, p, terminated, eval,
@@ -106,6 +116,11 @@ void register_pair(tipl::hip::context& ctx, const tiplhip_reg_request& req,
     res.cost = cost;
     res.evaluations = int64_t(evals);
     res.cdm_iterations = 0;
+    const auto t2 = steady::now();
+    // linear_impl runs has_mask / estimate_affine_param on the host first
+    res.prep_ms = ms_between(t0, t1);
+    res.affine_ms = ms_between(t1, t2);
+    res.cdm_ms = 0.0;
     if (!req.nonlinear) return;
 
     // J: the moving image on the fixed grid (tipl::resample, CPU crop rule)
@@ -123,6 +138,7 @@ void register_pair(tipl::hip::context& ctx, const tiplhip_reg_request& req,
     int iterations = 0;
     const int rc = tiplhip_cdm(ctx.get(), &It, &Is, 1, &req.cdm, field, nullptr, &iterations);
     res.cdm_iterations = iterations;
+    res.cdm_ms = ms_between(t2, steady::now());
     rc_check(rc);
 }
 
@@ -131,7 +147,13 @@ void register_pair(tipl::hip::context& ctx, const tiplhip_reg_request& req,
 // ---------------------------------------------------------------------------
 class Batcher {
 public:
-    explicit Batcher(int device) : ctx_(device), thread_([this] { run(); }) {}
+    // `launchers` threads, each with its own context (stream): while one
+    // batch runs on the device the next one is formed and launched.
+    Batcher(int device, int launchers)
+    {
+        for (int i = 0; i < launchers; ++i) ctx_.push_back(std::make_unique<tipl::hip::context>(device));
+        for (int i = 0; i < launchers; ++i) threads_.emplace_back([this, i] { run(*ctx_[size_t(i)]); });
+    }
     ~Batcher()
     {
         {
@@ -139,7 +161,7 @@ public:
             stop_ = true;
         }
         cv_.notify_all();
-        thread_.join();
+        for (auto& t : threads_) t.join();
     }
 
     // The cost_evaluator of a served registration: blocks until the
@@ -190,7 +212,7 @@ private:
         std::string err;
     };
 
-    void run()
+    void run(tipl::hip::context& ctx)
     {
         for (;;) {
             std::vector<Job*> batch;
@@ -216,7 +238,7 @@ private:
             }
             std::vector<tiplhip_pair_costs> pc;
             for (Job* j : batch) pc.push_back({j->from, j->to, j->T.data(), j->K, j->costs});
-            const int rc = tiplhip_cost_batch_pairs(ctx_.get(), pc.data(), int(pc.size()), cost, bins);
+            const int rc = tiplhip_cost_batch_pairs(ctx.get(), pc.data(), int(pc.size()), cost, bins);
             const std::string err = rc == TIPLHIP_OK ? std::string() : std::string(tiplhip_last_error());
             {
                 std::lock_guard<std::mutex> lk(mu_);
@@ -232,13 +254,13 @@ private:
         }
     }
 
-    tipl::hip::context ctx_;
+    std::vector<std::unique_ptr<tipl::hip::context>> ctx_;
     std::mutex mu_;
     std::condition_variable cv_, done_cv_;
     std::deque<Job*> pending_;
     bool stop_ = false;
     int64_t launches_ = 0, pairs_ = 0;
-    std::thread thread_;
+    std::vector<std::thread> threads_;
 };
 
 struct Ticket {
@@ -382,7 +404,8 @@ int tiplhip_server_create(int device, int slots, size_t hbm_budget, tiplhip_serv
         auto s = std::make_unique<tiplhip_server>();
         s->device = device;
         s->budget = hbm_budget;
-        s->batcher = std::make_unique<Batcher>(device);
+        // one launcher per 4 slots (at least 1, at most 4)
+        s->batcher = std::make_unique<Batcher>(device, std::min(4, std::max(1, slots / 4)));
         for (int i = 0; i < slots; ++i) s->contexts.push_back(std::make_unique<tipl::hip::context>(device));
         for (int i = 0; i < slots; ++i)
             s->workers.emplace_back([p = s.get(), c = s->contexts[size_t(i)].get()] { p->worker(*c); });
