@@ -5,11 +5,13 @@
 #define TIPL_HIP_TIPL_HPP
 
 #include "def.hpp"
+#include "filter/gaussian.hpp"
 #include "hip.hpp"
 #include "image.hpp"
 #include "optimization.hpp"
 #include "reg/cdm.hpp"
 #include "reg/linear.hpp"
+#include "reg/mm_reg.hpp"
 #include "transformation.hpp"
 
 #endif

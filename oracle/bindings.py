@@ -50,7 +50,84 @@ class _Lib:
         return f
 
 
-class Oracle(_Lib):
+class _Warp:
+    """Post-registration warping (numerical/dif.hpp) and mm_reg's r, shared
+    by both libraries: the C entry points have the same signatures."""
+
+    def _bind_warp(self, prefix: str):
+        _vp = C.c_void_p
+        self._d2m = self._fn(prefix + "disp_to_mapping", None, _f32p, _i32p, _vp, _f32p)
+        self._m2d = (self._fn(prefix + "mapping_to_disp", None, _f32p, _i32p, _f32p)
+                     if hasattr(self.lib, prefix + "mapping_to_disp") else None)
+        self._inv_d2m = self._fn(prefix + "inv_disp_to_mapping", None, _f32p, _i32p, _f32p, _i32p,
+                                 _f32p)
+        self._cm_u8 = self._fn(prefix + "compose_mapping_u8", None, _u8p, _i32p, _f32p, _i32p, _u8p)
+        self._cm_f32 = self._fn(prefix + "compose_mapping_f32", None, _f32p, _i32p, _f32p, _i32p,
+                                _f32p)
+        self._jdd = self._fn(prefix + "jacobian_det_dis", None, _f32p, _i32p, _f32p)
+        self._jd = self._fn(prefix + "jacobian_det", None, _f32p, _i32p, _f32p)
+        self._corr_r = self._fn(prefix + "correlation", C.c_double, _u8p, _u8p, C.c_size_t, C.c_int)
+
+    @staticmethod
+    def _fdims(field):
+        return np.array(field.shape[:3][::-1], np.int32)
+
+    def displacement_to_mapping(self, dis, T=None) -> np.ndarray:
+        dis = _f3(dis)
+        out = np.zeros_like(dis)
+        Tp = None
+        if T is not None:
+            Tarr = np.ascontiguousarray(T, np.float32)
+            Tp = Tarr.ctypes.data
+        self._d2m(dis, self._fdims(dis), Tp, out)
+        return out
+
+    def mapping_to_displacement(self, mapping) -> np.ndarray:
+        mapping = _f3(mapping)
+        out = np.zeros_like(mapping)
+        self._m2d(mapping, self._fdims(mapping), out)
+        return out
+
+    def inv_displacement_to_mapping(self, inv_dis, T, out_shape) -> np.ndarray:
+        inv_dis = _f3(inv_dis)
+        out = np.zeros(tuple(out_shape) + (3,), np.float32)
+        self._inv_d2m(inv_dis, self._fdims(inv_dis), np.ascontiguousarray(T, np.float32),
+                      np.array(tuple(out_shape)[::-1], np.int32), out)
+        return out
+
+    def compose_mapping(self, src, mapping) -> np.ndarray:
+        mapping = _f3(mapping)
+        shape = mapping.shape[:3]
+        if np.asarray(src).dtype == np.uint8:
+            src = np.ascontiguousarray(src, np.uint8)
+            out = np.zeros(shape, np.uint8)
+            self._cm_u8(src, _dims(src), mapping, self._fdims(mapping), out)
+        else:
+            src = np.ascontiguousarray(src, np.float32)
+            out = np.zeros(shape, np.float32)
+            self._cm_f32(src, _dims(src), mapping, self._fdims(mapping), out)
+        return out
+
+    def jacobian_determinant_dis(self, dis) -> np.ndarray:
+        dis = _f3(dis)
+        out = np.zeros(dis.shape[:3], np.float32)
+        self._jdd(dis, self._fdims(dis), out)
+        return out
+
+    def jacobian_determinant(self, mapping) -> np.ndarray:
+        mapping = _f3(mapping)
+        out = np.zeros(mapping.shape[:3], np.float32)
+        self._jd(mapping, self._fdims(mapping), out)
+        return out
+
+    def correlation(self, x, y, masked=True) -> float:
+        x = np.ascontiguousarray(x, np.uint8).ravel()
+        y = np.ascontiguousarray(y, np.uint8).ravel()
+        assert x.size == y.size
+        return float(self._corr_r(x, y, x.size, int(bool(masked))))
+
+
+class Oracle(_Lib, _Warp):
     """The C restatement; every method mirrors one reference routine."""
 
     def __init__(self, path: str = ORACLE_SO):
@@ -90,6 +167,15 @@ class Oracle(_Lib):
         self._cdm = self._fn("orc_cdm", C.c_int, P(C.c_void_p), P(C.c_void_p), C.c_int, _i32p,
                              C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, C.c_char, _f32p,
                              P(C.c_int))
+        self._bind_warp("orc_")
+        self._fov = self._fn("orc_match_fov", None, _u8p, _i32p, _f32p, _i32p)
+
+    def match_fov(self, It, T, moving_shape) -> np.ndarray:
+        """mm_reg::It_match_fov on a copy of It (T maps It voxels to the moving grid)."""
+        out = np.array(It, np.uint8, order="C")
+        self._fov(out, _dims(out), np.ascontiguousarray(T, np.float32),
+                  np.array(tuple(moving_shape)[::-1], np.int32))
+        return out
 
     # -- geometry ---------------------------------------------------------
     def affine_matrix(self, param, fdim, fvs, tdim, tvs) -> np.ndarray:
@@ -277,7 +363,7 @@ class Oracle(_Lib):
         return bool(ok), field, iters.value
 
 
-class Reference(_Lib):
+class Reference(_Lib, _Warp):
     """The reference headers compiled through oracle/ref_shim.cpp."""
 
     def __init__(self, path: str = REF_SO):
@@ -316,6 +402,32 @@ class Reference(_Lib):
         self._median = self._fn("tref_median", C.c_float, _f32p, C.c_size_t)
         self._cdm_iters = self._fn("tref_cdm_iterations", C.c_double, _u8p, _u8p, _i32p, C.c_int,
                                    C.c_void_p)
+        self._bind_warp("tref_")
+        self._fov = self._fn("tref_match_fov", None, _u8p, _i32p, _f32p, _i32p, _f32p, _f32p)
+        self._mm = self._fn("tref_mm_reg", None, _u8p, _i32p, _f32p, _u8p, _i32p, _f32p, _f32p,
+                            C.c_int, C.c_int, C.c_int, _f32p, _u8p, _u8p, _f32p, _f32p, _f32p,
+                            _f32p, _u8p)
+
+    def match_fov(self, It, tvs, moving_shape, mvs, arg) -> np.ndarray:
+        out = np.array(It, np.uint8, order="C")
+        self._fov(out, _dims(out), np.asarray(tvs, np.float32),
+                  np.array(tuple(moving_shape)[::-1], np.int32), np.asarray(mvs, np.float32),
+                  np.asarray(arg, np.float32))
+        return out
+
+    def mm_reg(self, It, tvs, I, svs, arg, reverse=True, match_fov=True, masked_r=True):
+        """The reference mm_reg's nonlinear half with a given affine; returns a dict."""
+        It = np.ascontiguousarray(It, np.uint8)
+        I = np.ascontiguousarray(I, np.uint8)
+        f = lambda shp: np.zeros(tuple(shp) + (3,), np.float32)
+        out = dict(r=np.zeros(2, np.float32), J_lin=np.zeros_like(It), It_fov=np.zeros_like(It),
+                   t2f=f(It.shape), f2t=f(It.shape), to2from=f(It.shape), from2to=f(I.shape),
+                   JJ=np.zeros_like(It))
+        self._mm(It, _dims(It), np.asarray(tvs, np.float32), I, _dims(I),
+                 np.asarray(svs, np.float32), np.asarray(arg, np.float32), int(reverse),
+                 int(match_fov), int(masked_r), out["r"], out["J_lin"], out["It_fov"], out["t2f"],
+                 out["f2t"], out["to2from"], out["from2to"], out["JJ"])
+        return out
 
     def cdm_iterations(self, It, Is, iters, want_field=False):
         """Wall seconds of `iters` reference cdm() loop-body iterations at one level."""

@@ -1222,3 +1222,226 @@ ORC_API void orc_subject_pre(const float* in, const int* dim, uint8_t* out)
     for (size_t i = 0; i < n; ++i) out[i] = orc_to_u8(I[i]);
     free(I);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Post-registration warping (numerical/dif.hpp) and mm_reg's r values       */
+/* ------------------------------------------------------------------------ */
+
+/* displacement_to_mapping(dis, mapping) numerical/dif.hpp:19-28 (T == NULL)
+ * and displacement_to_mapping(dis, mapping, T) :42-50:
+ * mapping(x) = dis(x) + vector(x), then T applied in place
+ * (transformation_matrix::operator()(vector&) numerical/transformation.hpp:846-850). */
+ORC_API void orc_disp_to_mapping(const float* dis, const int* dim, const float* T, float* out)
+{
+    size_t i = 0;
+    for (int z = 0; z < dim[2]; ++z)
+        for (int y = 0; y < dim[1]; ++y)
+            for (int x = 0; x < dim[0]; ++x, ++i) {
+                float m[3] = {dis[3 * i] + (float)x, dis[3 * i + 1] + (float)y,
+                              dis[3 * i + 2] + (float)z};
+                if (T) {
+                    float o[3];
+                    orc_apply(T, m[0], m[1], m[2], o);
+                    memcpy(m, o, sizeof(m));
+                }
+                memcpy(out + 3 * i, m, sizeof(m));
+            }
+}
+
+/* mapping_to_displacement numerical/dif.hpp:30-38. */
+ORC_API void orc_mapping_to_disp(const float* mapping, const int* dim, float* out)
+{
+    size_t i = 0;
+    for (int z = 0; z < dim[2]; ++z)
+        for (int y = 0; y < dim[1]; ++y)
+            for (int x = 0; x < dim[0]; ++x, ++i) {
+                out[3 * i] = mapping[3 * i] - (float)x;
+                out[3 * i + 1] = mapping[3 * i + 1] - (float)y;
+                out[3 * i + 2] = mapping[3 * i + 2] - (float)z;
+            }
+}
+
+/* inv_displacement_to_mapping<linear>(inv_dis, inv_mapping, T)
+ * numerical/dif.hpp:52-64: p = T^-1(x) over the mapping's grid, then
+ * p += estimate<linear>(inv_dis, p) (zero vector when outside,
+ * numerical/interpolation.hpp:726-740). */
+ORC_API void orc_inv_disp_to_mapping(const float* inv_dis, const int* ddim, const float* T_in,
+                                     const int* odim, float* out)
+{
+    float iT[12];
+    memcpy(iT, T_in, sizeof(iT));
+    orc_matrix_inverse(iT);   /* the reference ignores the return value */
+    size_t i = 0;
+    for (int z = 0; z < odim[2]; ++z)
+        for (int y = 0; y < odim[1]; ++y)
+            for (int x = 0; x < odim[0]; ++x, ++i) {
+                float p[3], v[3] = {0.0f, 0.0f, 0.0f};
+                orc_apply(iT, (float)x, (float)y, (float)z, p);
+                orc_tri t;
+                if (orc_tri_locate(ddim, p[0], p[1], p[2], &t))
+                    orc_tri_f3(inv_dis, &t, v);
+                out[3 * i] = p[0] + v[0];
+                out[3 * i + 1] = p[1] + v[1];
+                out[3 * i + 2] = p[2] + v[2];
+            }
+}
+
+/* compose_mapping<linear>(from, mapping, to) numerical/dif.hpp:66-86 for a
+ * uint8 image: to(x) = from(mapping(x)), 0 where the sample is outside. */
+ORC_API void orc_compose_mapping_u8(const uint8_t* from, const int* fdim, const float* mapping,
+                                    const int* dim, uint8_t* out)
+{
+    size_t n = vox(dim);
+    for (size_t i = 0; i < n; ++i)
+        out[i] = orc_sample_u8(from, fdim, mapping[3 * i], mapping[3 * i + 1], mapping[3 * i + 2]);
+}
+
+/* The same for a float image (weighted_sum in float, no truncation). */
+ORC_API void orc_compose_mapping_f32(const float* from, const int* fdim, const float* mapping,
+                                     const int* dim, float* out)
+{
+    size_t n = vox(dim);
+    for (size_t i = 0; i < n; ++i) {
+        orc_tri t;
+        float r = 0.0f;
+        if (orc_tri_locate(fdim, mapping[3 * i], mapping[3 * i + 1], mapping[3 * i + 2], &t)) {
+            r = from[t.idx[0]] * t.w[0];
+            for (int k = 1; k < 8; ++k) {
+                float v = from[t.idx[k]];
+                v *= t.w[k];
+                r += v;
+            }
+        }
+        out[i] = r;
+    }
+}
+
+static int orc_is_edge(const int* dim, int x, int y, int z)
+{
+    return x == 0 || y == 0 || z == 0 || x + 1 == dim[0] || y + 1 == dim[1] || z + 1 == dim[2];
+}
+
+/* jacobian_determinant_dis(displacement, dest) numerical/dif.hpp:451-465 with
+ * jacobian_determinant_dis_at<3> :399-423: central differences (not halved)
+ * with +1.0 on the diagonal.  The mixed float/double expression types of the
+ * reference are kept: differences are float, the diagonal terms and every
+ * product with one of them are double, the other products are float.  The
+ * double result is stored as float; edge voxels are 1. */
+ORC_API void orc_jacobian_det_dis(const float* d, const int* dim, float* out)
+{
+    const size_t w = (size_t)dim[0], wh = w * (size_t)dim[1];
+    size_t i = 0;
+    for (int z = 0; z < dim[2]; ++z)
+        for (int y = 0; y < dim[1]; ++y)
+            for (int x = 0; x < dim[0]; ++x, ++i) {
+                if (orc_is_edge(dim, x, y, z)) {
+                    out[i] = 1.0f;
+                    continue;
+                }
+                const float* v1_0 = d + 3 * (i + 1);
+                const float* v1_1 = d + 3 * (i - 1);
+                const float* v2_0 = d + 3 * (i + w);
+                const float* v2_1 = d + 3 * (i - w);
+                const float* v3_0 = d + 3 * (i + wh);
+                const float* v3_1 = d + 3 * (i - wh);
+                float d2_0 = v2_0[0] - v2_1[0];
+                double d2_1 = (double)(v2_0[1] - v2_1[1]) + 1.0;
+                float d2_2 = v2_0[2] - v2_1[2];
+                float d3_0 = v3_0[0] - v3_1[0];
+                float d3_1 = v3_0[1] - v3_1[1];
+                double d3_2 = (double)(v3_0[2] - v3_1[2]) + 1.0;
+                double a = (double)(v1_0[0] - v1_1[0]) + 1.0;
+                float b = v1_0[1] - v1_1[1];
+                float c = v1_0[2] - v1_1[2];
+                float p22_31 = d2_2 * d3_1;
+                float p22_30 = d2_2 * d3_0;
+                float p20_31 = d2_0 * d3_1;
+                double r = a * (d2_1 * d3_2 - (double)p22_31) +
+                           (double)b * ((double)p22_30 - (double)d2_0 * d3_2) +
+                           (double)c * ((double)p20_31 - d2_1 * (double)d3_0);
+                out[i] = (float)r;
+            }
+}
+
+/* jacobian_determinant(src, dest) numerical/dif.hpp:356-388 of a mapping:
+ * float central differences; edge voxels are left untouched (0 here, as in
+ * a freshly sized dest).  The reference's expression applies its 0.125f
+ * factor to the third product only; that is kept. */
+ORC_API void orc_jacobian_det(const float* m, const int* dim, float* out)
+{
+    const size_t w = (size_t)dim[0], wh = w * (size_t)dim[1];
+    size_t i = 0;
+    for (int z = 0; z < dim[2]; ++z)
+        for (int y = 0; y < dim[1]; ++y)
+            for (int x = 0; x < dim[0]; ++x, ++i) {
+                if (orc_is_edge(dim, x, y, z)) {
+                    out[i] = 0.0f;
+                    continue;
+                }
+                float dx[3], dy[3], dz[3];
+                for (int c = 0; c < 3; ++c) {
+                    dx[c] = m[3 * (i + 1) + c] - m[3 * (i - 1) + c];
+                    dy[c] = m[3 * (i + w) + c] - m[3 * (i - w) + c];
+                    dz[c] = m[3 * (i + wh) + c] - m[3 * (i - wh) + c];
+                }
+                float t0 = dx[0] * (dy[1] * dz[2] - dy[2] * dz[1]);
+                float t1 = dx[1] * (dy[2] * dz[0] - dy[0] * dz[2]);
+                float t2 = dx[2] * (dy[0] * dz[1] - dy[1] * dz[0]) * 0.125f;
+                out[i] = t0 + t1 + t2;
+            }
+}
+
+/* tipl::correlation(x, y) as r (numerical/statistics.hpp:569-597) from the
+ * exact uint8 sums s = {Sx, Sxx, Sy, Syy, Sxy} over n voxels; 0 for n = 0. */
+ORC_API double orc_corr_r_from_sums(const int64_t* s, size_t n)
+{
+    if (n == 0) return 0.0;
+    double dn = (double)n;
+    double mean_x = (double)s[0] / dn;
+    double mean_y = (double)s[2] / dn;
+    double var_x = (double)s[1] / dn - mean_x * mean_x;
+    double var_y = (double)s[3] / dn - mean_y * mean_y;
+    double sd1 = var_x > 0.0 ? sqrt(var_x) : 0.0;
+    double sd2 = var_y > 0.0 ? sqrt(var_y) : 0.0;
+    if (sd1 == 0.0 || sd2 == 0.0) return 0.0;
+    double cov = (double)s[4] / dn - mean_x * mean_y;
+    return cov / sd1 / sd2;
+}
+
+/* mm_reg's r (reg/mm_reg.hpp:302-305, 341-343): correlation_ygz
+ * (numerical/statistics.hpp:599-619, only voxels with y > 0) when masked,
+ * else tipl::correlation over every voxel. */
+ORC_API double orc_correlation(const uint8_t* x, const uint8_t* y, size_t n, int masked)
+{
+    int64_t s[5] = {0, 0, 0, 0, 0};
+    size_t m = 0;
+    for (size_t i = 0; i < n; ++i) {
+        if (masked && y[i] == 0) continue;
+        int64_t a = x[i], b = y[i];
+        s[0] += a;
+        s[1] += a * a;
+        s[2] += b;
+        s[3] += b * b;
+        s[4] += a * b;
+        ++m;
+    }
+    return orc_corr_r_from_sums(s, m);
+}
+
+/* mm_reg::It_match_fov reg/mm_reg.hpp:313-328: zero every template voxel whose
+ * T(x) lies outside the subject grid; "inside" is shape::is_valid
+ * (utility/shape.hpp:582-587): 0 <= p < dim as float comparisons. */
+ORC_API void orc_match_fov(uint8_t* It, const int* tdim, const float* T, const int* sdim)
+{
+    size_t i = 0;
+    for (int z = 0; z < tdim[2]; ++z)
+        for (int y = 0; y < tdim[1]; ++y)
+            for (int x = 0; x < tdim[0]; ++x, ++i) {
+                float p[3];
+                orc_apply(T, (float)x, (float)y, (float)z, p);
+                int valid = p[0] >= 0.0f && p[1] >= 0.0f && p[2] >= 0.0f &&
+                            p[0] < (float)(unsigned)sdim[0] && p[1] < (float)(unsigned)sdim[1] &&
+                            p[2] < (float)(unsigned)sdim[2];
+                if (!valid) It[i] = 0;
+            }
+}

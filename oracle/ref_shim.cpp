@@ -32,6 +32,12 @@ tipl::transformation_matrix<float, 3> mat(const float* T)
 
 using field_t = tipl::image<3, tipl::vector<3>>;
 
+// Log sink for mm_reg<out_type>, whose show_r writes unconditionally.
+struct null_out {
+    template <class T>
+    null_out& operator<<(const T&) { return *this; }
+};
+
 field_t to_field(const float* f, const int* d)
 {
     field_t F(shp(d));
@@ -362,4 +368,127 @@ float tref_linear(const uint8_t* from, const int* fdim, const float* fvs, const 
     return r;
 }
 
+// Post-registration warping (numerical/dif.hpp) and mm_reg's r values.
+void tref_disp_to_mapping(const float* dis, const int* dim, const float* T, float* out)
+{
+    field_t D = to_field(dis, dim), M;
+    if (T)
+        tipl::displacement_to_mapping(D, M, mat(T));
+    else
+        tipl::displacement_to_mapping(D, M);
+    from_field(M, out);
+}
+
+void tref_mapping_to_disp(const float* mapping, const int* dim, float* out)
+{
+    field_t M = to_field(mapping, dim);
+    tipl::mapping_to_displacement(M);
+    from_field(M, out);
+}
+
+void tref_inv_disp_to_mapping(const float* inv_dis, const int* ddim, const float* T,
+                              const int* odim, float* out)
+{
+    field_t D = to_field(inv_dis, ddim), M(shp(odim));
+    tipl::inv_displacement_to_mapping(D, M, mat(T));
+    from_field(M, out);
+}
+
+void tref_compose_mapping_u8(const uint8_t* from, const int* fdim, const float* mapping,
+                             const int* dim, uint8_t* out)
+{
+    tipl::const_pointer_image<3, uint8_t> I(from, shp(fdim));
+    field_t M = to_field(mapping, dim);
+    tipl::image<3, uint8_t> J;
+    tipl::compose_mapping(I, M, J);
+    std::memcpy(out, J.data(), J.size());
+}
+
+void tref_compose_mapping_f32(const float* from, const int* fdim, const float* mapping,
+                              const int* dim, float* out)
+{
+    tipl::const_pointer_image<3, float> I(from, shp(fdim));
+    field_t M = to_field(mapping, dim);
+    tipl::image<3, float> J;
+    tipl::compose_mapping(I, M, J);
+    std::memcpy(out, J.data(), sizeof(float) * J.size());
+}
+
+void tref_jacobian_det_dis(const float* dis, const int* dim, float* out)
+{
+    field_t D = to_field(dis, dim);
+    tipl::image<3, float> J;
+    tipl::jacobian_determinant_dis(D, J);
+    std::memcpy(out, J.data(), sizeof(float) * J.size());
+}
+
+void tref_jacobian_det(const float* mapping, const int* dim, float* out)
+{
+    field_t M = to_field(mapping, dim);
+    auto J = tipl::jacobian_determinant(M);
+    std::memcpy(out, J.data(), sizeof(float) * J.size());
+}
+
+double tref_correlation(const uint8_t* x, const uint8_t* y, size_t n, int masked)
+{
+    return masked ? tipl::correlation_ygz(x, x + n, y) : tipl::correlation(x, x + n, y);
+}
+
+// mm_reg::It_match_fov on a one-modality mm_reg (reg/mm_reg.hpp:313-328);
+// arg is the affine parameter vector, as mm_reg holds it.
+void tref_match_fov(uint8_t* It, const int* tdim, const float* tvs, const int* sdim,
+                    const float* svs, const float* arg)
+{
+    tipl::reg::mm_reg<null_out> m;
+    m.It[0] = tipl::image<3, uint8_t>(It, shp(tdim));
+    m.Its = shp(tdim);
+    m.Is = shp(sdim);
+    m.Itvs = tipl::vector<3>(tvs);
+    m.Ivs = tipl::vector<3>(svs);
+    m.arg = tipl::affine_param<float, 3>(arg);
+    m.It_match_fov();
+    std::memcpy(It, m.It[0].data(), m.It[0].size());
+}
+
+// The nonlinear half of mm_reg with a given affine (skip_linear): linear_reg
+// -> calculate_linear_r (J, r, It_match_fov), then nonlinear_reg (reverse CDM
+// on its own thread, forward CDM, mappings, calculate_nonlinear_r), all on
+// the CPU (reg/mm_reg.hpp:271-311, 330-416).  r[0] / r[1] are the linear and
+// nonlinear r; the images and fields are copied out after each stage.
+void tref_mm_reg(const uint8_t* It, const int* tdim, const float* tvs, const uint8_t* I,
+                 const int* sdim, const float* svs, const float* arg, int reverse, int match_fov,
+                 int masked_r, float* r, uint8_t* J_lin, uint8_t* It_fov, float* t2f, float* f2t,
+                 float* to2from, float* from2to, uint8_t* JJ)
+{
+    tipl::reg::mm_reg<null_out> m;
+    m.It[0] = tipl::image<3, uint8_t>(It, shp(tdim));
+    m.I[0] = tipl::image<3, uint8_t>(I, shp(sdim));
+    m.Its = shp(tdim);
+    m.Is = shp(sdim);
+    m.Itvs = tipl::vector<3>(tvs);
+    m.Ivs = tipl::vector<3>(svs);
+    m.arg = tipl::affine_param<float, 3>(arg);
+    m.skip_linear = true;
+    m.skip_reverse = !reverse;
+    m.match_fov = match_fov != 0;
+    m.masked_r = masked_r != 0;
+    m.use_cuda = false;
+    bool terminated = false;
+    m.linear_reg(terminated);
+    r[0] = m.r[0];
+    std::memcpy(J_lin, m.J[0].data(), m.J[0].size());
+    std::memcpy(It_fov, m.It[0].data(), m.It[0].size());
+    m.nonlinear_reg(terminated);
+    r[1] = m.r[0];
+    from_field(m.t2f_dis, t2f);
+    if (reverse) {
+        from_field(m.f2t_dis, f2t);
+        from_field(m.from2to, from2to);
+    }
+    from_field(m.to2from, to2from);
+    std::memcpy(JJ, m.J[0].data(), m.J[0].size());
+}
+
 }  // extern "C"
+
+

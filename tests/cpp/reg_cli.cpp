@@ -6,6 +6,10 @@
 //       parameters and the final cost.
 //   reg_cli cdm <It.u8> <Is.u8> <w> <h> <d> <out.f32>
 //       tipl::reg::cdm_common, writes the displacement field (w*h*d*3 floats).
+//   reg_cli gaussian <in.f32> <w> <h> <d> <channels> <bandwidth> <out.f32>
+//       tipl::filter::gaussian on a float image (1) or vector<3> field (3).
+//   reg_cli subject <in.f32> <w> <h> <d> <out.u8>
+//       tipl::reg::subject_image_pre.
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -24,9 +28,52 @@ static std::vector<unsigned char> read_u8(const char* path, size_t n)
     return v;
 }
 
+template <class T>
+static std::vector<T> read_raw(const char* path, size_t n)
+{
+    std::vector<T> v(n);
+    std::ifstream f(path, std::ios::binary);
+    if (!f.read(reinterpret_cast<char*>(v.data()), std::streamsize(n * sizeof(T))))
+        throw std::runtime_error(std::string("cannot read ") + path);
+    return v;
+}
+
+template <class T>
+static void write_raw(const char* path, const T* p, size_t n)
+{
+    std::ofstream f(path, std::ios::binary);
+    f.write(reinterpret_cast<const char*>(p), std::streamsize(n * sizeof(T)));
+}
+
 int main(int argc, char** argv)
 {
     try {
+        if (argc >= 9 && std::string(argv[1]) == "gaussian") {
+            const tipl::shape<3> s(unsigned(std::atoi(argv[3])), unsigned(std::atoi(argv[4])),
+                                   unsigned(std::atoi(argv[5])));
+            const int ch = std::atoi(argv[6]);
+            const unsigned bw = unsigned(std::atoi(argv[7]));
+            auto v = read_raw<float>(argv[2], s.size() * size_t(ch));
+            if (ch == 3) {
+                tipl::image<3, tipl::vector<3>> F(reinterpret_cast<const tipl::vector<3>*>(v.data()), s);
+                tipl::filter::gaussian(F, bw);
+                write_raw(argv[8], reinterpret_cast<const float*>(F.data()), 3 * s.size());
+            } else {
+                tipl::image<3, float> I(v.data(), s);
+                tipl::filter::gaussian(I, bw);
+                write_raw(argv[8], I.data(), s.size());
+            }
+            return 0;
+        }
+        if (argc >= 7 && std::string(argv[1]) == "subject") {
+            const tipl::shape<3> s(unsigned(std::atoi(argv[3])), unsigned(std::atoi(argv[4])),
+                                   unsigned(std::atoi(argv[5])));
+            auto v = read_raw<float>(argv[2], s.size());
+            tipl::image<3, float> I(v.data(), s);
+            auto J = tipl::reg::subject_image_pre(I);
+            write_raw(argv[6], J.data(), J.size());
+            return 0;
+        }
         if (argc < 8) {
             std::cerr << "usage: reg_cli linear|cdm ...\n";
             return 2;

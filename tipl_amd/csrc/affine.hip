@@ -422,6 +422,7 @@ __global__ __launch_bounds__(kEntropyThreads) void mi_entropy_kernel(
     {
         const int c = threadIdx.x % BINS, sl = threadIdx.x / BINS;
         uint32_t s = 0;
+#pragma unroll 16
         for (int r = sl * kRows; r < (sl + 1) * kRows; ++r) s += uint32_t(h[r * BINS + c]);
         colpart[sl][c] = s;
     }
@@ -432,22 +433,35 @@ __global__ __launch_bounds__(kEntropyThreads) void mi_entropy_kernel(
         to_h[threadIdx.x] = s;
     }
     __syncthreads();
+    // four bins per 16-byte load, all of a thread's loads issued before use
+    constexpr int kVec = BINS * BINS / 4 / kEntropyThreads;
+    const int4* h4 = reinterpret_cast<const int4*>(h);
+    int4 q[kVec];
+#pragma unroll
+    for (int u = 0; u < kVec; ++u) q[u] = h4[threadIdx.x + u * kEntropyThreads];
     double acc = 0.0;
-    for (int i = threadIdx.x; i < BINS * BINS; i += kEntropyThreads) {
-        const int mu = h[i];
-        if (mu == 0) continue;
-        const double m = double(mu);
-        acc += m * log(m / double(from_hist[i / BINS]) / double(to_h[i % BINS]));
+#pragma unroll
+    for (int u = 0; u < kVec; ++u) {
+        const int i0 = 4 * (int(threadIdx.x) + u * kEntropyThreads);
+        const int mus[4] = {q[u].x, q[u].y, q[u].z, q[u].w};
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int mu = mus[c];
+            if (mu == 0) continue;
+            const int i = i0 + c;
+            const double m = double(mu);
+            acc += m * log(m / double(from_hist[i / BINS]) / double(to_h[i % BINS]));
+        }
     }
     acc = wave_sum_f64(acc);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
     __syncthreads();
     if (threadIdx.x == 0) {
-        double q[kWaves];
-        for (int i = 0; i < kWaves; ++i) q[i] = part[i];
+        double t[kWaves];
+        for (int i = 0; i < kWaves; ++i) t[i] = part[i];
         for (int w = kWaves / 2; w > 0; w /= 2)
-            for (int i = 0; i < w; ++i) q[i] = q[2 * i] + q[2 * i + 1];
-        costs[blockIdx.x] = -(q[0] / double(float(BINS * BINS)));
+            for (int i = 0; i < w; ++i) t[i] = t[2 * i] + t[2 * i + 1];
+        costs[blockIdx.x] = -(t[0] / double(float(BINS * BINS)));
     }
 }
 

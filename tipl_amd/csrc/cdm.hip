@@ -19,6 +19,7 @@
 // cdm_improved, smoothing schedule) runs on the host from one float cost
 // read back per iteration; everything else stays on the device.
 #include "device.hpp"
+#include "field.hpp"
 #include "internal.hpp"
 
 #include <algorithm>
@@ -27,100 +28,6 @@
 
 namespace tiplhip {
 namespace {
-
-struct FieldGeo {
-    int w, h, d;
-    uint32_t wh;
-    uint32_t n;
-    float wm1, hm1, dm1;
-};
-
-FieldGeo field_geo(const int* dim)
-{
-    FieldGeo g;
-    g.w = dim[0];
-    g.h = dim[1];
-    g.d = dim[2];
-    g.wh = uint32_t(dim[0]) * uint32_t(dim[1]);
-    g.n = g.wh * uint32_t(dim[2]);
-    g.wm1 = float(dim[0] - 1);
-    g.hm1 = float(dim[1] - 1);
-    g.dm1 = float(dim[2] - 1);
-    return g;
-}
-
-constexpr int kTX = 64, kTY = 4;
-
-dim3 tile_grid(const FieldGeo& g)
-{
-    return dim3(uint32_t((g.w + kTX - 1) / kTX), uint32_t((g.h + kTY - 1) / kTY), uint32_t(g.d));
-}
-
-struct Vox {
-    int x, y, z;
-    uint32_t p;
-    bool ok;
-};
-
-__device__ inline Vox this_voxel(const FieldGeo& g)
-{
-    Vox v;
-    v.x = int(blockIdx.x) * kTX + int(threadIdx.x);
-    v.y = int(blockIdx.y) * kTY + int(threadIdx.y);
-    v.z = int(blockIdx.z);
-    v.ok = v.x < g.w && v.y < g.h;
-    v.p = uint32_t(v.z) * g.wh + uint32_t(v.y) * uint32_t(g.w) + uint32_t(v.x);
-    return v;
-}
-
-__device__ inline float3 ld3(const float* f, uint32_t p)
-{
-    return make_float3(f[3 * size_t(p)], f[3 * size_t(p) + 1], f[3 * size_t(p) + 2]);
-}
-
-__device__ inline void st3(float* f, uint32_t p, float3 v)
-{
-    f[3 * size_t(p)] = v.x;
-    f[3 * size_t(p) + 1] = v.y;
-    f[3 * size_t(p) + 2] = v.z;
-}
-
-// Trilinear location as interpolator::linear<3>::get_location
-// (numerical/interpolation.hpp:337-383); false when outside.
-struct Tri {
-    uint32_t i0;
-    int ix, iy, iz;
-    float w[8];
-};
-
-__device__ inline bool locate(const FieldGeo& g, float x, float y, float z, Tri& t)
-{
-    // min(x,y,z) >= 0 is x,y,z >= 0 (a NaN coordinate fails its upper test)
-    const bool in = (fminf(fminf(x, y), z) >= 0.0f) & (x < g.wm1) & (y < g.hm1) & (z < g.dm1);
-    if (!in) return false;
-    const uint32_t ix = uint32_t(x), iy = uint32_t(y), iz = uint32_t(z);
-    const float p0 = x - float(ix), p1 = y - float(iy), p2 = z - float(iz);
-    const float n0 = 1.0f - p0, n1 = 1.0f - p1, n2 = 1.0f - p2;
-    t.i0 = iz * g.wh + iy * uint32_t(g.w) + ix;
-    t.ix = int(ix);
-    t.iy = int(iy);
-    t.iz = int(iz);
-    const float n0n1 = n0 * n1, p0n1 = p0 * n1, n0p1 = n0 * p1, p0p1 = p0 * p1;
-    t.w[0] = n0n1 * n2;
-    t.w[1] = p0n1 * n2;
-    t.w[2] = n0p1 * n2;
-    t.w[3] = p0p1 * n2;
-    t.w[4] = n0n1 * p2;
-    t.w[5] = p0n1 * p2;
-    t.w[6] = n0p1 * p2;
-    t.w[7] = p0p1 * p2;
-    return true;
-}
-
-__device__ inline uint32_t corner(const FieldGeo& g, uint32_t i0, int k)
-{
-    return i0 + uint32_t(k & 1) + ((k & 2) ? uint32_t(g.w) : 0u) + ((k & 4) ? g.wh : 0u);
-}
 
 // ---------------------------------------------------------------------------
 // compose_displacement: Js(x) = Is(x + d(x)); exact copy where d(x) == 0.
@@ -610,49 +517,85 @@ __global__ __launch_bounds__(kTX * kPTY) void poisson_zm_kernel(const float* __r
 // outside [0, n) is dropped by the reference; here it holds +0.0, and
 // r - (+0.0) == r for every float r, so the results are bit-identical.
 //
-// Per plane m the block loads s_t on its tile grown by 2 (S ring, 3 planes),
+// Per plane m the block holds s_t on its tile grown by 2 (S ring, 3 planes),
 // computes s_t+1 on the tile grown by 1 (T ring, 3 planes), then s_t+2 of
-// plane m-1 on the tile itself.  HBM traffic per two sweeps: new_d and s_t
-// read once, s_t+2 written once (36 B per voxel instead of 72).
+// plane m-1 on the tile itself.  The loop is software-pipelined: while it
+// works on plane m from LDS, each thread's global loads for plane m+2 of s_t
+// and for the new_d values of plane m+1 are in flight in registers.  Thread
+// (tx, ty) computes s_t+1 at its own voxel (tx, ty) and keeps that new_d
+// value for its s_t+2 output one plane later; threads 0..147 also compute
+// the 148 positions of the ring around the tile.  HBM traffic per two
+// sweeps: new_d and s_t read once, s_t+2 written once (36 B per voxel
+// instead of 72).  A workgroup covers zq planes; small volumes use a small
+// zq so that they still get enough workgroups.
 // ---------------------------------------------------------------------------
-constexpr int kQX = 64, kQY = 8, kQZ = 16;
+constexpr int kQX = 64, kQY = 8, kQZMax = 16;
 constexpr int kS2X = kQX + 4, kS2Y = kQY + 4;  // tile grown by 2
 constexpr int kS1X = kQX + 2, kS1Y = kQY + 2;  // tile grown by 1
+constexpr int kRing = kS1X * kS1Y - kQX * kQY;  // 148 positions around the tile
+constexpr int kS2N = kS2X * kS2Y;               // 816
 
-dim3 poisson2_grid(const FieldGeo& g)
+// planes per workgroup: about 512 workgroups or more, 2..16 planes each
+int poisson2_zq(const FieldGeo& g)
+{
+    const long long tiles = (long long)((g.w + kQX - 1) / kQX) * ((g.h + kQY - 1) / kQY);
+    const long long want = std::max<long long>(1, (tiles * g.d + 511) / 512);
+    return int(std::min<long long>(kQZMax, std::max<long long>(2, want)));
+}
+
+dim3 poisson2_grid(const FieldGeo& g, int zq)
 {
     return dim3(uint32_t((g.w + kQX - 1) / kQX), uint32_t((g.h + kQY - 1) / kQY),
-                uint32_t((g.d + kQZ - 1) / kQZ));
+                uint32_t((g.d + zq - 1) / zq));
+}
+
+// offset (i, j) of ring position r (-1 <= i <= kQX, -1 <= j <= kQY, not inside)
+__device__ inline void ring_offset(int r, int& i, int& j)
+{
+    if (r < kS1X) { i = r - 1; j = -1; }
+    else if (r < 2 * kS1X) { i = r - kS1X - 1; j = kQY; }
+    else if (r < 2 * kS1X + kQY) { i = -1; j = r - 2 * kS1X; }
+    else { i = kQX; j = r - 2 * kS1X - kQY; }
 }
 
 template <bool FIRST, bool MAXLEN>
 __global__ __launch_bounds__(kQX * kQY) void poisson2_kernel(const float* __restrict__ nd,
                                                              const float* __restrict__ a,
-                                                             FieldGeo g, float* __restrict__ b,
+                                                             FieldGeo g, int zq,
+                                                             float* __restrict__ b,
                                                              float* __restrict__ maxparts)
 {
     __shared__ float3 S[3][kS2Y][kS2X];
     __shared__ float3 T[3][kS1Y][kS1X];
     const int tx = int(threadIdx.x), ty = int(threadIdx.y), tid = ty * kQX + tx;
     const int x0 = int(blockIdx.x) * kQX, y0 = int(blockIdx.y) * kQY;
-    const int z0 = int(blockIdx.z) * kQZ, z1 = min(z0 + kQZ, g.d);
+    const int z0 = int(blockIdx.z) * zq, z1 = min(z0 + zq, g.d);
     const long long n = g.n, w = g.w, wh = g.wh;
     const long long base = (long long)y0 * w + x0;
-    // s_t at flat q, +0 outside the volume (FIRST: s_0 = new_d * (-1/6))
+    const float3 zero = make_float3(0.0f, 0.0f, 0.0f);
+
+    // this thread's S positions (two at most) and its ring position
+    const int e0 = tid, e1 = tid + kQX * kQY;
+    const long long so0 = (long long)(e0 / kS2X - 2) * w + (e0 % kS2X - 2);
+    const long long so1 = (long long)(e1 / kS2X - 2) * w + (e1 % kS2X - 2);
+    const bool has1 = e1 < kS2N;
+    int ri = 0, rj = 0;
+    const bool has_ring = tid < kRing;
+    if (has_ring) ring_offset(tid, ri, rj);
+    const long long ro = (long long)rj * w + ri;
+    const long long io = (long long)ty * w + tx;  // this thread's own voxel
+
     auto load_s = [&](long long q) -> float3 {
         if (q < 0 || q >= n) return make_float3(0.0f, 0.0f, 0.0f);
         const float3 t = ld3(FIRST ? nd : a, uint32_t(q));
         if (FIRST) return make_float3(t.x * kInvD2, t.y * kInvD2, t.z * kInvD2);
         return t;
     };
-    auto load_plane = [&](int m) {
-        const long long p0 = (long long)m * wh + base;
-        float3(*dst)[kS2X] = S[(m + 3) % 3];
-        for (int e = tid; e < kS2Y * kS2X; e += kQX * kQY) {
-            const int jj = e / kS2X, ii = e - jj * kS2X;
-            dst[jj][ii] = load_s(p0 + (long long)(jj - 2) * w + (ii - 2));
-        }
+    auto load_r = [&](long long q) -> float3 {
+        if (q < 0 || q >= n) return make_float3(0.0f, 0.0f, 0.0f);
+        return ld3(nd, uint32_t(q));
     };
+    auto valid = [&](long long q) { return q >= 0 && q < n; };
     // one Jacobi update in the reference's order from the six neighbours
     auto update = [](float3 r, float3 xm, float3 xp, float3 ym, float3 yp, float3 zm, float3 zp) {
         r.x = r.x - xm.x; r.y = r.y - xm.y; r.z = r.z - xm.z;
@@ -663,50 +606,79 @@ __global__ __launch_bounds__(kQX * kQY) void poisson2_kernel(const float* __rest
         r.x = r.x - zp.x; r.y = r.y - zp.y; r.z = r.z - zp.z;
         return make_float3(r.x * kInvD2, r.y * kInvD2, r.z * kInvD2);
     };
+    auto store_s = [&](int m, float3 v0, float3 v1) {
+        float3(*dst)[kS2X] = S[(m + 3) % 3];
+        dst[e0 / kS2X][e0 % kS2X] = v0;
+        if (has1) dst[e1 / kS2X][e1 % kS2X] = v1;
+    };
 
-    load_plane(z0 - 2);
-    load_plane(z0 - 1);
+    // prologue: S planes z0-2, z0-1 in LDS; S plane z0 and new_d of plane z0-1 in registers
+    {
+        const long long p = (long long)(z0 - 2) * wh + base;
+        store_s(z0 - 2, load_s(p + so0), has1 ? load_s(p + so1) : zero);
+        const long long p1 = p + wh;
+        store_s(z0 - 1, load_s(p1 + so0), has1 ? load_s(p1 + so1) : zero);
+    }
+    long long pz = (long long)z0 * wh + base;  // plane m+1 during iteration m
+    float3 s0 = load_s(pz + so0), s1 = has1 ? load_s(pz + so1) : zero;
+    const long long pm = pz - wh;              // plane z0-1
+    float3 r_in = load_r(pm + io), r_ring = has_ring ? load_r(pm + ro) : zero;
+    bool v_in = valid(pm + io), v_ring = has_ring && valid(pm + ro);
+    float3 r_prev = zero;  // new_d at this thread's voxel of plane m-1
     float len_max = 0.0f;
+
     for (int m = z0 - 1; m <= z1; ++m) {
-        load_plane(m + 1);
+        // S(m+1) from registers into the ring slot of plane m-2, then the next prefetch
+        store_s(m + 1, s0, s1);
+        const float3 c_in = r_in, c_ring = r_ring;
+        const bool cv_in = v_in, cv_ring = v_ring;
+        pz += wh;  // plane m+2
+        s0 = load_s(pz + so0);
+        s1 = has1 ? load_s(pz + so1) : zero;
+        const long long pn = pz - wh;  // plane m+1
+        r_in = load_r(pn + io);
+        r_ring = has_ring ? load_r(pn + ro) : zero;
+        v_in = valid(pn + io);
+        v_ring = has_ring && valid(pn + ro);
         __syncthreads();
-        // s_t+1 of plane m on the tile grown by 1
+        // s_t+1 of plane m: this thread's voxel and its ring position
         {
-            const long long p0 = (long long)m * wh + base;
             float3(*sm)[kS2X] = S[(m + 2) % 3];
             float3(*sc)[kS2X] = S[(m + 3) % 3];
             float3(*sp)[kS2X] = S[(m + 4) % 3];
             float3(*dst)[kS1X] = T[(m + 3) % 3];
-            for (int e = tid; e < kS1Y * kS1X; e += kQX * kQY) {
-                const int jj = e / kS1X, ii = e - jj * kS1X;  // offset (ii-1, jj-1)
-                const long long q = p0 + (long long)(jj - 1) * w + (ii - 1);
-                float3 v = make_float3(0.0f, 0.0f, 0.0f);
-                if (q >= 0 && q < n) {
-                    const int cj = jj + 1, ci = ii + 1;  // the same voxel in S coordinates
-                    v = update(ld3(nd, uint32_t(q)), sc[cj][ci - 1], sc[cj][ci + 1], sc[cj - 1][ci],
-                               sc[cj + 1][ci], sm[cj][ci], sp[cj][ci]);
-                }
-                dst[jj][ii] = v;
+            {
+                const int ci = tx + 2, cj = ty + 2;
+                dst[ty + 1][tx + 1] = cv_in ? update(c_in, sc[cj][ci - 1], sc[cj][ci + 1], sc[cj - 1][ci],
+                                                     sc[cj + 1][ci], sm[cj][ci], sp[cj][ci])
+                                            : zero;
+            }
+            if (has_ring) {
+                const int ci = ri + 2, cj = rj + 2;
+                dst[rj + 1][ri + 1] = cv_ring ? update(c_ring, sc[cj][ci - 1], sc[cj][ci + 1],
+                                                       sc[cj - 1][ci], sc[cj + 1][ci], sm[cj][ci],
+                                                       sp[cj][ci])
+                                              : zero;
             }
         }
         __syncthreads();
-        // s_t+2 of plane m-1 on the tile
+        // s_t+2 of plane m-1 at this thread's voxel
         const int z = m - 1;
         if (z >= z0 && z < z1 && x0 + tx < g.w && y0 + ty < g.h) {
-            const long long p = (long long)z * wh + base + (long long)ty * w + tx;
             float3(*tm)[kS1X] = T[(z + 2) % 3];
             float3(*tc)[kS1X] = T[(z + 3) % 3];
             float3(*tp)[kS1X] = T[(z + 4) % 3];
             const int cj = ty + 1, ci = tx + 1;
-            const float3 r = update(ld3(nd, uint32_t(p)), tc[cj][ci - 1], tc[cj][ci + 1],
-                                    tc[cj - 1][ci], tc[cj + 1][ci], tm[cj][ci], tp[cj][ci]);
-            st3(b, uint32_t(p), r);
+            const float3 r = update(r_prev, tc[cj][ci - 1], tc[cj][ci + 1], tc[cj - 1][ci],
+                                    tc[cj + 1][ci], tm[cj][ci], tp[cj][ci]);
+            st3(b, uint32_t((long long)z * wh + base + io), r);
             if (MAXLEN) {
                 float s2 = r.x * r.x + r.y * r.y;
                 s2 = s2 + r.z * r.z;
                 len_max = fmaxf(len_max, float(sqrt(double(s2))));
             }
         }
+        r_prev = c_in;
     }
     if (MAXLEN) {
         __shared__ float part[kQY];
@@ -830,17 +802,6 @@ __global__ __launch_bounds__(256) void upsample_kernel(const float* __restrict__
 // ---------------------------------------------------------------------------
 
 // Scratch memory of one CDM call, stream-ordered on the context's stream.
-struct DevMem {
-    tiplhip_ctx* ctx = nullptr;
-    void* p = nullptr;
-    DevMem(tiplhip_ctx* c, size_t bytes) : ctx(c), p(pool_alloc(c, bytes)) {}
-    DevMem(const DevMem&) = delete;
-    DevMem& operator=(const DevMem&) = delete;
-    DevMem(DevMem&& o) noexcept : ctx(o.ctx), p(o.p) { o.p = nullptr; }
-    ~DevMem() { pool_free(ctx, p); }
-    template <class T>
-    T* as() const { return static_cast<T*>(p); }
-};
 
 void zero(tiplhip_ctx* ctx, void* p, size_t bytes)
 {
@@ -885,10 +846,6 @@ void launch_gradient(tiplhip_ctx* ctx, const uint8_t* Js, const uint8_t* It, con
     TIPLHIP_CHECK(hipGetLastError());
 }
 
-// `sweeps` Jacobi sweeps of nd; returns the buffer holding the result (a or b).
-// With maxlen != nullptr the last sweep also forms per-block maxima of the
-// vector length in maxparts (block_count(g) floats) and reduces them into
-// *maxlen (float bits).
 // `sweeps` Jacobi sweeps, two per launch (poisson2_kernel) and a single
 // z-marching sweep for an odd remainder; returns the buffer holding the
 // result.  With maxlen, the last launch also forms per-block maxima of the
@@ -905,19 +862,20 @@ float* launch_poisson(tiplhip_ctx* ctx, const float* nd, float* a, float* b, con
         const bool pair = sweeps - done >= 2;
         const bool last = done + (pair ? 2 : 1) == sweeps && maxlen;
         if (pair) {
-            const dim3 grid = poisson2_grid(g), block(kQX, kQY);
+            const int zq = poisson2_zq(g);
+            const dim3 grid = poisson2_grid(g, zq), block(kQX, kQY);
             if (first && last)
                 hipLaunchKernelGGL((poisson2_kernel<true, true>), grid, block, 0, ctx->stream, nd, src,
-                                   g, dst, maxparts);
+                                   g, zq, dst, maxparts);
             else if (first)
                 hipLaunchKernelGGL((poisson2_kernel<true, false>), grid, block, 0, ctx->stream, nd,
-                                   src, g, dst, maxparts);
+                                   src, g, zq, dst, maxparts);
             else if (last)
                 hipLaunchKernelGGL((poisson2_kernel<false, true>), grid, block, 0, ctx->stream, nd,
-                                   src, g, dst, maxparts);
+                                   src, g, zq, dst, maxparts);
             else
                 hipLaunchKernelGGL((poisson2_kernel<false, false>), grid, block, 0, ctx->stream, nd,
-                                   src, g, dst, maxparts);
+                                   src, g, zq, dst, maxparts);
             last_blocks = grid.x * grid.y * grid.z;
             done += 2;
         } else {
@@ -1188,12 +1146,6 @@ std::vector<const uint8_t*> volume_ptrs(tiplhip_ctx* ctx, const tiplhip_volume* 
         out.push_back(v[i]->d);
     }
     return out;
-}
-
-void check_dim(const int* dim)
-{
-    TIPLHIP_REQUIRE(dim && dim[0] > 0 && dim[1] > 0 && dim[2] > 0, "dims must be positive");
-    TIPLHIP_REQUIRE(vox(dim) < (size_t(1) << 31), "volume exceeds 2^31 voxels");
 }
 
 }  // namespace
