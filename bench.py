@@ -304,6 +304,34 @@ def cdm_config2(ctx):
             "image_correlation_after": round(corr_after, 4)}
 
 
+def mm_reg_pipeline(ctx):
+    """mm_reg's nonlinear half on the 160x192x160 warped pair with a given
+    (identity) affine: J and r_linear, It_match_fov, cdm in both directions
+    (fixed schedule, 3 levels x 48 iterations, sigma-2 field smoothing), both
+    mappings, the warped subject and r_nonlinear.  Timed with the reverse
+    direction off and on (it runs on a second stream at the same time); the
+    host copies of the requested outputs are inside the timing."""
+    import tipl_amd
+    from tipl_amd import phantoms
+    fixed, moving, _ = phantoms.warped_pair(2000, (160, 192, 160))
+    ident = np.array([0, 0, 0, 0, 0, 0, 1, 1, 1, 0, 0, 0], np.float32)
+    kw = dict(nonlinear=True, arg=ident, levels=3, level_iterations=48, field_bandwidth=8)
+    out = {"shape": [160, 192, 160], "levels": 3, "iterations_per_level": 48}
+    for reverse in (False, True):
+        outs = ("to2from", "warped") + (("f2t", "from2to") if reverse else ())
+        tipl_amd.register(ctx, fixed, moving, reverse=reverse, outputs=outs, **kw)  # warm-up
+        t0 = time.perf_counter()
+        r = tipl_amd.register(ctx, fixed, moving, reverse=reverse, outputs=outs, **kw)
+        dt = time.perf_counter() - t0
+        key = "bidirectional" if reverse else "forward_only"
+        out[key + "_s"] = round(dt, 4)
+        out[key + "_nonlinear_ms"] = round(r["cdm_ms"], 2)
+        out["r_linear"] = round(r["r_linear"], 5)
+        out["r_nonlinear"] = round(r["r_nonlinear"], 5)
+    out["reverse_overlap"] = round(2 * out["forward_only_nonlinear_ms"] / out["bidirectional_nonlinear_ms"], 3)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -416,6 +444,9 @@ def main():
     log("CDM fixed schedule at 160x192x160")
     cdm2 = cdm_config2(ctx)
     log(f"cdm config 2 {cdm2}")
+    log("mm_reg nonlinear half at 160x192x160")
+    mmreg = mm_reg_pipeline(ctx)
+    log(f"mm_reg pipeline {mmreg}")
 
     serve = None
     if reqs:
@@ -459,6 +490,7 @@ def main():
         "corr_cpu_baseline": cpu_corr,
         "hbm_copy_gbps": round(copy_gbps, 1),
         "cdm_fixed_schedule": cdm2,
+        "mm_reg_pipeline": mmreg,
         "multi_pair": mp_res,
         "serving": serve,
         **{k: round(v, 4) for k, v in res.items()},

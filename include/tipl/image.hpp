@@ -74,6 +74,9 @@ public:
     double length() const { return std::sqrt(double(v[0] * v[0] + v[1] * v[1] + v[2] * v[2])); }
 };
 
+// a displacement field is handed to the library as a packed float array
+static_assert(sizeof(vector<3, float>) == 3 * sizeof(float), "vector<3> must be three packed floats");
+
 template <int dim, typename T>
 class image;
 

@@ -444,6 +444,34 @@ float linear_impl(const std::vector<const_pointer_image<3, unsigned char>>& from
 
 }  // namespace detail
 
+// make_list (reg/linear.hpp:566-592): image lists as const views; a vector
+// of images stops at its first empty entry.
+template <typename image_type>
+auto make_list(const image_type& I, const image_type& I2)
+{
+    auto pI = tipl::make_shared(I);
+    if (I2.empty()) return std::vector<decltype(pI)>({pI});
+    return std::vector<decltype(pI)>({pI, tipl::make_shared(I2)});
+}
+
+template <typename image_type>
+auto make_list(const image_type& I)
+{
+    auto pI = tipl::make_shared(I);
+    return std::vector<decltype(pI)>({pI});
+}
+
+template <typename T>
+auto make_list(const std::vector<T>& data)
+{
+    std::vector<tipl::const_pointer_image<T::dimension, typename T::value_type>> out;
+    for (const auto& each : data) {
+        if (each.empty()) break;
+        out.push_back(tipl::make_shared(each));
+    }
+    return out;
+}
+
 // tipl::reg::linear (reg/linear.hpp:622-700): registers `from` onto `to`
 // (lists of modalities, uint8), arg in/out; returns the final cost.
 template <typename out_type = void, int dim>

@@ -285,11 +285,68 @@ int tiplhip_volume_resample(tiplhip_ctx* ctx, const tiplhip_volume* src, const i
                             const float T[12], int crop_shortcut, tiplhip_volume** out);
 
 /* ---------------------------------------------------------------------------
- * Registration of one image pair: the reference's mm_reg order
- * (reg/mm_reg.hpp:348-416) -- linear(fixed, moving) and, when nonlinear is
- * set, the moving image resampled onto the fixed grid and cdm(fixed, J).
- * fixed plays `from` / It, moving plays `to` / Is.  Inputs of type int16 or
- * float32 are quantised on the device (normalize_upper_lower2, upper 255.99).
+ * Post-registration warping (numerical/dif.hpp) and mm_reg's r.  Fields and
+ * mappings are host arrays of vox(dim) packed float3 (tipl::image<3,
+ * tipl::vector<3>>), x fastest.  A transform T is 12 floats: sr[9] row-major,
+ * then shift[3] (tipl::transformation_matrix).  All results are bit-exact.
+ * ------------------------------------------------------------------------- */
+
+/* displacement_to_mapping (numerical/dif.hpp:9-28; with T :41-50):
+ * mapping(x) = T(x + dis(x)); T == NULL is the identity.  mapping may equal dis. */
+int tiplhip_displacement_to_mapping(tiplhip_ctx* ctx, const float* dis, const int dim[3],
+                                    const float* T, float* mapping);
+
+/* mapping_to_displacement (numerical/dif.hpp:30-38): dis(x) = mapping(x) - x. */
+int tiplhip_mapping_to_displacement(tiplhip_ctx* ctx, const float* mapping, const int dim[3],
+                                    float* dis);
+
+/* inv_displacement_to_mapping<linear> (numerical/dif.hpp:52-64): over a grid
+ * of out_dim, mapping(x) = p + inv_dis(p) with p = T^-1(x); inv_dis (dis_dim)
+ * is sampled trilinearly and counts as zero outside its grid. */
+int tiplhip_inv_displacement_to_mapping(tiplhip_ctx* ctx, const float* inv_dis,
+                                        const int dis_dim[3], const float T[12],
+                                        const int out_dim[3], float* mapping);
+
+/* compose_mapping<linear> (numerical/dif.hpp:66-86): out(x) = from(mapping(x))
+ * over mapping's grid dim, trilinear, 0 outside `from`.  dtype TIPLHIP_U8
+ * (interp_type<uint8> truncation) or TIPLHIP_F32; out has vox(dim) voxels. */
+int tiplhip_compose_mapping(tiplhip_ctx* ctx, const void* from, int dtype, const int from_dim[3],
+                            const float* mapping, const int dim[3], void* out);
+
+/* jacobian_determinant_dis<3> (numerical/dif.hpp:399-465): per voxel of a
+ * displacement field, in the reference's mixed float/double arithmetic,
+ * stored as float; 1 on the volume's edge voxels. */
+int tiplhip_jacobian_determinant_dis(tiplhip_ctx* ctx, const float* dis, const int dim[3],
+                                     float* out);
+
+/* jacobian_determinant (numerical/dif.hpp:356-397) of a mapping, float; 0 on
+ * edge voxels (the value-returning overload's fresh image). */
+int tiplhip_jacobian_determinant(tiplhip_ctx* ctx, const float* mapping, const int dim[3],
+                                 float* out);
+
+/* mm_reg::It_match_fov (reg/mm_reg.hpp:313-328) in place on a device volume:
+ * voxels x with T(x) outside [0, moving_dim) are set to 0. */
+int tiplhip_match_fov(tiplhip_ctx* ctx, tiplhip_volume* It, const float T[12],
+                      const int moving_dim[3]);
+
+/* r of two device volumes of one shape (reg/mm_reg.hpp:302-305): with masked,
+ * correlation_ygz (only voxels where y > 0, numerical/statistics.hpp:599-619),
+ * else tipl::correlation (:569-597); exact integer sums, FP64 tail. */
+int tiplhip_correlation(tiplhip_ctx* ctx, const tiplhip_volume* x, const tiplhip_volume* y,
+                        int masked, double* r);
+
+/* ---------------------------------------------------------------------------
+ * Registration of one image pair in the reference's mm_reg order
+ * (reg/mm_reg.hpp:285-416), every step on the device:
+ *   linear_reg: linear(fixed, moving) (or the given arg with skip_linear);
+ *   calculate_linear_r: J = resample(moving) onto the fixed grid, r_linear;
+ *   nonlinear requests: It_match_fov on the fixed image (match_fov), then
+ *   cdm(fixed, J) -> t2f and, with reverse, cdm(J, fixed) -> f2t on a
+ *   second stream at the same time; to2from = displacement_to_mapping(t2f, T),
+ *   from2to = inv_displacement_to_mapping(f2t, T); calculate_nonlinear_r:
+ *   J = compose_mapping(moving, to2from), r_nonlinear.
+ * fixed plays `from` / It (template), moving plays `to` / I (subject).  Inputs
+ * of type int16 or float32 are quantised on the device (preprocess).
  * ------------------------------------------------------------------------- */
 typedef struct tiplhip_reg_request {
     const void* fixed;        /* host voxels, x fastest */
@@ -309,6 +366,18 @@ typedef struct tiplhip_reg_request {
                                  linear cost functor's rule (reg/linear.hpp:65-71);
                                  1: mm_reg's rule -- fixed by template_image_pre,
                                  moving by subject_image_pre (reg/mm_reg.hpp:14-34) */
+    int reverse;              /* nonlinear: also cdm(J, fixed) -> f2t, concurrently
+                                 (mm_reg's skip_reverse = false); default 0 */
+    int match_fov;            /* nonlinear: It_match_fov before cdm; default 1 (mm_reg) */
+    int masked_r;             /* r over fixed > 0 (correlation_ygz); default 1 (mm_reg) */
+    int skip_linear;          /* 1: no linear(); `arg` is used as given */
+    float arg[12];            /* affine_param for skip_linear: t, r, s, a */
+    float* f2t;               /* host vox(fixed) x 3: reverse field (needs reverse), or NULL */
+    float* to2from;           /* host vox(fixed) x 3: fixed voxel -> moving position, or NULL */
+    float* from2to;           /* host vox(moving) x 3: moving voxel -> fixed position
+                                 (needs reverse), or NULL */
+    uint8_t* warped;          /* host vox(fixed): the last J -- moving warped onto the fixed
+                                 grid (affine or nonlinear), or NULL */
 } tiplhip_reg_request;
 
 typedef struct tiplhip_reg_result {
@@ -322,7 +391,9 @@ typedef struct tiplhip_reg_result {
     double run_ms;            /* start to done */
     double prep_ms;           /* upload, quantisation, host mask / moment estimate */
     double affine_ms;         /* the linear() optimisation */
-    double cdm_ms;            /* resample and cdm (nonlinear requests) */
+    double cdm_ms;            /* match_fov, cdm (both directions), mappings, r_nonlinear */
+    float r_linear;           /* mm_reg's r after the linear step (reg/mm_reg.hpp:302-305) */
+    float r_nonlinear;        /* mm_reg's r after the nonlinear step (:341-343); 0 if none */
 } tiplhip_reg_result;
 
 void tiplhip_reg_request_default(tiplhip_reg_request* r);

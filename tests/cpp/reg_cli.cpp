@@ -10,6 +10,12 @@
 //       tipl::filter::gaussian on a float image (1) or vector<3> field (3).
 //   reg_cli subject <in.f32> <w> <h> <d> <out.u8>
 //       tipl::reg::subject_image_pre.
+//   reg_cli mmreg <It.u8> <tw> <th> <td> <I.u8> <sw> <sh> <sd> <param.f32> <out>
+//       tipl::reg::mm_reg with skip_linear: param holds arg[12], Itvs[3],
+//       Ivs[3], then levels, level_iterations, smoothing, field_bandwidth of
+//       the fixed-schedule cdm.  Writes <out>.r (r after the linear and the
+//       nonlinear step), .t2f, .f2t, .to2from, .from2to (float3 fields),
+//       .JJ (the warped subject) and .jac (jacobian_determinant_dis of t2f).
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -72,6 +78,47 @@ int main(int argc, char** argv)
             tipl::image<3, float> I(v.data(), s);
             auto J = tipl::reg::subject_image_pre(I);
             write_raw(argv[6], J.data(), J.size());
+            return 0;
+        }
+        if (argc >= 12 && std::string(argv[1]) == "mmreg") {
+            const tipl::shape<3> ts(unsigned(std::atoi(argv[3])), unsigned(std::atoi(argv[4])),
+                                    unsigned(std::atoi(argv[5])));
+            const tipl::shape<3> ss(unsigned(std::atoi(argv[7])), unsigned(std::atoi(argv[8])),
+                                    unsigned(std::atoi(argv[9])));
+            auto It = read_u8(argv[2], ts.size());
+            auto I = read_u8(argv[6], ss.size());
+            auto p = read_raw<float>(argv[10], 22);
+            tipl::reg::mm_reg<void> m;
+            m.It[0] = tipl::image<3, unsigned char>(It.data(), ts);
+            m.I[0] = tipl::image<3, unsigned char>(I.data(), ss);
+            m.Its = ts;
+            m.Is = ss;
+            m.arg = tipl::affine_param<float, 3>(p.data());
+            m.Itvs = tipl::vector<3>(p.data() + 12);
+            m.Ivs = tipl::vector<3>(p.data() + 15);
+            m.param.levels = int(p[18]);
+            m.param.level_iterations = int(p[19]);
+            m.param.smoothing = p[20];
+            m.param.field_bandwidth = int(p[21]);
+            m.skip_linear = true;
+            bool terminated = false;
+            m.linear_reg(terminated);
+            float r[2] = {m.r[0], 0.0f};
+            m.nonlinear_reg(terminated);
+            r[1] = m.r[0];
+            const std::string out = argv[11];
+            write_raw((out + ".r").c_str(), r, 2);
+            auto field = [&](const char* ext, const tipl::image<3, tipl::vector<3>>& f) {
+                write_raw((out + ext).c_str(), reinterpret_cast<const float*>(f.data()), 3 * f.size());
+            };
+            field(".t2f", m.t2f_dis);
+            field(".f2t", m.f2t_dis);
+            field(".to2from", m.to2from);
+            field(".from2to", m.from2to);
+            write_raw((out + ".JJ").c_str(), m.J[0].data(), m.J[0].size());
+            tipl::image<3, float> jac;
+            tipl::jacobian_determinant_dis(m.t2f_dis, jac);
+            write_raw((out + ".jac").c_str(), jac.data(), jac.size());
             return 0;
         }
         if (argc < 8) {

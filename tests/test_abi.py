@@ -74,3 +74,42 @@ def test_cdm_param_defaults():
         (2.0, np.float32(0.3), np.float32(0.05), 16, 200)
     assert p.gradient_type == b"rrrrrrrr"
     assert (p.levels, p.level_iterations, p.field_bandwidth) == (0, 0, 0)
+
+
+def test_reg_request_defaults():
+    """mm_reg's defaults (reg/mm_reg.hpp:53-67): match_fov and masked_r on,
+    the reverse direction off unless asked, identity affine for skip_linear."""
+    import tipl_amd
+    r = tipl_amd.RegRequest()
+    tipl_amd.load_library().tiplhip_reg_request_default(ctypes.byref(r))
+    assert (r.match_fov, r.masked_r, r.reverse, r.skip_linear) == (1, 1, 0, 0)
+    assert list(r.arg) == [0.0] * 6 + [1.0] * 3 + [0.0] * 3
+    assert not any([r.f2t, r.to2from, r.from2to, r.warped, r.field])
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """The ctypes mirrors have the C header's sizes and field offsets."""
+    import subprocess
+    import tipl_amd
+    structs = {"tiplhip_reg_request": tipl_amd.RegRequest, "tiplhip_reg_result": tipl_amd.RegResult,
+               "tiplhip_cdm_param": tipl_amd.CdmParam, "tiplhip_server_stats": tipl_amd.ServerStats,
+               "tiplhip_pair_costs": tipl_amd.PairCosts}
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "tiplhip.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            cfield = fname.rstrip("_")  # `from_` mirrors the C member `from`
+            lines.append(f'printf("{cname}.{fname} %zu\\n", offsetof({cname}, {cfield}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(root, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                       check=True).stdout.splitlines())
+    for cname, py in structs.items():
+        assert int(got[cname]) == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert int(got[f"{cname}.{fname}"]) == getattr(py, fname).offset, (cname, fname)

@@ -98,3 +98,38 @@ def test_subject_pre_cli_matches_oracle(tmp_path, oracle):
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert np.array_equal(np.fromfile(out, np.uint8).reshape(f.shape), oracle.subject_pre(f))
+
+
+def test_mm_reg_cli_matches_oracle(tmp_path, oracle):
+    """tipl::reg::mm_reg (the C++ mirror) with a given affine: linear_reg ->
+    calculate_linear_r and It_match_fov, nonlinear_reg with the reverse cdm
+    on its own thread, mappings, calculate_nonlinear_r; every output equals
+    the oracle's composition of the same steps bit for bit (fixed-schedule
+    cdm, so the whole run is bit-exact)."""
+    from scipy.ndimage import shift
+    from conftest import smooth_volume
+    from mmreg_oracle import oracle_mm_reg
+    It = smooth_volume((30, 34, 28), 3, sigma=2.5)
+    I = shift(It.astype(np.float32), (1.0, -1.5, 2.0), order=1).astype(np.uint8)[1:, :, 2:]
+    arg = np.array([0.5, -0.3, 0.2, 0.02, -0.01, 0.015, 1.0, 1.0, 1.0, 0, 0, 0], np.float32)
+    vt, vs = (1.0, 1.0, 1.0), (1.1, 1.0, 0.9)
+    param = np.concatenate([arg, vt, vs, [2, 4, 0.5, 2]]).astype(np.float32)
+    a, b, p, out = tmp_path / "It.u8", tmp_path / "I.u8", tmp_path / "p.f32", tmp_path / "out"
+    It.tofile(a)
+    I.tofile(b)
+    param.tofile(p)
+    tz, ty, tx = It.shape
+    sz, sy, sx = I.shape
+    r = subprocess.run([CLI, "mmreg", str(a), str(tx), str(ty), str(tz), str(b), str(sx), str(sy),
+                        str(sz), str(p), str(out)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    want = oracle_mm_reg(oracle, It, vt, I, vs, arg, cdm_fixed=(2, 4, 0.5, 2))
+    rr = np.fromfile(str(out) + ".r", np.float32)
+    assert rr[0] == want["r"][0] and rr[1] == want["r"][1]
+    f3 = lambda ext, shp: np.fromfile(str(out) + ext, np.float32).reshape(shp + (3,))
+    for ext, key, shp in ((".t2f", "t2f", It.shape), (".f2t", "f2t", It.shape),
+                          (".to2from", "to2from", It.shape), (".from2to", "from2to", I.shape)):
+        assert np.array_equal(f3(ext, shp).view(np.uint32), want[key].view(np.uint32)), key
+    assert np.array_equal(np.fromfile(str(out) + ".JJ", np.uint8).reshape(It.shape), want["JJ"])
+    jac = np.fromfile(str(out) + ".jac", np.float32).reshape(It.shape)
+    assert np.array_equal(jac.view(np.uint32), oracle.jacobian_determinant_dis(want["t2f"]).view(np.uint32))

@@ -51,8 +51,13 @@ def _same(a, b):
     assert np.array_equal(a["T"].view(np.uint32), b["T"].view(np.uint32))
     assert a["cost"] == b["cost"] and a["evaluations"] == b["evaluations"]
     assert a["cdm_iterations"] == b["cdm_iterations"]
-    if a["field"] is not None or b["field"] is not None:
-        assert np.array_equal(a["field"].view(np.uint32), b["field"].view(np.uint32))
+    assert a["r_linear"] == b["r_linear"] and a["r_nonlinear"] == b["r_nonlinear"]
+    for k in ("field", "f2t", "to2from", "from2to", "warped"):
+        if a.get(k) is not None or b.get(k) is not None:
+            x, y = np.asarray(a[k]), np.asarray(b[k])
+            if x.dtype == np.float32:
+                x, y = x.view(np.uint32), y.view(np.uint32)
+            assert np.array_equal(x, y), k
 
 
 @pytest.fixture(scope="module")
@@ -112,9 +117,13 @@ def test_plain_path_nonlinear(mix, plain):
               f"{res['evaluations']} evaluations, {res['cdm_iterations']} CDM iterations, "
               f"{res['run_ms']:.1f} ms")
         if r["nonlinear"]:
+            # cdm() keeps its best field by cost: on a pair the affine step
+            # already aligns well (r ~ 0.98) that can be the zero field, as
+            # the oracle's cdm() gives for request 4 with the reference's
+            # own linear() result
             assert res["cdm_iterations"] > 0
-            f = res["field"]
-            assert np.isfinite(f).all() and np.abs(f).max() > 0.1
+            assert np.isfinite(res["field"]).all()
+            assert res["r_nonlinear"] >= res["r_linear"] - 0.01
 
 
 @pytest.mark.parametrize("slots,order", [(4, "forward"), (8, "reverse")])
@@ -184,6 +193,27 @@ def test_mm_reg_preprocessing_served_equals_plain(ctx):
              for i, (f, m, _) in enumerate(reqs)]
     with tipl_amd.Server(0, slots=3, hbm_budget=8 << 30) as srv:
         ts = [srv.submit(f, m, nonlinear=(i == 1), preprocess="mm_reg") for i, (f, m, _) in enumerate(reqs)]
+        got = [srv.wait(t) for t in ts]
+    for a, b in zip(got, plain):
+        _same(a, b)
+
+
+def test_bidirectional_served_equals_plain(ctx, mix):
+    """mm_reg's full nonlinear half (reverse CDM on each slot's second stream,
+    both mappings, the warped image and the r values): served == plain."""
+    import tipl_amd
+    outs = ("f2t", "to2from", "from2to", "warped")
+    sel = [i for i, r in enumerate(mix) if r["nonlinear"]] + [0]
+    kw = lambda i: dict(cost=mix[i]["cost"], nonlinear=mix[i]["nonlinear"],
+                        reverse=mix[i]["nonlinear"],
+                        outputs=outs if mix[i]["nonlinear"] else ("warped",))
+    plain = [tipl_amd.register(ctx, mix[i]["fixed"], mix[i]["moving"], **kw(i)) for i in sel]
+    for i, p in zip(sel, plain):
+        if mix[i]["nonlinear"]:
+            assert p["r_nonlinear"] >= p["r_linear"] - 0.01 and p["r_linear"] > 0.5
+            assert np.isfinite(p["f2t"]).all() and np.isfinite(p["from2to"]).all()
+    with tipl_amd.Server(0, slots=4, hbm_budget=16 << 30) as srv:
+        ts = [srv.submit(mix[i]["fixed"], mix[i]["moving"], **kw(i)) for i in sel]
         got = [srv.wait(t) for t in ts]
     for a, b in zip(got, plain):
         _same(a, b)

@@ -197,3 +197,84 @@ def test_gaussian_closed_forms(oracle):
     ramp = np.broadcast_to(np.arange(9, dtype=np.float32), (3, 4, 9)).copy()
     r = oracle.gaussian(ramp, 1)[1, 1]
     assert np.allclose(r[1:-1], np.arange(1, 8)) and r[0] == 0.25 and r[-1] == 7.75
+
+
+def _linear_field(shape, A):
+    """d(x) = A x over the (z, y, x) grid, x = (i, j, k) voxel coordinates."""
+    z, y, x = np.meshgrid(*[np.arange(n, dtype=np.float64) for n in shape], indexing="ij")
+    X = np.stack([x, y, z], -1)
+    return (X @ np.asarray(A, np.float64).T).astype(np.float32)
+
+
+def test_jacobian_closed_forms(oracle):
+    """jacobian_determinant_dis of d(x) = A x is det(I + 2A) (central
+    differences not halved) with 1 on the edges; jacobian_determinant of a
+    mapping m(x) = B x is 8 det(B) with the reference's 0.125 applied to the
+    third cofactor term only.  A and B are multiples of 1/8, so every float
+    and double step is exact."""
+    A = np.array([[0.125, -0.25, 0.0], [0.375, 0.0, 0.125], [-0.125, 0.25, 0.5]])
+    shape = (6, 7, 8)
+    J = oracle.jacobian_determinant_dis(_linear_field(shape, A))
+    inner = J[1:-1, 1:-1, 1:-1]
+    assert np.all(inner == np.float32(np.linalg.det(np.eye(3) + 2 * A)))
+    edge = np.ones(shape, bool)
+    edge[1:-1, 1:-1, 1:-1] = False
+    assert np.all(J[edge] == 1.0)
+    B = np.array([[1.0, 0.5, 0.25], [0.125, 1.25, -0.5], [0.75, 0.25, 2.0]])
+    Jm = oracle.jacobian_determinant(_linear_field(shape, B))
+    full = 8 * (B[0, 0] * (B[1, 1] * B[2, 2] - B[2, 1] * B[1, 2])
+                + B[1, 0] * (B[2, 1] * B[0, 2] - B[0, 1] * B[2, 2]))
+    quirk = 8 * B[2, 0] * (B[0, 1] * B[1, 2] - B[1, 1] * B[0, 2]) * 0.125
+    assert np.all(Jm[1:-1, 1:-1, 1:-1] == np.float32(full + quirk))
+    assert np.all(Jm[edge] == 0.0)
+    assert np.float32(full + quirk) != np.float32(8 * np.linalg.det(B))
+
+
+def test_mapping_identities(oracle):
+    """A zero field maps to the voxel grid; T is applied after adding the
+    index; an integer-shift T with a zero inverse field inverts to x - shift;
+    compose_mapping through the identity mapping copies the image except at
+    the far edges (where the trilinear cell is incomplete)."""
+    shape = (5, 6, 7)
+    zero = np.zeros(shape + (3,), np.float32)
+    grid = oracle.displacement_to_mapping(zero)
+    z, y, x = np.meshgrid(*[np.arange(n, dtype=np.float32) for n in shape], indexing="ij")
+    assert np.array_equal(grid, np.stack([x, y, z], -1))
+    d = np.full(shape + (3,), 0.5, np.float32)
+    m = oracle.displacement_to_mapping(d, shift_T(2, -1, 3))
+    assert np.array_equal(m, np.stack([x + 2.5, y - 0.5, z + 3.5], -1))
+    assert np.array_equal(oracle.mapping_to_displacement(m), np.stack([x * 0 + 2.5, y * 0 - 0.5,
+                                                                         z * 0 + 3.5], -1))
+    inv = oracle.inv_displacement_to_mapping(zero, shift_T(2, -1, 3), (4, 5, 6))
+    z2, y2, x2 = np.meshgrid(*[np.arange(n, dtype=np.float32) for n in (4, 5, 6)], indexing="ij")
+    assert np.array_equal(inv, np.stack([x2 - 2, y2 + 1, z2 - 3], -1))
+    img = (np.arange(np.prod(shape)) % 251).astype(np.uint8).reshape(shape)
+    out = oracle.compose_mapping(img, grid)
+    assert np.array_equal(out[:-1, :-1, :-1], img[:-1, :-1, :-1])
+    assert not out[-1].any() and not out[:, -1].any() and not out[:, :, -1].any()
+
+
+def test_masked_correlation_closed_forms(oracle):
+    """r = 1 for an exact increasing affine relation on the masked voxels even
+    when the unmasked voxels break it; r = -1 for a decreasing one; the mask
+    is y > 0 only."""
+    rng = np.random.default_rng(2)
+    y = rng.integers(1, 100, 500).astype(np.uint8)
+    y[::7] = 0
+    x = (2 * y.astype(np.int32) + 3).astype(np.uint8)
+    x[y == 0] = rng.integers(0, 255, int((y == 0).sum()))
+    assert abs(oracle.correlation(x, y, True) - 1.0) < 1e-12
+    assert oracle.correlation(x, y, False) < 0.99
+    xd = (250 - 2 * y.astype(np.int32)).astype(np.uint8)
+    assert abs(oracle.correlation(xd, y, True) + 1.0) < 1e-12
+    ref_r = np.corrcoef(x[y > 0].astype(np.float64), y[y > 0].astype(np.float64))[0, 1]
+    assert abs(oracle.correlation(x, y, True) - ref_r) < 1e-12
+
+
+def test_match_fov_shift(oracle):
+    """An integer shift of +2 along x moves the last two template columns
+    outside a subject grid of the same shape; nothing else is zeroed."""
+    It = np.full((4, 5, 6), 9, np.uint8)
+    out = oracle.match_fov(It, shift_T(2, 0, 0), (4, 5, 6))
+    assert not out[:, :, 4:].any() and np.all(out[:, :, :4] == 9)
+    assert np.array_equal(oracle.match_fov(It, I12, (4, 5, 6)), It)

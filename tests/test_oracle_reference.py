@@ -158,3 +158,82 @@ def test_cdm_fixed_schedule(oracle, ref, levels, iters, smoothing, bandwidth):
     fr = ref.cdm_fixed(It, Is, levels, iters, smoothing, bandwidth)
     assert np.array_equal(_bits(fo), _bits(fr))
     assert n == levels * iters
+
+
+def _wfield(shape, seed, scale=3.0):
+    from scipy.ndimage import gaussian_filter
+    r = np.random.default_rng(seed)
+    return gaussian_filter(r.standard_normal(shape + (3,)), (1, 1, 1, 0)).astype(np.float32) * scale
+
+
+WARP_SHAPES = [(9, 11, 13), (20, 17, 15), (3, 4, 5), (1, 6, 7)]
+
+
+@pytest.mark.parametrize("shape", WARP_SHAPES)
+def test_warping(oracle, ref, shape):
+    """displacement_to_mapping (with and without T), mapping_to_displacement,
+    inv_displacement_to_mapping, compose_mapping (uint8 and float) and both
+    Jacobian determinants (numerical/dif.hpp:9-86, 356-465)."""
+    rng = np.random.default_rng(sum(shape))
+    d = _wfield(shape, 1)
+    T = oracle.affine_matrix([1.5, -2, 0.7, 0.1, -0.05, 0.08, 1.05, 0.97, 1.02, 0.02, -0.01, 0.03],
+                             np.array(shape[::-1], np.int32), [1, 1, 1],
+                             np.array(shape[::-1], np.int32), [1.2, 1, 1])
+    for TT in (None, T):
+        assert np.array_equal(_bits(oracle.displacement_to_mapping(d, TT)),
+                              _bits(ref.displacement_to_mapping(d, TT)))
+    m = oracle.displacement_to_mapping(d, T)
+    assert np.array_equal(_bits(oracle.mapping_to_displacement(m)), _bits(ref.mapping_to_displacement(m)))
+    out_shape = (shape[0] + 2, max(1, shape[1] - 1), shape[2] + 3)
+    assert np.array_equal(_bits(oracle.inv_displacement_to_mapping(d, T, out_shape)),
+                          _bits(ref.inv_displacement_to_mapping(d, T, out_shape)))
+    mm = m * np.float32(0.9)
+    I8 = (rng.random(shape) * 255).astype(np.uint8)
+    assert np.array_equal(oracle.compose_mapping(I8, mm), ref.compose_mapping(I8, mm))
+    If = rng.standard_normal(shape).astype(np.float32)
+    assert np.array_equal(_bits(oracle.compose_mapping(If, mm)), _bits(ref.compose_mapping(If, mm)))
+    assert np.array_equal(_bits(oracle.jacobian_determinant_dis(d)), _bits(ref.jacobian_determinant_dis(d)))
+    assert np.array_equal(_bits(oracle.jacobian_determinant(m)), _bits(ref.jacobian_determinant(m)))
+
+
+def test_correlation_r(oracle, ref):
+    """mm_reg's r: correlation_ygz and correlation (numerical/statistics.hpp:569-619)."""
+    rng = np.random.default_rng(3)
+    for shape in [(20, 17, 15), (1, 1, 7), (5, 5, 5)]:
+        y = (rng.random(shape) * 255).astype(np.uint8)
+        y[rng.random(shape) < 0.3] = 0
+        x = (rng.random(shape) * 255).astype(np.uint8)
+        for masked in (0, 1):
+            assert oracle.correlation(x, y, masked) == ref.correlation(x, y, masked)
+    z = np.zeros((4, 4, 4), np.uint8)
+    assert oracle.correlation(z, z, 1) == ref.correlation(z, z, 1) == 0.0
+
+
+@pytest.mark.parametrize("tshape,mshape", [((20, 22, 18), (24, 20, 26)), ((9, 10, 11), (5, 6, 7))])
+def test_match_fov(oracle, ref, tshape, mshape):
+    rng = np.random.default_rng(5)
+    It = (rng.random(tshape) * 255).astype(np.uint8)
+    arg = np.array([2, -3, 1.5, 0.1, 0.05, -0.08, 1.1, 0.95, 1.02, 0.01, 0, 0.02], np.float32)
+    tvs, mvs = [1, 1, 1.2], [1.1, 1, 1]
+    T = oracle.affine_matrix(arg, np.array(tshape[::-1], np.int32), tvs,
+                             np.array(mshape[::-1], np.int32), mvs)
+    assert np.array_equal(oracle.match_fov(It, T, mshape), ref.match_fov(It, tvs, mshape, mvs, arg))
+
+
+@pytest.mark.parametrize("reverse,masked", [(True, True), (False, False)])
+def test_mm_reg_pipeline(oracle, ref, reverse, masked):
+    """The nonlinear half of the reference's mm_reg (skip_linear with a given
+    affine: calculate_linear_r, It_match_fov, forward and reverse cdm,
+    compute_mapping_from_displacement, calculate_nonlinear_r) against the same
+    steps composed from the oracle (tests/mmreg_oracle.py)."""
+    from scipy.ndimage import shift
+    from mmreg_oracle import oracle_mm_reg
+    It = smooth_volume((30, 34, 28), 3, sigma=2.5)
+    I = shift(It.astype(np.float32), (1.0, -1.5, 2.0), order=1).astype(np.uint8)[1:, :, 2:]
+    arg = np.array([0.5, -0.3, 0.2, 0.02, -0.01, 0.015, 1.0, 1.0, 1.0, 0, 0, 0], np.float32)
+    vs_t, vs_s = [1, 1, 1], [1.1, 1.0, 0.9]
+    a = ref.mm_reg(It, vs_t, I, vs_s, arg, reverse=reverse, masked_r=masked)
+    b = oracle_mm_reg(oracle, It, vs_t, I, vs_s, arg, reverse=reverse, masked_r=masked)
+    keys = ["r", "J_lin", "It_fov", "t2f", "to2from", "JJ"] + (["f2t", "from2to"] if reverse else [])
+    for k in keys:
+        assert np.array_equal(_bits(a[k]), _bits(b[k])), k

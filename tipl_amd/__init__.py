@@ -37,7 +37,10 @@ EXPORTS = (
     "tiplhip_reg_request_default", "tiplhip_register", "tiplhip_server_create",
     "tiplhip_request_bytes", "tiplhip_server_submit", "tiplhip_server_wait",
     "tiplhip_server_stats_get", "tiplhip_server_destroy", "tiplhip_volume_create_subject",
-    "tiplhip_autotune",
+    "tiplhip_autotune", "tiplhip_displacement_to_mapping", "tiplhip_mapping_to_displacement",
+    "tiplhip_inv_displacement_to_mapping", "tiplhip_compose_mapping",
+    "tiplhip_jacobian_determinant_dis", "tiplhip_jacobian_determinant", "tiplhip_match_fov",
+    "tiplhip_correlation",
 )
 
 
@@ -66,14 +69,18 @@ class RegRequest(C.Structure):
                 ("fixed_vs", C.c_float * 3), ("moving", C.c_void_p), ("moving_dtype", C.c_int),
                 ("moving_dim", C.c_int * 3), ("moving_vs", C.c_float * 3), ("cost", C.c_int),
                 ("bins", C.c_int), ("nonlinear", C.c_int), ("cdm", CdmParam), ("field", C.c_void_p),
-                ("preprocess", C.c_int)]
+                ("preprocess", C.c_int), ("reverse", C.c_int), ("match_fov", C.c_int),
+                ("masked_r", C.c_int), ("skip_linear", C.c_int), ("arg", C.c_float * 12),
+                ("f2t", C.c_void_p), ("to2from", C.c_void_p), ("from2to", C.c_void_p),
+                ("warped", C.c_void_p)]
 
 
 class RegResult(C.Structure):
     _fields_ = [("status", C.c_int), ("affine", C.c_float * 12), ("T", C.c_float * 12),
                 ("cost", C.c_float), ("cdm_iterations", C.c_int), ("evaluations", C.c_int64),
                 ("queue_ms", C.c_double), ("run_ms", C.c_double), ("prep_ms", C.c_double),
-                ("affine_ms", C.c_double), ("cdm_ms", C.c_double)]
+                ("affine_ms", C.c_double), ("cdm_ms", C.c_double), ("r_linear", C.c_float),
+                ("r_nonlinear", C.c_float)]
 
 
 class ServerStats(C.Structure):
@@ -145,6 +152,14 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "tiplhip_server_destroy": (ip, [vp]),
         "tiplhip_volume_create_subject": (ip, [vp, vp, ip, P(ip), P(vp)]),
         "tiplhip_autotune": (ip, [vp, vp, vp, vp, ip, ip, ip, ip, P(ip), P(ip), P(C.c_double)]),
+        "tiplhip_displacement_to_mapping": (ip, [vp, vp, P(ip), vp, vp]),
+        "tiplhip_mapping_to_displacement": (ip, [vp, vp, P(ip), vp]),
+        "tiplhip_inv_displacement_to_mapping": (ip, [vp, vp, P(ip), vp, P(ip), vp]),
+        "tiplhip_compose_mapping": (ip, [vp, vp, ip, P(ip), vp, P(ip), vp]),
+        "tiplhip_jacobian_determinant_dis": (ip, [vp, vp, P(ip), vp]),
+        "tiplhip_jacobian_determinant": (ip, [vp, vp, P(ip), vp]),
+        "tiplhip_match_fov": (ip, [vp, vp, vp, P(ip)]),
+        "tiplhip_correlation": (ip, [vp, vp, vp, ip, P(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -405,6 +420,67 @@ class Context:
             _ptr(field) if want_field else None))
         return field
 
+    # -- post-registration warping (numerical/dif.hpp) -----------------------
+    def displacement_to_mapping(self, dis: np.ndarray, T=None) -> np.ndarray:
+        dis = np.ascontiguousarray(dis, np.float32)
+        out = np.empty_like(dis)
+        Tm = None if T is None else np.ascontiguousarray(T, np.float32)
+        _check(load_library().tiplhip_displacement_to_mapping(
+            self.handle, _ptr(dis), _dims(dis.shape[:3]), None if Tm is None else _ptr(Tm), _ptr(out)))
+        return out
+
+    def mapping_to_displacement(self, mapping: np.ndarray) -> np.ndarray:
+        mapping = np.ascontiguousarray(mapping, np.float32)
+        out = np.empty_like(mapping)
+        _check(load_library().tiplhip_mapping_to_displacement(self.handle, _ptr(mapping),
+                                                              _dims(mapping.shape[:3]), _ptr(out)))
+        return out
+
+    def inv_displacement_to_mapping(self, inv_dis: np.ndarray, T, out_shape) -> np.ndarray:
+        inv_dis = np.ascontiguousarray(inv_dis, np.float32)
+        Tm = np.ascontiguousarray(T, np.float32)
+        out = np.empty(tuple(out_shape) + (3,), np.float32)
+        _check(load_library().tiplhip_inv_displacement_to_mapping(
+            self.handle, _ptr(inv_dis), _dims(inv_dis.shape[:3]), _ptr(Tm), _dims(out_shape), _ptr(out)))
+        return out
+
+    def compose_mapping(self, src: np.ndarray, mapping: np.ndarray) -> np.ndarray:
+        mapping = np.ascontiguousarray(mapping, np.float32)
+        if src.dtype == np.uint8:
+            src, dt = np.ascontiguousarray(src), U8
+        else:
+            src, dt = np.ascontiguousarray(src, np.float32), F32
+        out = np.empty(mapping.shape[:3], src.dtype)
+        _check(load_library().tiplhip_compose_mapping(self.handle, _ptr(src), dt, _dims(src.shape),
+                                                      _ptr(mapping), _dims(mapping.shape[:3]),
+                                                      _ptr(out)))
+        return out
+
+    def jacobian_determinant_dis(self, dis: np.ndarray) -> np.ndarray:
+        dis = np.ascontiguousarray(dis, np.float32)
+        out = np.empty(dis.shape[:3], np.float32)
+        _check(load_library().tiplhip_jacobian_determinant_dis(self.handle, _ptr(dis),
+                                                               _dims(dis.shape[:3]), _ptr(out)))
+        return out
+
+    def jacobian_determinant(self, mapping: np.ndarray) -> np.ndarray:
+        mapping = np.ascontiguousarray(mapping, np.float32)
+        out = np.empty(mapping.shape[:3], np.float32)
+        _check(load_library().tiplhip_jacobian_determinant(self.handle, _ptr(mapping),
+                                                           _dims(mapping.shape[:3]), _ptr(out)))
+        return out
+
+    def match_fov(self, It: Volume, T, moving_shape):
+        """mm_reg::It_match_fov in place on a device volume."""
+        Tm = np.ascontiguousarray(T, np.float32)
+        _check(load_library().tiplhip_match_fov(self.handle, It.handle, _ptr(Tm), _dims(moving_shape)))
+
+    def correlation(self, x: Volume, y: Volume, masked: bool = True) -> float:
+        r = C.c_double(0)
+        _check(load_library().tiplhip_correlation(self.handle, x.handle, y.handle, int(bool(masked)),
+                                                  C.byref(r)))
+        return r.value
+
     def compose(self, Is: np.ndarray, d: np.ndarray) -> np.ndarray:
         Is = np.ascontiguousarray(Is, np.uint8)
         d = np.ascontiguousarray(d, np.float32)
@@ -466,12 +542,15 @@ _DTYPES = {np.dtype(np.uint8): U8, np.dtype(np.int16): I16, np.dtype(np.float32)
 
 
 def make_request(fixed, moving, fixed_vs=(1, 1, 1), moving_vs=(1, 1, 1), cost="mi", bins=256,
-                 nonlinear=False, want_field=True, preprocess="linear", **cdm):
+                 nonlinear=False, want_field=True, preprocess="linear", reverse=False,
+                 match_fov=True, masked_r=True, arg=None, outputs=(), **cdm):
     """A RegRequest over numpy (z, y, x) volumes of uint8 / int16 / float32.
     preprocess: 'linear' quantises both with normalize_upper_lower2 (255.99);
     'mm_reg' uses template_image_pre for fixed and subject_image_pre for moving.
-    Returns (request, keep) where keep holds the arrays the request points to
-    (and the field output, when nonlinear and want_field)."""
+    reverse / match_fov / masked_r: mm_reg's nonlinear options; arg: a given
+    affine_param (skips linear()).  outputs: any of 'f2t', 'to2from',
+    'from2to', 'warped'.  Returns (request, keep) where keep holds the arrays
+    the request points to and the requested outputs."""
     fixed = np.ascontiguousarray(fixed)
     moving = np.ascontiguousarray(moving)
     if fixed.dtype not in _DTYPES or moving.dtype not in _DTYPES:
@@ -495,20 +574,36 @@ def make_request(fixed, moving, fixed_vs=(1, 1, 1), moving_vs=(1, 1, 1), cost="m
         setattr(r.cdm, k, v)
     if gt is not None:
         r.cdm.gradient_type = (gt * 8)[:8].encode() if len(gt) == 1 else gt.encode()[:8].ljust(8, b"r")
-    field = None
+    r.reverse = int(bool(reverse))
+    r.match_fov = int(bool(match_fov))
+    r.masked_r = int(bool(masked_r))
+    if arg is not None:
+        r.skip_linear = 1
+        r.arg[:] = list(map(float, np.asarray(arg, np.float32)))
+    keep = {"fixed": fixed, "moving": moving, "field": None}
     if nonlinear and want_field:
-        field = np.zeros(fixed.shape + (3,), np.float32)
-        r.field = field.ctypes.data
-    return r, {"fixed": fixed, "moving": moving, "field": field}
+        keep["field"] = np.zeros(fixed.shape + (3,), np.float32)
+        r.field = keep["field"].ctypes.data
+    shapes = {"f2t": fixed.shape + (3,), "to2from": fixed.shape + (3,),
+              "from2to": moving.shape + (3,), "warped": fixed.shape}
+    for name in outputs:
+        a = np.zeros(shapes[name], np.uint8 if name == "warped" else np.float32)
+        setattr(r, name, a.ctypes.data)
+        keep[name] = a
+    return r, keep
 
 
-def _result(res: RegResult, field):
-    return {"status": res.status, "affine": np.array(res.affine[:], np.float32),
-            "T": np.array(res.T[:], np.float32), "cost": float(res.cost),
-            "cdm_iterations": int(res.cdm_iterations), "evaluations": int(res.evaluations),
-            "queue_ms": float(res.queue_ms), "run_ms": float(res.run_ms),
-            "prep_ms": float(res.prep_ms), "affine_ms": float(res.affine_ms),
-            "cdm_ms": float(res.cdm_ms), "field": field}
+def _result(res: RegResult, keep):
+    out = {"status": res.status, "affine": np.array(res.affine[:], np.float32),
+           "T": np.array(res.T[:], np.float32), "cost": float(res.cost),
+           "cdm_iterations": int(res.cdm_iterations), "evaluations": int(res.evaluations),
+           "queue_ms": float(res.queue_ms), "run_ms": float(res.run_ms),
+           "prep_ms": float(res.prep_ms), "affine_ms": float(res.affine_ms),
+           "cdm_ms": float(res.cdm_ms), "r_linear": float(res.r_linear),
+           "r_nonlinear": float(res.r_nonlinear)}
+    for name in ("field", "f2t", "to2from", "from2to", "warped"):
+        out[name] = keep.get(name)
+    return out
 
 
 def register(ctx: "Context", fixed, moving, **kw) -> dict:
@@ -516,7 +611,7 @@ def register(ctx: "Context", fixed, moving, **kw) -> dict:
     req, keep = make_request(fixed, moving, **kw)
     res = RegResult()
     _check(load_library().tiplhip_register(ctx.handle, C.byref(req), C.byref(res)))
-    return _result(res, keep["field"])
+    return _result(res, keep)
 
 
 def request_bytes(fixed, moving, **kw) -> int:
@@ -544,7 +639,7 @@ class Server:
         res = RegResult()
         keep = self._keep.pop(ticket, {"field": None})
         _check(load_library().tiplhip_server_wait(self.handle, C.c_uint64(ticket), C.byref(res)))
-        return _result(res, keep["field"])
+        return _result(res, keep)
 
     def stats(self) -> dict:
         st = ServerStats()

@@ -719,14 +719,6 @@ void launch_mi_entropy(tiplhip_ctx* ctx, const Staged& st, int bins)
     TIPLHIP_CHECK(hipGetLastError());
 }
 
-void read_back(tiplhip_ctx* ctx, const void* dev, void* host, size_t bytes)
-{
-    ctx->h_out.reserve(bytes);
-    TIPLHIP_CHECK(hipMemcpyAsync(ctx->h_out.p, dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
-    std::memcpy(host, ctx->h_out.p, bytes);
-}
-
 // tipl::resample of src into a device grid of shape out_dim.
 void launch_resample(tiplhip_ctx* ctx, const tiplhip_volume* src, const int* out_dim, const float* T,
                      bool crop_shortcut, uint8_t* out)

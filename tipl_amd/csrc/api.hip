@@ -374,6 +374,41 @@ void volume_ensure_packed(tiplhip_ctx* ctx, tiplhip_volume* vol)
     vol->d_pack = static_cast<uint64_t*>(p);
 }
 
+void read_back(tiplhip_ctx* ctx, const void* dev, void* host, size_t bytes)
+{
+    ctx->h_out.reserve(bytes);
+    TIPLHIP_CHECK(hipMemcpyAsync(ctx->h_out.p, dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
+    std::memcpy(host, ctx->h_out.p, bytes);
+}
+
+tiplhip_ctx* aux_context(tiplhip_ctx* ctx)
+{
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    if (!ctx->aux) {
+        tiplhip_ctx* a = nullptr;
+        if (tiplhip_ctx_create(ctx->device, &a) != TIPLHIP_OK)
+            throw Error(TIPLHIP_ERR_DEVICE, tiplhip_last_error());
+        ctx->aux = a;
+    }
+    return ctx->aux;
+}
+
+void volume_invalidate(tiplhip_ctx* ctx, tiplhip_volume* vol)
+{
+    // the packed copy is rebuilt on its next use; the old one is released
+    // once ctx's stream (which wrote the new voxels) has passed this point
+    {
+        std::lock_guard<std::mutex> lk(vol->pack_mu);
+        if (vol->d_pack) {
+            TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
+            pool_free(vol->ctx, vol->d_pack);
+            vol->d_pack = nullptr;
+        }
+    }
+    volume_compute_stats(ctx, vol);
+}
+
 void volume_compute_stats(tiplhip_ctx* ctx, tiplhip_volume* vol)
 {
     ctx->d_tmp2.reserve(sizeof(unsigned long long) * 2 + sizeof(uint32_t) * 256);
@@ -388,8 +423,7 @@ void volume_compute_stats(tiplhip_ctx* ctx, tiplhip_volume* vol)
         unsigned long long s[2];
         uint32_t h[256];
     } host;
-    TIPLHIP_CHECK(hipMemcpyAsync(&host, sums, sizeof(host), hipMemcpyDeviceToHost, ctx->stream));
-    TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
+    read_back(ctx, sums, &host, sizeof(host));
     vol->sum = int64_t(host.s[0]);
     vol->sum_sq = int64_t(host.s[1]);
     std::copy(host.h, host.h + 256, vol->hist256);
@@ -414,8 +448,7 @@ void quantise(tiplhip_ctx* ctx, const T* d_in, size_t n, float upper, uint8_t* d
                        ctx->d_tmp2.as<int>());
     TIPLHIP_CHECK(hipGetLastError());
     int mm[2];
-    TIPLHIP_CHECK(hipMemcpyAsync(mm, ctx->d_tmp2.p, sizeof(mm), hipMemcpyDeviceToHost, ctx->stream));
-    TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
+    read_back(ctx, ctx->d_tmp2.p, mm, sizeof(mm));
     float mn, mx;
     if constexpr (std::is_same<T, float>::value) {
         auto unkey = [](int k) {
@@ -527,6 +560,7 @@ int tiplhip_ctx_destroy(tiplhip_ctx* ctx)
 {
     return guarded([&] {
         if (!ctx) return;
+        if (ctx->aux) (void)tiplhip_ctx_destroy(ctx->aux);
         {
             DeviceGuard dg(ctx->device);
             (void)hipStreamSynchronize(ctx->stream);

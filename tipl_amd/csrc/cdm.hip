@@ -1021,9 +1021,7 @@ bool cdm_level(CdmRun& run, const std::vector<const uint8_t*>& It,
             launch_gradient(ctx, Js.as<uint8_t>(), It[size_t(i)], g, type, i > 0 ? 1 : 0,
                             nd.as<float>(), partials.as<double>(), d_cost + i);
         }
-        TIPLHIP_CHECK(hipMemcpyAsync(mod_cost.data(), d_cost, sizeof(float) * size_t(run.nmod),
-                                     hipMemcpyDeviceToHost, ctx->stream));
-        TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
+        read_back(ctx, d_cost, mod_cost.data(), sizeof(float) * size_t(run.nmod));
         float sum_cost = 0.0f;
         for (int i = 0; i < run.nmod; ++i) {
             sum_cost += mod_cost[size_t(i)];
@@ -1045,9 +1043,7 @@ bool cdm_level(CdmRun& run, const std::vector<const uint8_t*>& It,
                                        need_theta ? d_maxlen : nullptr, maxparts.as<float>());
         if (need_theta) {
             unsigned int bits = 0;
-            TIPLHIP_CHECK(hipMemcpyAsync(&bits, d_maxlen, sizeof(bits), hipMemcpyDeviceToHost,
-                                         ctx->stream));
-            TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
+            read_back(ctx, d_maxlen, &bits, sizeof(bits));
             std::memcpy(&theta, &bits, sizeof(theta));
         }
         if (theta == 0.0f) break;
@@ -1117,9 +1113,7 @@ void cdm_fixed(CdmRun& run, const std::vector<const uint8_t*>& It,
                                        need_theta ? d_maxlen : nullptr, maxparts.as<float>());
         if (need_theta) {
             unsigned int bits = 0;
-            TIPLHIP_CHECK(hipMemcpyAsync(&bits, d_maxlen, sizeof(bits), hipMemcpyDeviceToHost,
-                                         ctx->stream));
-            TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
+            read_back(ctx, d_maxlen, &bits, sizeof(bits));
             std::memcpy(&theta, &bits, sizeof(theta));
         }
         if (theta == 0.0f) break;
@@ -1149,6 +1143,41 @@ std::vector<const uint8_t*> volume_ptrs(tiplhip_ctx* ctx, const tiplhip_volume* 
 }
 
 }  // namespace
+
+int cdm_device(tiplhip_ctx* ctx, const tiplhip_volume* const* It, const tiplhip_volume* const* Is,
+               int n, const tiplhip_cdm_param* param, float* d_field,
+               const volatile uint8_t* terminated, int* iterations)
+{
+    TIPLHIP_REQUIRE(n >= 1 && n <= 8, "between 1 and 8 modality pairs");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    DeviceGuard dg(ctx->device);
+    const int* dim = It[0] ? It[0]->dim : nullptr;
+    TIPLHIP_REQUIRE(dim != nullptr, "null volume");
+    auto vt = volume_ptrs(ctx, It, n, dim);
+    auto vs = volume_ptrs(ctx, Is, n, dim);
+    CdmRun run{ctx, {}, n, terminated};
+    if (param)
+        run.param = *param;
+    else
+        tiplhip_cdm_param_default(&run.param);
+    TIPLHIP_REQUIRE(run.param.max_iterations >= 0, "negative iteration cap");
+    TIPLHIP_REQUIRE(run.param.levels >= 0 && run.param.levels <= 16, "levels must be in [0, 16]");
+    TIPLHIP_REQUIRE(run.param.field_bandwidth >= 0 &&
+                        run.param.field_bandwidth <= int(kMaxGaussianBandwidth),
+                    "field_bandwidth must be in [0, 16]");
+    zero(ctx, d_field, sizeof(float) * 3 * vox(dim));
+    bool ok = true;
+    if (run.param.levels > 0) {
+        TIPLHIP_REQUIRE(run.param.level_iterations >= 0, "negative iteration count");
+        cdm_fixed(run, vt, vs, dim, d_field, run.param.levels);
+    } else {
+        ok = cdm_level(run, vt, vs, dim, d_field, run.param.resolution);
+    }
+    if (iterations) *iterations = run.iterations;
+    if (!ok) return TIPLHIP_ERR_FAILED;
+    return run.stopped ? TIPLHIP_ERR_TERMINATED : TIPLHIP_OK;
+}
+
 }  // namespace tiplhip
 
 using namespace tiplhip;
@@ -1175,39 +1204,17 @@ int tiplhip_cdm(tiplhip_ctx* ctx, const tiplhip_volume* const* It, const tiplhip
 {
     return guarded([&] {
         TIPLHIP_REQUIRE(ctx && It && Is && field, "null argument");
-        TIPLHIP_REQUIRE(n >= 1 && n <= 8, "between 1 and 8 modality pairs");
+        TIPLHIP_REQUIRE(n >= 1 && It[0], "null volume");
         std::lock_guard<std::recursive_mutex> lk(ctx->mu);
         DeviceGuard dg(ctx->device);
-        const int* dim = It[0] ? It[0]->dim : nullptr;
-        TIPLHIP_REQUIRE(dim != nullptr, "null volume");
-        auto vt = volume_ptrs(ctx, It, n, dim);
-        auto vs = volume_ptrs(ctx, Is, n, dim);
-        CdmRun run{ctx, {}, n, terminated};
-        if (param)
-            run.param = *param;
-        else
-            tiplhip_cdm_param_default(&run.param);
-        TIPLHIP_REQUIRE(run.param.max_iterations >= 0, "negative iteration cap");
-        TIPLHIP_REQUIRE(run.param.levels >= 0 && run.param.levels <= 16, "levels must be in [0, 16]");
-        TIPLHIP_REQUIRE(run.param.field_bandwidth >= 0 &&
-                            run.param.field_bandwidth <= int(kMaxGaussianBandwidth),
-                        "field_bandwidth must be in [0, 16]");
-        const size_t fb = sizeof(float) * 3 * vox(dim);
+        const size_t fb = sizeof(float) * 3 * vox(It[0]->dim);
         DevMem best(ctx, fb);
-        zero(ctx, best.p, fb);
-        bool ok = true;
-        if (run.param.levels > 0) {
-            TIPLHIP_REQUIRE(run.param.level_iterations >= 0, "negative iteration count");
-            cdm_fixed(run, vt, vs, dim, best.as<float>(), run.param.levels);
-        } else {
-            ok = cdm_level(run, vt, vs, dim, best.as<float>(), run.param.resolution);
-        }
-        if (iterations) *iterations = run.iterations;
-        if (!ok) throw Error(TIPLHIP_ERR_FAILED, "cdm returned false (zero cost)");
+        const int rc = cdm_device(ctx, It, Is, n, param, best.as<float>(), terminated, iterations);
+        if (rc == TIPLHIP_ERR_FAILED) throw Error(TIPLHIP_ERR_FAILED, "cdm returned false (zero cost)");
         TIPLHIP_CHECK(hipMemcpyAsync(field, best.p, fb, hipMemcpyDeviceToHost, ctx->stream));
         TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
         // As in the reference, a terminated run still returns its best field.
-        if (run.stopped) throw Error(TIPLHIP_ERR_TERMINATED, "terminated by the caller");
+        if (rc == TIPLHIP_ERR_TERMINATED) throw Error(TIPLHIP_ERR_TERMINATED, "terminated by the caller");
     });
 }
 

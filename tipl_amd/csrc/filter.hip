@@ -24,25 +24,53 @@ namespace {
 
 constexpr int kGX = 64;        // output columns per block (one wave)
 constexpr int kGY = 8;         // output rows per block (2 per thread row)
-constexpr int kGZ = 16;        // output planes per block
 constexpr int kGRows = kGY / 4;
+constexpr int kGThreads = 256;
 
 struct GaussGeo {
     int w, h, d;
     uint32_t wh;
 };
 
+// One round over a 64 x 8 column of `zq` output planes.  Each thread owns
+// fixed slots of the haloed plane tile A (AR x AC floats): their in-plane
+// source offsets (with the replicate clamp) are computed once, and the next
+// plane's values are loaded into registers while the current plane is
+// filtered, so the global-load latency overlaps the LDS passes.
 template <int CH>
-__global__ __launch_bounds__(256) void gaussian_round_kernel(const float* __restrict__ a,
-                                                             float* __restrict__ b, GaussGeo g,
-                                                             float scale)
+__global__ __launch_bounds__(kGThreads) void gaussian_round_kernel(const float* __restrict__ a,
+                                                                   float* __restrict__ b, GaussGeo g,
+                                                                   float scale, int zq)
 {
     constexpr int AR = kGY + 2, AC = (kGX + 2) * CH, XC = kGX * CH;
-    __shared__ float A[AR][AC];
+    constexpr int NA = AR * AC;
+    constexpr int NL = (NA + kGThreads - 1) / kGThreads;
+    __shared__ float A[NA];
     __shared__ float X[AR][XC];
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * kGX + tx;
-    const int x0 = int(blockIdx.x) * kGX, y0 = int(blockIdx.y) * kGY, z0 = int(blockIdx.z) * kGZ;
-    const int zend = min(z0 + kGZ, g.d);
+    const int x0 = int(blockIdx.x) * kGX, y0 = int(blockIdx.y) * kGY, z0 = int(blockIdx.z) * zq;
+    const int zend = min(z0 + zq, g.d);
+    const uint32_t plane = g.wh * uint32_t(CH);
+
+    // slot k holds A[tid + 256 k]: row r = e / AC, column (x0-1+col) and
+    // channel comp; a clamped neighbour is the centre itself (the
+    // reference's replicate edge)
+    uint32_t off[NL];
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+        const int e = min(tid + kGThreads * k, NA - 1);
+        const int r = e / AC, cc = e - r * AC;
+        const int col = cc / CH, comp = cc - col * CH;
+        const int gy = min(max(y0 - 1 + r, 0), g.h - 1);
+        const int gx = min(max(x0 - 1 + col, 0), g.w - 1);
+        off[k] = (uint32_t(gy) * uint32_t(g.w) + uint32_t(gx)) * uint32_t(CH) + uint32_t(comp);
+    }
+    float v[NL];
+    {
+        const uint32_t pz = uint32_t(min(max(z0 - 1, 0), g.d - 1));
+#pragma unroll
+        for (int k = 0; k < NL; ++k) v[k] = a[size_t(pz) * plane + off[k]];
+    }
 
     float Yp[kGRows][CH], Yc[kGRows][CH];
 #pragma unroll
@@ -51,27 +79,23 @@ __global__ __launch_bounds__(256) void gaussian_round_kernel(const float* __rest
         for (int c = 0; c < CH; ++c) Yp[j][c] = Yc[j][c] = 0.0f;
 
     for (int zz = z0 - 1; zz <= zend; ++zz) {
-        // Plane zz with replicate clamping in z, rows y0-1 .. y0+kGY and
-        // columns x0-1 .. x0+kGX clamped the same way: a clamped neighbour is
-        // the centre itself, which is the reference's edge rule.
-        const uint32_t pz = uint32_t(min(max(zz, 0), g.d - 1));
-        for (int e = tid; e < AR * AC; e += 256) {
-            const int r = e / AC, cc = e - r * AC;
-            const int col = cc / CH, comp = cc - col * CH;
-            const int gy = min(max(y0 - 1 + r, 0), g.h - 1);
-            const int gx = min(max(x0 - 1 + col, 0), g.w - 1);
-            const uint32_t v = pz * g.wh + uint32_t(gy) * uint32_t(g.w) + uint32_t(gx);
-            A[r][cc] = a[size_t(v) * CH + comp];
+#pragma unroll
+        for (int k = 0; k < NL; ++k)
+            if (tid + kGThreads * k < NA) A[tid + kGThreads * k] = v[k];
+        if (zz < zend) {
+            const uint32_t pz = uint32_t(min(zz + 1, g.d - 1));
+#pragma unroll
+            for (int k = 0; k < NL; ++k) v[k] = a[size_t(pz) * plane + off[k]];
         }
         __syncthreads();
         // x pass: (left + (c + c)) + right
-        for (int e = tid; e < AR * kGX; e += 256) {
+        for (int e = tid; e < AR * kGX; e += kGThreads) {
             const int r = e / kGX, col = e - r * kGX;
 #pragma unroll
             for (int c = 0; c < CH; ++c) {
-                float center = A[r][(col + 1) * CH + c];
+                float center = A[r * AC + (col + 1) * CH + c];
                 center += center;
-                X[r][col * CH + c] = (A[r][col * CH + c] + center) + A[r][(col + 2) * CH + c];
+                X[r][col * CH + c] = (A[r * AC + col * CH + c] + center) + A[r * AC + (col + 2) * CH + c];
             }
         }
         __syncthreads();
@@ -120,16 +144,26 @@ GaussGeo gauss_geo(const int* dim)
     return GaussGeo{dim[0], dim[1], dim[2], uint32_t(dim[0]) * uint32_t(dim[1])};
 }
 
+// Planes per block: about 1024 blocks over the volume (each block reads two
+// halo planes beyond its zq), at least 4 and at most 16 planes.
+int gauss_zq(const GaussGeo& g)
+{
+    const long tiles = long((g.w + kGX - 1) / kGX) * long((g.h + kGY - 1) / kGY);
+    const long zq = long(g.d) * tiles / 1024;
+    return int(std::min(16L, std::max(4L, zq)));
+}
+
 void launch_round(tiplhip_ctx* ctx, const float* in, float* out, const GaussGeo& g, int channels,
                   float scale)
 {
+    const int zq = gauss_zq(g);
     const dim3 grid(uint32_t((g.w + kGX - 1) / kGX), uint32_t((g.h + kGY - 1) / kGY),
-                    uint32_t((g.d + kGZ - 1) / kGZ));
+                    uint32_t((g.d + zq - 1) / zq));
     const dim3 block(kGX, 4);
     if (channels == 1)
-        hipLaunchKernelGGL(gaussian_round_kernel<1>, grid, block, 0, ctx->stream, in, out, g, scale);
+        hipLaunchKernelGGL(gaussian_round_kernel<1>, grid, block, 0, ctx->stream, in, out, g, scale, zq);
     else
-        hipLaunchKernelGGL(gaussian_round_kernel<3>, grid, block, 0, ctx->stream, in, out, g, scale);
+        hipLaunchKernelGGL(gaussian_round_kernel<3>, grid, block, 0, ctx->stream, in, out, g, scale, zq);
     TIPLHIP_CHECK(hipGetLastError());
 }
 

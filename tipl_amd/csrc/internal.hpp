@@ -171,6 +171,7 @@ struct tiplhip_ctx {
     // timing
     hipEvent_t t0 = nullptr, t1 = nullptr, k0 = nullptr, k1 = nullptr;
     bool kernel_timing = false;
+    tiplhip_ctx* aux = nullptr;      // see aux_context
     double kernel_ms = 0.0;
     int64_t kernel_launches = 0;
 };
@@ -211,6 +212,25 @@ void launch_downsample_u8(tiplhip_ctx* ctx, const uint8_t* in, const int* dim, u
 void volume_ensure_packed(tiplhip_ctx* ctx, tiplhip_volume* vol);
 // Computes sum / sum_sq / histograms of vol->d on the device and stores them.
 void volume_compute_stats(tiplhip_ctx* ctx, tiplhip_volume* vol);
+// After vol->d was rewritten on ctx's stream: drops the packed copy and
+// recomputes the statistics.
+void volume_invalidate(tiplhip_ctx* ctx, tiplhip_volume* vol);
+
+// cdm() / the fixed-schedule extension into the device field d_field
+// (vox(It[0]->dim) x 3 floats, zeroed here) on ctx's stream.  Returns
+// TIPLHIP_OK, TIPLHIP_ERR_FAILED (zero cost; d_field undefined) or
+// TIPLHIP_ERR_TERMINATED (d_field holds the best field so far).
+int cdm_device(tiplhip_ctx* ctx, const tiplhip_volume* const* It, const tiplhip_volume* const* Is,
+               int n, const tiplhip_cdm_param* param, float* d_field,
+               const volatile uint8_t* terminated, int* iterations);
+
+// Copies `bytes` of device memory to host memory through ctx's pinned
+// staging buffer and waits for it (ctx's stream is synchronised).
+void read_back(tiplhip_ctx* ctx, const void* dev, void* host, size_t bytes);
+
+// A second context on ctx's device (own stream), created on first use and
+// destroyed with ctx: the reverse CDM of a registration runs on it.
+tiplhip_ctx* aux_context(tiplhip_ctx* ctx);
 
 // filter::gaussian (filter/gaussian.hpp:18-89) of a device float image with
 // `channels` interleaved channels (1 or 3), ping-ponging between `a` and

@@ -212,9 +212,7 @@ uint32_t stream_blocks(size_t n) { return uint32_t(std::min<size_t>((n + kPB - 1
 template <class T>
 void read_sync(tiplhip_ctx* ctx, const void* dev, std::vector<T>& host)
 {
-    TIPLHIP_CHECK(hipMemcpyAsync(host.data(), dev, sizeof(T) * host.size(), hipMemcpyDeviceToHost,
-                                 ctx->stream));
-    TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
+    read_back(ctx, dev, host.data(), sizeof(T) * host.size());
 }
 
 // otsu_threshold_from_hist (numerical/otsu.hpp:15-65), the float scan.

@@ -32,7 +32,9 @@
 #include <vector>
 
 #include "../../include/tipl/reg/linear.hpp"
+#include "field.hpp"
 #include "internal.hpp"
+#include "warp.hpp"
 
 namespace tiplhip {
 namespace {
@@ -60,6 +62,10 @@ void check_request(const tiplhip_reg_request* r)
     TIPLHIP_REQUIRE(r->cost == TIPLHIP_COST_MI || r->cost == TIPLHIP_COST_CORR, "unknown cost type");
     TIPLHIP_REQUIRE(r->bins == 256 || r->bins == 64, "bins must be 256 or 64");
     TIPLHIP_REQUIRE(r->preprocess == 0 || r->preprocess == 1, "preprocess must be 0 or 1");
+    TIPLHIP_REQUIRE(!(r->f2t || r->from2to) || (r->nonlinear && r->reverse),
+                    "f2t and from2to need a nonlinear request with reverse set");
+    if (r->skip_linear)
+        for (float v : r->arg) TIPLHIP_REQUIRE(std::isfinite(v), "arg must be finite");
 }
 
 size_t request_bytes(const tiplhip_reg_request& r)
@@ -70,9 +76,29 @@ size_t request_bytes(const tiplhip_reg_request& r)
     b += 8.0 * std::max(nf, nm) * pyramid;            // 8-tap packed copy of the sampled side
     auto es = [](int dt) { return dt == TIPLHIP_F32 ? 4.0 : dt == TIPLHIP_I16 ? 2.0 : 0.0; };
     b += std::max(es(r.fixed_dtype) * nf, es(r.moving_dtype) * nm);  // quantisation staging
-    if (r.nonlinear) b += nf * 75.0 * pyramid;        // J, best field, 4 work fields, Js
+    b += 2.0 * nf;                                     // J and its correlation with the fixed image
+    if (r.nonlinear) {
+        b += nf * 75.0 * pyramid;                      // best field, 4 work fields, Js per direction
+        if (r.reverse) b += nf * 75.0 * pyramid;
+        b += 25.0 * nf + (r.from2to ? 12.0 * nm : 0.0);  // mappings and the warped image
+    }
     if (r.preprocess == 1) b += 12.0 * nm;            // subject preprocessing: 3 float images
     return size_t(b) + (size_t(4) << 20);
+}
+
+// Joins a thread when the scope unwinds (the reverse CDM must not outlive
+// the buffers it reads if the forward direction throws).
+struct Joiner {
+    std::thread t;
+    ~Joiner()
+    {
+        if (t.joinable()) t.join();
+    }
+};
+
+void to_host(tiplhip_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+    TIPLHIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
 }
 
 // One registration on ctx; costs go through `eval`.
@@ -80,66 +106,135 @@ void register_pair(tipl::hip::context& ctx, const tiplhip_reg_request& req,
                    const tipl::reg::detail::cost_evaluator& eval, tiplhip_reg_result& res)
 {
     const auto t0 = steady::now();
+    tiplhip_ctx* c = ctx.get();
     tiplhip_volume* raw = nullptr;
     const bool mm = req.preprocess == 1;
-    rc_check(tiplhip_volume_create(ctx.get(), req.fixed, req.fixed_dtype, req.fixed_dim,
+    rc_check(tiplhip_volume_create(c, req.fixed, req.fixed_dtype, req.fixed_dim,
                                    mm ? 255.999f : 255.99f, &raw));
     auto df = std::make_shared<tipl::hip::volume>(ctx, raw);
     raw = nullptr;
     if (mm)
-        rc_check(tiplhip_volume_create_subject(ctx.get(), req.moving, req.moving_dtype, req.moving_dim,
-                                               &raw));
+        rc_check(tiplhip_volume_create_subject(c, req.moving, req.moving_dtype, req.moving_dim, &raw));
     else
-        rc_check(tiplhip_volume_create(ctx.get(), req.moving, req.moving_dtype, req.moving_dim,
-                                       255.99f, &raw));
+        rc_check(tiplhip_volume_create(c, req.moving, req.moving_dtype, req.moving_dim, 255.99f, &raw));
     auto dm = std::make_shared<tipl::hip::volume>(ctx, raw);
-    // has_mask and the moment estimate read the quantised images on the host
-    const tipl::image<3, uint8_t> hf = df->read(), hm = dm->read();
     const tipl::vector<3> fvs(req.fixed_vs[0], req.fixed_vs[1], req.fixed_vs[2]);
     const tipl::vector<3> mvs(req.moving_vs[0], req.moving_vs[1], req.moving_vs[2]);
+    const tipl::shape<3> fs(req.fixed_dim[0], req.fixed_dim[1], req.fixed_dim[2]);
+    const tipl::shape<3> ms(req.moving_dim[0], req.moving_dim[1], req.moving_dim[2]);
 
-    tipl::reg::linear_reg_param p;
-    p.cuda = true;
-    p.cost_type = req.cost == TIPLHIP_COST_MI ? tipl::reg::mutual_info : tipl::reg::corr;
-    p.mi_bins = req.bins;
+    // linear_reg (reg/mm_reg.hpp:348-368)
     tipl::affine_param<float, 3> arg;
-    bool terminated = false;
+    float cost = 0.0f;
     size_t evals = 0;
-    const auto t1 = steady::now();
-    const float cost = tipl::reg::detail::linear_impl<void>(
-        {tipl::const_pointer_image<3, unsigned char>(hf)}, fvs,
-        {tipl::const_pointer_image<3, unsigned char>(hm)}, mvs, {df}, {dm}, arg, p, terminated, eval,
-        &evals);
-    const tipl::transformation_matrix<float, 3> T(arg, hf.shape(), fvs, hm.shape(), mvs);
+    auto t1 = steady::now();
+    if (req.skip_linear) {
+        std::copy_n(req.arg, 12, arg.data());
+    } else {
+        tipl::reg::linear_reg_param p;
+        p.cuda = true;
+        p.cost_type = req.cost == TIPLHIP_COST_MI ? tipl::reg::mutual_info : tipl::reg::corr;
+        p.mi_bins = req.bins;
+        bool terminated = false;
+        // has_mask and the moment estimate read the quantised images on the host
+        const tipl::image<3, uint8_t> hf = df->read(), hm = dm->read();
+        t1 = steady::now();
+        cost = tipl::reg::detail::linear_impl<void>(
+            {tipl::const_pointer_image<3, unsigned char>(hf)}, fvs,
+            {tipl::const_pointer_image<3, unsigned char>(hm)}, mvs, {df}, {dm}, arg, p, terminated,
+            eval, &evals);
+    }
+    const tipl::transformation_matrix<float, 3> T(arg, fs, fvs, ms, mvs);
     std::copy_n(arg.data(), 12, res.affine);
     std::copy_n(T.data(), 12, res.T);
     res.cost = cost;
     res.evaluations = int64_t(evals);
     res.cdm_iterations = 0;
     const auto t2 = steady::now();
-    // linear_impl runs has_mask / estimate_affine_param on the host first
     res.prep_ms = ms_between(t0, t1);
     res.affine_ms = ms_between(t1, t2);
-    res.cdm_ms = 0.0;
-    if (!req.nonlinear) return;
 
-    // J: the moving image on the fixed grid (tipl::resample, CPU crop rule)
+    // calculate_linear_r (reg/mm_reg.hpp:285-311): J = resample(I, Its, T)
+    const size_t nf = vox(req.fixed_dim), nm = vox(req.moving_dim);
+    const bool masked = req.masked_r != 0;
     raw = nullptr;
-    rc_check(tiplhip_volume_resample(ctx.get(), dm->get(), req.fixed_dim, T.data(), 1, &raw));
+    rc_check(tiplhip_volume_resample(c, dm->get(), req.fixed_dim, T.data(), 1, &raw));
     tipl::hip::volume dj(ctx, raw);
-    std::vector<float> scratch;
-    float* field = req.field;
-    if (!field) {
-        scratch.resize(3 * vox(req.fixed_dim));
-        field = scratch.data();
+    tiplhip_volume* It = df->get();
+    res.r_linear = float(device_correlation(c, dj.get()->d, It->d, nf, masked));
+    if (!req.nonlinear) {
+        if (req.warped) {
+            to_host(c, req.warped, dj.get()->d, nf);
+            TIPLHIP_CHECK(hipStreamSynchronize(c->stream));
+        }
+        res.cdm_ms = ms_between(t2, steady::now());
+        return;
     }
-    const tiplhip_volume* It = df->get();
-    const tiplhip_volume* Is = dj.get();
-    int iterations = 0;
-    const int rc = tiplhip_cdm(ctx.get(), &It, &Is, 1, &req.cdm, field, nullptr, &iterations);
-    res.cdm_iterations = iterations;
+    if (req.match_fov) {
+        launch_match_fov(c, It->d, It->dim, T.data(), req.moving_dim);
+        volume_invalidate(c, It);
+    }
+    TIPLHIP_CHECK(hipStreamSynchronize(c->stream));
+
+    // nonlinear_reg (reg/mm_reg.hpp:370-410): the reverse cdm(J, It) on the
+    // auxiliary stream while the forward cdm(It, J) runs on this one
+    const size_t fb = sizeof(float) * 3 * nf;
+    DevMem t2f(c, fb);
+    tiplhip_ctx* aux = req.reverse ? aux_context(c) : nullptr;
+    std::unique_ptr<DevMem> f2t;
+    int rev_rc = TIPLHIP_OK;
+    std::string rev_err;
+    {
+        Joiner rev;
+        if (aux) {
+            f2t.reset(new DevMem(aux, fb));
+            rev.t = std::thread([&] {
+                try {
+                    DeviceGuard dg(aux->device);
+                    const tiplhip_volume* a = dj.get();
+                    const tiplhip_volume* b = It;
+                    const int rc = cdm_device(aux, &a, &b, 1, &req.cdm, f2t->as<float>(), nullptr, nullptr);
+                    if (rc == TIPLHIP_ERR_FAILED) throw Error(rc, "reverse cdm returned false (zero cost)");
+                    TIPLHIP_CHECK(hipStreamSynchronize(aux->stream));
+                } catch (const Error& e) {
+                    rev_rc = e.code;
+                    rev_err = e.what();
+                } catch (const std::exception& e) {
+                    rev_rc = TIPLHIP_ERR_INVALID;
+                    rev_err = e.what();
+                }
+            });
+        }
+        const tiplhip_volume* a = It;
+        const tiplhip_volume* b = dj.get();
+        int iterations = 0;
+        const int rc = cdm_device(c, &a, &b, 1, &req.cdm, t2f.as<float>(), nullptr, &iterations);
+        res.cdm_iterations = iterations;
+        if (rc == TIPLHIP_ERR_FAILED) throw Error(rc, "cdm returned false (zero cost)");
+    }
+    if (rev_rc != TIPLHIP_OK) throw Error(rev_rc, rev_err);
+
+    // every buffer is allocated before the first launch that reads f2t (an
+    // allocation failure then cannot release f2t under a running kernel)
+    DevMem to2from(c, fb), jj(c, nf);
+    std::unique_ptr<DevMem> from2to;
+    if (aux && req.from2to) from2to.reset(new DevMem(c, sizeof(float) * 3 * nm));
+    // compute_mapping_from_displacement (reg/mm_reg.hpp:271-284)
+    launch_mapping(c, t2f.as<float>(), req.fixed_dim, T.data(), to2from.as<float>());
+    if (from2to)
+        launch_inv_mapping(c, f2t->as<float>(), req.fixed_dim, T.data(), req.moving_dim,
+                           from2to->as<float>());
+    // calculate_nonlinear_r (reg/mm_reg.hpp:330-346): J = compose_mapping(I, to2from)
+    launch_compose_mapping_u8(c, dm->get()->d, req.moving_dim, to2from.as<float>(), req.fixed_dim,
+                              jj.as<uint8_t>());
+    res.r_nonlinear = float(device_correlation(c, jj.as<uint8_t>(), It->d, nf, masked));
+    if (req.field) to_host(c, req.field, t2f.p, fb);
+    if (aux && req.f2t) to_host(c, req.f2t, f2t->p, fb);
+    if (req.to2from) to_host(c, req.to2from, to2from.p, fb);
+    if (from2to) to_host(c, req.from2to, from2to->p, sizeof(float) * 3 * nm);
+    if (req.warped) to_host(c, req.warped, jj.p, nf);
+    TIPLHIP_CHECK(hipStreamSynchronize(c->stream));
     res.cdm_ms = ms_between(t2, steady::now());
-    rc_check(rc);
 }
 
 // ---------------------------------------------------------------------------
@@ -364,6 +459,9 @@ void tiplhip_reg_request_default(tiplhip_reg_request* r)
     r->cost = TIPLHIP_COST_MI;
     r->bins = 256;
     tiplhip_cdm_param_default(&r->cdm);
+    r->match_fov = 1;
+    r->masked_r = 1;
+    for (int i = 6; i < 9; ++i) r->arg[i] = 1.0f;  // affine_param's identity scaling
 }
 
 size_t tiplhip_request_bytes(const tiplhip_reg_request* req)
