@@ -167,10 +167,11 @@ int tiplhip_cost_batch(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplh
 /* Autotuner of the cost kernels' launch shape (no reference counterpart):
  * times the histogram (MI) or sums (correlation) kernel for these K
  * transforms over a grid of work-item depths (8, 16, 32 planes) and
- * workgroups per CU, `reps` launches each, and keeps the fastest for every
- * later single-pair launch on ctx with the same shapes, K, cost and bins.
- * Results do not depend on the shape (integer accumulation).  Outputs may be
- * NULL. */
+ * workgroups per CU (per_cu 0: the fewest workgroups, each as full as the
+ * histogram counters allow), `reps` launches each, and keeps the fastest for
+ * every later single-pair launch on ctx with the same shapes, K, cost and
+ * bins.  Results do not depend on the shape (integer accumulation).  Outputs
+ * may be NULL. */
 int tiplhip_autotune(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_volume* to,
                      const float* T, int K, int cost, int bins, int reps, int* zc, int* per_cu,
                      double* best_ms);

@@ -330,7 +330,7 @@ struct MiTraits<64> {
 template <int BINS>
 __global__ __launch_bounds__(MiTraits<BINS>::kThreads) void mi_hist_kernel(
     const Segment* __restrict__ segs, uint32_t nseg, uint32_t total_blocks,
-    const float* __restrict__ Ts, int* __restrict__ hist)
+    const float* __restrict__ Ts, int* __restrict__ hist, uint32_t* __restrict__ to_hist)
 {
     using Tr = MiTraits<BINS>;
     __shared__ uint32_t lds[Tr::kWords];
@@ -384,9 +384,26 @@ __global__ __launch_bounds__(MiTraits<BINS>::kThreads) void mi_hist_kernel(
         }
     }
     __syncthreads();
+    // Flush the non-zero words into the candidate's histogram.  The thread
+    // count is a multiple of the words per histogram row, so all of a
+    // thread's words lie in one column (pair): it also sums that column over
+    // its rows, and the partial column sums go to the candidate's to_hist
+    // (the histogram of the sampled values, get_mutual_info's to_hist).
+    constexpr uint32_t kWpr = uint32_t(Tr::kWords) / BINS;   // words per row
+    constexpr uint32_t kBpw = BINS / kWpr;                   // bins per word
+    constexpr uint32_t kQ = uint32_t(Tr::kThreads) / kWpr;   // threads per column word
+    static_assert(uint32_t(Tr::kThreads) % kWpr == 0 && kQ * BINS <= uint32_t(Tr::kWords),
+                  "flush layout");
     int* h = hist + size_t(cand) * BINS * BINS;
+    uint32_t col_lo = 0, col_hi = 0;
     for (uint32_t i = threadIdx.x; i < uint32_t(Tr::kWords); i += blockDim.x) {
         const uint32_t v = lds[i];
+        if constexpr (BINS == 256) {
+            col_lo += v & 0xffffu;
+            col_hi += v >> 16;
+        } else {
+            col_lo += v;
+        }
         if (!v) continue;
         if constexpr (BINS == 256) {
             if (v & 0xffffu) atomicAdd(&h[2 * i], int(v & 0xffffu));
@@ -395,54 +412,59 @@ __global__ __launch_bounds__(MiTraits<BINS>::kThreads) void mi_hist_kernel(
             atomicAdd(&h[i], int(v));
         }
     }
+    __syncthreads();  // every word has been read: lds becomes kQ x BINS column partials
+    {
+        const uint32_t j = threadIdx.x % kWpr, q = threadIdx.x / kWpr;
+        lds[q * BINS + kBpw * j] = col_lo;
+        if constexpr (kBpw == 2) lds[q * BINS + 2 * j + 1] = col_hi;
+    }
+    __syncthreads();
+    if (threadIdx.x < uint32_t(BINS)) {
+        uint32_t c = 0;
+        for (uint32_t q = 0; q < kQ; ++q) c += lds[q * BINS + threadIdx.x];
+        if (c) atomicAdd(&to_hist[size_t(cand) * BINS + threadIdx.x], c);
+    }
 }
 
-// get_mutual_info_mean (reg/linear.hpp:87-104): to_hist = column sums,
-// sum of mu*log(mu/from_h[row]/to_h[col]) in FP64 over non-zero bins,
-// divided by float(BINS*BINS) and negated (reg/linear.hpp:201).  One
-// 1024-thread workgroup per candidate: column sums in 1024/BINS row slices
-// (exact integers), then 16 waves of FP64 terms combined by a fixed tree,
-// so the result is deterministic (the reference's sum is sequential in
-// row-major order; the two agree to rounding).
-constexpr int kEntropyThreads = 1024;
+// get_mutual_info_mean (reg/linear.hpp:87-104): the sum of
+// mu*log(mu/from_h[row]/to_h[col]) in FP64 over non-zero bins, divided by
+// float(BINS*BINS) and negated (reg/linear.hpp:201).  to_h comes from the
+// histogram kernel.  Each candidate's bins are split into slices of 4096
+// (16 per thread), one workgroup per slice, so even a single candidate
+// fills many CUs; the slice sums are combined by mi_entropy_final_kernel
+// in a fixed tree.  The result is deterministic (the reference's sum is
+// sequential in row-major order; the two agree to rounding).
+constexpr int kEntropyThreads = 256;
+constexpr int kEntropySlice = 4096;
+
 template <int BINS>
-__global__ __launch_bounds__(kEntropyThreads) void mi_entropy_kernel(
+constexpr int entropy_slices() { return BINS * BINS / kEntropySlice; }
+
+template <int BINS>
+__global__ __launch_bounds__(kEntropyThreads) void mi_entropy_part_kernel(
     const Segment* __restrict__ segs, uint32_t nseg, const int* __restrict__ hist,
-    double* __restrict__ costs)
+    const uint32_t* __restrict__ to_hist, double* __restrict__ part)
 {
-    constexpr int kSlices = kEntropyThreads / BINS;
-    constexpr int kRows = BINS / kSlices;
+    constexpr int G = entropy_slices<BINS>();
+    constexpr int kVec = kEntropySlice / 4 / kEntropyThreads;
     constexpr int kWaves = kEntropyThreads / 64;
-    const int* h = hist + size_t(blockIdx.x) * BINS * BINS;
+    const uint32_t cand = blockIdx.x / G, slice = blockIdx.x % G;
     const uint32_t* __restrict__ from_hist =
-        segs[find_segment(segs, nseg, blockIdx.x, ByCandidate{})].from_hist;
-    __shared__ uint32_t colpart[kSlices][BINS];
+        segs[find_segment(segs, nseg, cand, ByCandidate{})].from_hist;
     __shared__ uint32_t to_h[BINS];
-    __shared__ double part[kWaves];
-    {
-        const int c = threadIdx.x % BINS, sl = threadIdx.x / BINS;
-        uint32_t s = 0;
-#pragma unroll 16
-        for (int r = sl * kRows; r < (sl + 1) * kRows; ++r) s += uint32_t(h[r * BINS + c]);
-        colpart[sl][c] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x < BINS) {
-        uint32_t s = 0;
-        for (int sl = 0; sl < kSlices; ++sl) s += colpart[sl][threadIdx.x];
-        to_h[threadIdx.x] = s;
-    }
-    __syncthreads();
+    __shared__ double wpart[kWaves];
+    for (int i = threadIdx.x; i < BINS; i += kEntropyThreads) to_h[i] = to_hist[size_t(cand) * BINS + i];
     // four bins per 16-byte load, all of a thread's loads issued before use
-    constexpr int kVec = BINS * BINS / 4 / kEntropyThreads;
-    const int4* h4 = reinterpret_cast<const int4*>(h);
+    const int4* h4 =
+        reinterpret_cast<const int4*>(hist + size_t(cand) * BINS * BINS + size_t(slice) * kEntropySlice);
     int4 q[kVec];
 #pragma unroll
     for (int u = 0; u < kVec; ++u) q[u] = h4[threadIdx.x + u * kEntropyThreads];
+    __syncthreads();
     double acc = 0.0;
 #pragma unroll
     for (int u = 0; u < kVec; ++u) {
-        const int i0 = 4 * (int(threadIdx.x) + u * kEntropyThreads);
+        const int i0 = int(slice) * kEntropySlice + 4 * (int(threadIdx.x) + u * kEntropyThreads);
         const int mus[4] = {q[u].x, q[u].y, q[u].z, q[u].w};
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
@@ -454,15 +476,32 @@ __global__ __launch_bounds__(kEntropyThreads) void mi_entropy_kernel(
         }
     }
     acc = wave_sum_f64(acc);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    if ((threadIdx.x & 63) == 0) wpart[threadIdx.x >> 6] = acc;
     __syncthreads();
     if (threadIdx.x == 0) {
         double t[kWaves];
-        for (int i = 0; i < kWaves; ++i) t[i] = part[i];
+        for (int i = 0; i < kWaves; ++i) t[i] = wpart[i];
         for (int w = kWaves / 2; w > 0; w /= 2)
             for (int i = 0; i < w; ++i) t[i] = t[2 * i] + t[2 * i + 1];
-        costs[blockIdx.x] = -(t[0] / double(float(BINS * BINS)));
+        part[blockIdx.x] = t[0];
     }
+}
+
+template <int BINS>
+__global__ __launch_bounds__(256) void mi_entropy_final_kernel(const double* __restrict__ part,
+                                                               uint32_t K, double* __restrict__ costs)
+{
+    constexpr int G = entropy_slices<BINS>();
+    const uint32_t cand = blockIdx.x * 256u + threadIdx.x;
+    if (cand >= K) return;
+    double t[G];
+#pragma unroll
+    for (int i = 0; i < G; ++i) t[i] = part[size_t(cand) * G + i];
+#pragma unroll
+    for (int w = G / 2; w > 0; w /= 2)
+#pragma unroll
+        for (int i = 0; i < w; ++i) t[i] = t[2 * i] + t[2 * i + 1];
+    costs[cand] = -(t[0] / double(float(BINS * BINS)));
 }
 
 struct TransformArg {
@@ -587,7 +626,12 @@ bool hist256(int cost, int bins) { return cost == TIPLHIP_COST_MI && bins == 256
 constexpr double kMaxItemsPerBlock = 256.0;
 
 // The launch shape of one pair: planes per work item and workgroups per CU.
-LaunchShape default_shape(int cost, int bins) { return LaunchShape{16u, hist256(cost, bins) ? 4u : 8u}; }
+// Measured on the device over 48^3 .. 182x218x182 pairs with 12 and 64
+// candidates: the 256-bin histogram (128 KB of LDS, one 1024-thread
+// workgroup per CU) is fastest with about one workgroup per CU -- fewer,
+// fuller workgroups amortise the LDS clear and flush -- and the 64-bin one
+// with about eight.
+LaunchShape default_shape(int cost, int bins) { return LaunchShape{16u, hist256(cost, bins) ? 1u : 8u}; }
 
 TuneKey tune_key(const tiplhip_volume* from, const tiplhip_volume* to, int K, int cost, int bins)
 {
@@ -597,9 +641,8 @@ TuneKey tune_key(const tiplhip_volume* from, const tiplhip_volume* to, int K, in
 
 // Builds the segment table of `jobs` (T holds their transforms, job after
 // job; crop, when not null, one entry per candidate) and uploads all of it
-// in one copy.  Each pair gets a share of about cu_count x 8 workgroups (x 4
-// for the 1024-thread 256-bin histogram) in proportion to its voxel-candidate
-// work.
+// in one copy.  Each pair gets a share of about cu_count x per_cu workgroups
+// in proportion to its voxel-candidate work.
 Staged stage_jobs(tiplhip_ctx* ctx, const std::vector<PairJob>& jobs, const float* T,
                   const int4* crop, int cost, int bins, const LaunchShape* force = nullptr)
 {
@@ -622,8 +665,11 @@ Staged stage_jobs(tiplhip_ctx* ctx, const std::vector<PairJob>& jobs, const floa
     // at least per_cu workgroups per CU, and no workgroup with more than
     // kMaxItemsPerBlock items (large batches get more workgroups, not longer ones:
     // concurrent workgroups then share z slabs of one pair in L2)
-    const double target = std::max(double(ctx->cu_count) * double(shape.per_cu),
-                                    std::ceil(work / double(kMaxItemsPerBlock)));
+    // (per_cu 0 asks for the fewest workgroups: each as full as the counters allow)
+    const double target = shape.per_cu == 0
+                              ? 1.0
+                              : std::max(double(ctx->cu_count) * double(shape.per_cu),
+                                         std::ceil(work / double(kMaxItemsPerBlock)));
     uint64_t blocks = 0;
     for (size_t j = 0; j < jobs.size(); ++j) {
         Segment& sg = segs[j];
@@ -691,32 +737,53 @@ void launch_corr_finalize(tiplhip_ctx* ctx, const Staged& st, long long* sums_ou
 }
 
 // Joint histograms of every staged candidate into ctx->d_hist.
+// d_hist holds K histograms of bins^2 int32, then K to_hist rows of bins.
+uint32_t* to_hist_of(tiplhip_ctx* ctx, const Staged& st, int bins)
+{
+    return reinterpret_cast<uint32_t*>(ctx->d_hist.as<int>() + size_t(bins) * bins * size_t(st.K));
+}
+
 void launch_mi_hist(tiplhip_ctx* ctx, const Staged& st, int bins)
 {
-    size_t hb = sizeof(int) * size_t(bins) * bins * size_t(st.K);
+    size_t hb = sizeof(int) * (size_t(bins) * bins + size_t(bins)) * size_t(st.K);
     ctx->d_hist.reserve(hb);
     zero_bytes16(ctx, ctx->d_hist.p, hb);
     timing_begin(ctx);
     if (bins == 256)
         hipLaunchKernelGGL(mi_hist_kernel<256>, dim3(st.total_blocks), dim3(1024), 0, ctx->stream,
-                           st.segs, st.nseg, st.total_blocks, st.T, ctx->d_hist.as<int>());
+                           st.segs, st.nseg, st.total_blocks, st.T, ctx->d_hist.as<int>(),
+                           to_hist_of(ctx, st, bins));
     else
         hipLaunchKernelGGL(mi_hist_kernel<64>, dim3(st.total_blocks), dim3(256), 0, ctx->stream,
-                           st.segs, st.nseg, st.total_blocks, st.T, ctx->d_hist.as<int>());
+                           st.segs, st.nseg, st.total_blocks, st.T, ctx->d_hist.as<int>(),
+                           to_hist_of(ctx, st, bins));
     TIPLHIP_CHECK(hipGetLastError());
     timing_end(ctx);
 }
 
+// d_costs holds the K costs, then K x slices partial sums.
+template <int BINS>
+void launch_mi_entropy_t(tiplhip_ctx* ctx, const Staged& st)
+{
+    constexpr int G = entropy_slices<BINS>();
+    ctx->d_costs.reserve(sizeof(double) * size_t(st.K) * (1 + G));
+    double* costs = ctx->d_costs.as<double>();
+    double* part = costs + st.K;
+    hipLaunchKernelGGL(mi_entropy_part_kernel<BINS>, dim3(st.K * uint32_t(G)), dim3(kEntropyThreads),
+                       0, ctx->stream, st.segs, st.nseg, ctx->d_hist.as<int>(),
+                       to_hist_of(ctx, st, BINS), part);
+    TIPLHIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(mi_entropy_final_kernel<BINS>, dim3((st.K + 255) / 256), dim3(256), 0,
+                       ctx->stream, part, st.K, costs);
+    TIPLHIP_CHECK(hipGetLastError());
+}
+
 void launch_mi_entropy(tiplhip_ctx* ctx, const Staged& st, int bins)
 {
-    ctx->d_costs.reserve(sizeof(double) * size_t(st.K));
     if (bins == 256)
-        hipLaunchKernelGGL(mi_entropy_kernel<256>, dim3(st.K), dim3(kEntropyThreads), 0, ctx->stream,
-                           st.segs, st.nseg, ctx->d_hist.as<int>(), ctx->d_costs.as<double>());
+        launch_mi_entropy_t<256>(ctx, st);
     else
-        hipLaunchKernelGGL(mi_entropy_kernel<64>, dim3(st.K), dim3(kEntropyThreads), 0, ctx->stream,
-                           st.segs, st.nseg, ctx->d_hist.as<int>(), ctx->d_costs.as<double>());
-    TIPLHIP_CHECK(hipGetLastError());
+        launch_mi_entropy_t<64>(ctx, st);
 }
 
 // tipl::resample of src into a device grid of shape out_dim.
@@ -915,7 +982,7 @@ int tiplhip_autotune(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip
         volume_ensure_packed(ctx, const_cast<tiplhip_volume*>(t));
         const bool wide = hist256(cost, bins);  // 1024-thread workgroups
         const uint32_t zcs[] = {8u, 16u, 32u};
-        const uint32_t pcs_narrow[] = {2u, 4u, 8u, 16u}, pcs_wide[] = {1u, 2u, 4u, 8u};
+        const uint32_t pcs_narrow[] = {2u, 4u, 8u, 16u, 0u}, pcs_wide[] = {1u, 2u, 4u, 8u, 0u};
         const uint32_t* pcs = wide ? pcs_wide : pcs_narrow;
         LaunchShape best = default_shape(cost, bins);
         double best_t = 1e30;
@@ -924,7 +991,7 @@ int tiplhip_autotune(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip
         const int64_t keep_n = ctx->kernel_launches;
         ctx->kernel_timing = true;
         for (uint32_t z : zcs)
-            for (int p = 0; p < 4; ++p) {
+            for (int p = 0; p < 5; ++p) {
                 const LaunchShape shape{z, pcs[p]};
                 const Staged st = stage_jobs(ctx, {PairJob{f, t, K}}, Tk.data(),
                                              crop.empty() ? nullptr : crop.data(), cost, bins, &shape);
