@@ -8,6 +8,8 @@
 #include "filter/gaussian.hpp"
 #include "hip.hpp"
 #include "image.hpp"
+#include "io/nifti.hpp"
+#include "numerical/dif.hpp"
 #include "optimization.hpp"
 #include "reg/cdm.hpp"
 #include "reg/linear.hpp"

@@ -88,6 +88,26 @@ public:
     }
 };
 
+// tipl::matrix<4,4> (numerical/matrix.hpp): the 4x4 row-major image
+// transformation of an image space (srow_x, srow_y, srow_z, 0 0 0 1).
+template <int rows, int cols, typename T = float>
+struct matrix {
+    T data_[rows * cols] = {};
+    void identity()
+    {
+        std::fill_n(data_, rows * cols, T(0));
+        for (int i = 0; i < std::min(rows, cols); ++i) data_[i * cols + i] = T(1);
+    }
+    T* begin() { return data_; }
+    T* end() { return data_ + rows * cols; }
+    const T* begin() const { return data_; }
+    const T* end() const { return data_ + rows * cols; }
+    T operator[](unsigned i) const { return data_[i]; }
+    T& operator[](unsigned i) { return data_[i]; }
+    bool operator==(const matrix& r) const { return std::equal(begin(), end(), r.begin()); }
+    bool operator!=(const matrix& r) const { return !(*this == r); }
+};
+
 template <typename value_type_ = float, int dim = 3>
 struct transformation_matrix;
 

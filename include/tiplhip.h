@@ -337,6 +337,45 @@ int tiplhip_correlation(tiplhip_ctx* ctx, const tiplhip_volume* x, const tiplhip
                         int masked, double* r);
 
 /* ---------------------------------------------------------------------------
+ * NIfTI files on the host (io/nifti.hpp), so real images can be handed to
+ * tiplhip_register.  NIfTI-1 (.nii, n+1 / ni1 + .img), NIfTI-2, Analyze
+ * .hdr/.img, either byte order, gzip-compressed or not.  Reading follows
+ * tipl::io::nifti's `>>`: qform handled when there is no sform, the axes
+ * swapped and flipped into LPS (toLPS), voxels converted to the output type,
+ * then scl_slope / scl_inter.
+ * ------------------------------------------------------------------------- */
+typedef struct tiplhip_nifti_info {
+    int dim[8];          /* after the LPS reorientation: dim[0] rank, dim[1..3] x, y, z,
+                            dim[4] volumes */
+    float pixdim[8];     /* voxel sizes in pixdim[1..3], after the reorientation */
+    float srow[12];      /* srow_x, srow_y, srow_z after qform handling and reorientation
+                            (the reference's image transformation, last row 0 0 0 1) */
+    int datatype, bitpix;
+    float scl_slope, scl_inter;
+    int sform_code, qform_code;
+    int is_mni;          /* nifti_base::is_mni (io/nifti.hpp:807-814) */
+    int version;         /* 1 (n+1), 2 (n+2), 0 (ni1 or Analyze pair) */
+    int big_endian;
+    int flip_swap[6];    /* the toLPS sequence: 0-2 flip x/y/z, 3 swap xy, 4 swap yz, 5 swap xz */
+    int flip_swap_n;
+    char descrip[80];
+} tiplhip_nifti_info;
+
+/* nifti_base::open + header (io/nifti.hpp:601-723). */
+int tiplhip_nifti_read_info(const char* path, tiplhip_nifti_info* info);
+
+/* `>> image` of volume `volume` (io/nifti.hpp:1126-1160) into out: dtype
+ * TIPLHIP_U8, TIPLHIP_I16 or TIPLHIP_F32, vox of the reoriented dim[1..3]
+ * voxels, x fastest.  info may be NULL. */
+int tiplhip_nifti_read(const char* path, int volume, int dtype, void* out, tiplhip_nifti_info* info);
+
+/* `<< vs << R << image` (io/nifti.hpp:937-985) on a fresh header: NIfTI-1
+ * n+1, sform_code 1 (4 when is_mni), qform_code 0, srow = the 12 floats;
+ * gzip-compressed when path ends in .gz. */
+int tiplhip_nifti_write(const char* path, const void* data, int dtype, const int dim[3],
+                        const float vs[3], const float srow[12], int is_mni);
+
+/* ---------------------------------------------------------------------------
  * Registration of one image pair in the reference's mm_reg order
  * (reg/mm_reg.hpp:285-416), every step on the device:
  *   linear_reg: linear(fixed, moving) (or the given arg with skip_linear);

@@ -404,6 +404,13 @@ class Reference(_Lib, _Warp):
                                    C.c_void_p)
         self._bind_warp("tref_")
         self._fov = self._fn("tref_match_fov", None, _u8p, _i32p, _f32p, _i32p, _f32p, _f32p)
+        _ip = C.POINTER(C.c_int)
+        self._nii_f = self._fn("tref_nifti_read_f32", C.c_int, C.c_char_p, C.c_int, _f32p,
+                               C.c_size_t, _i32p, _f32p, _f32p, _ip)
+        self._nii_u8 = self._fn("tref_nifti_read_u8", C.c_int, C.c_char_p, C.c_int, _u8p,
+                                C.c_size_t, _i32p, _f32p, _f32p, _ip)
+        self._nii_w = self._fn("tref_nifti_write_f32", C.c_int, C.c_char_p, _f32p, _i32p, _f32p,
+                               _f32p)
         self._mm = self._fn("tref_mm_reg", None, _u8p, _i32p, _f32p, _u8p, _i32p, _f32p, _f32p,
                             C.c_int, C.c_int, C.c_int, _f32p, _u8p, _u8p, _f32p, _f32p, _f32p,
                             _f32p, _u8p)
@@ -428,6 +435,27 @@ class Reference(_Lib, _Warp):
                  int(match_fov), int(masked_r), out["r"], out["J_lin"], out["It_fov"], out["t2f"],
                  out["f2t"], out["to2from"], out["from2to"], out["JJ"])
         return out
+
+    def nifti_read(self, path, volume=0, dtype=np.float32, cap=1 << 26):
+        """tipl::io::gz_nifti `>>` of one volume: (array (z, y, x), vs, srow, is_mni)."""
+        buf = np.zeros(cap, np.dtype(dtype))
+        dim = np.zeros(3, np.int32)
+        vs = np.zeros(3, np.float32)
+        srow = np.zeros(12, np.float32)
+        mni = C.c_int(0)
+        f = self._nii_u8 if np.dtype(dtype) == np.uint8 else self._nii_f
+        ok = f(os.fsencode(str(path)), int(volume), buf, cap, dim, vs, srow, C.byref(mni))
+        if not ok:
+            raise IOError(f"reference reader failed on {path}")
+        n = int(np.prod(dim))
+        return buf[:n].reshape(dim[::-1]).copy(), vs, srow, bool(mni.value)
+
+    def nifti_write(self, path, data, vs, srow):
+        data = np.ascontiguousarray(data, np.float32)
+        ok = self._nii_w(os.fsencode(str(path)), data.reshape(-1), _dims(data),
+                         np.asarray(vs, np.float32), np.asarray(srow, np.float32))
+        if not ok:
+            raise IOError(f"reference writer failed on {path}")
 
     def cdm_iterations(self, It, Is, iters, want_field=False):
         """Wall seconds of `iters` reference cdm() loop-body iterations at one level."""

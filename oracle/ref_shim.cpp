@@ -13,7 +13,12 @@
 #include <cstring>
 #include <vector>
 
+#include <map>     // io/gz_stream.hpp uses std::map without including it
+#include <zlib.h>  // before io/gz_stream.hpp, which is compiled only under ZLIB_H
+
 #include "filter/gaussian.hpp"
+#include "io/gz_stream.hpp"
+#include "io/nifti.hpp"
 #include "numerical/otsu.hpp"
 #include "reg/cdm.hpp"
 #include "reg/mm_reg.hpp"
@@ -490,5 +495,62 @@ void tref_mm_reg(const uint8_t* It, const int* tdim, const float* tvs, const uin
 }
 
 }  // extern "C"
+
+// tipl::io::gz_nifti reading (`>>`: LPS reorientation, type conversion,
+// scl) of volume `volume` into a float or uint8 image, with the header
+// values the registration uses.  Returns 0 on failure.
+template <class T>
+static int nifti_read_as(const char* path, int volume, T* out, size_t cap, int* dim, float* vs,
+                         float* srow, int* is_mni)
+{
+    tipl::io::gz_nifti in;
+    if (!in.open(path, std::ios::in)) return 0;
+    tipl::image<3, T> I;
+    for (int v = 0; v <= volume; ++v)
+        if (!(in >> I)) return 0;
+    if (I.size() > cap) return 0;
+    std::copy(I.begin(), I.end(), out);
+    tipl::shape<3> s;
+    tipl::vector<3> pv;
+    tipl::matrix<4, 4> R;
+    in >> s >> pv >> R;
+    for (int i = 0; i < 3; ++i) dim[i] = int(s[i]);
+    for (int i = 0; i < 3; ++i) vs[i] = pv[i];
+    std::copy_n(R.begin(), 12, srow);
+    *is_mni = in.is_mni() ? 1 : 0;
+    return 1;
+}
+
+extern "C" {
+
+int tref_nifti_read_f32(const char* path, int volume, float* out, size_t cap, int* dim, float* vs,
+                        float* srow, int* is_mni)
+{
+    return nifti_read_as(path, volume, out, cap, dim, vs, srow, is_mni);
+}
+
+int tref_nifti_read_u8(const char* path, int volume, uint8_t* out, size_t cap, int* dim, float* vs,
+                       float* srow, int* is_mni)
+{
+    return nifti_read_as(path, volume, out, cap, dim, vs, srow, is_mni);
+}
+
+// `<< vs << R << image` with gz_nifti (gzip when the path ends in .gz).
+int tref_nifti_write_f32(const char* path, const float* data, const int* dim, const float* vs,
+                         const float* srow)
+{
+    tipl::io::gz_nifti out;
+    if (!out.open(path, std::ios::out)) return 0;
+    tipl::image<3> I(tipl::shape<3>(dim[0], dim[1], dim[2]));
+    std::copy_n(data, I.size(), I.begin());
+    tipl::matrix<4, 4> R;
+    R.identity();
+    std::copy_n(srow, 12, R.begin());
+    out << tipl::vector<3>(vs) << R << I;
+    return bool(out) ? 1 : 0;
+}
+
+}  // extern "C"
+
 
 

@@ -16,6 +16,11 @@
 //       the fixed-schedule cdm.  Writes <out>.r (r after the linear and the
 //       nonlinear step), .t2f, .f2t, .to2from, .from2to (float3 fields),
 //       .JJ (the warped subject) and .jac (jacobian_determinant_dis of t2f).
+//   reg_cli nifti <in.nii[.gz]> <out>
+//       tipl::io::gz_nifti: reads the first volume as float (`>>` image, voxel
+//       size, shape, transformation), writes <out>.f32 and <out>.txt (w h d,
+//       vs, 12 srow values, is_mni), then writes the image back with `<<` as
+//       <out>.nii.gz.
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -119,6 +124,28 @@ int main(int argc, char** argv)
             tipl::image<3, float> jac;
             tipl::jacobian_determinant_dis(m.t2f_dis, jac);
             write_raw((out + ".jac").c_str(), jac.data(), jac.size());
+            return 0;
+        }
+        if (argc >= 4 && std::string(argv[1]) == "nifti") {
+            tipl::io::gz_nifti in;
+            if (!in.open(argv[2], std::ios::in)) throw std::runtime_error(in.error_msg);
+            tipl::image<3, float> I;
+            tipl::vector<3> vs;
+            tipl::shape<3> s;
+            tipl::matrix<4, 4> R;
+            in >> I >> vs >> s >> R;
+            if (!in) throw std::runtime_error(in.error_msg);
+            const std::string out = argv[3];
+            write_raw((out + ".f32").c_str(), I.data(), I.size());
+            std::ofstream txt(out + ".txt");
+            txt.precision(9);
+            txt << s[0] << " " << s[1] << " " << s[2] << "\n" << vs[0] << " " << vs[1] << " " << vs[2] << "\n";
+            for (int i = 0; i < 12; ++i) txt << R[unsigned(i)] << " ";
+            txt << "\n" << (in.is_mni() ? 1 : 0) << "\n";
+            tipl::io::gz_nifti o;
+            o.open(out + ".nii.gz", std::ios::out);
+            o << vs << R << I;
+            if (!o) throw std::runtime_error(o.error_msg);
             return 0;
         }
         if (argc < 8) {

@@ -151,3 +151,25 @@ def test_register_rejects_reverse_outputs_without_reverse(ctx):
     It, I, arg, vt, vs = _pipeline_case(6)
     with pytest.raises(tipl_amd.TiplHipError):
         tipl_amd.register(ctx, It, I, nonlinear=True, arg=arg, outputs=("from2to",))
+
+
+def test_register_files_equals_arrays(ctx, tmp_path):
+    """Real-data entry: the same pair written as gzip NIfTI (one of them in
+    RAS orientation with anisotropic voxels) registers exactly as the arrays
+    the reader returns."""
+    import tipl_amd
+    from tipl_amd import phantoms
+    f, m, _ = phantoms.serving_pair(4200, (40, 44, 36), True, np.float32)
+    tipl_amd.nifti_write(tmp_path / "t.nii.gz", f, (1.0, 1.0, 1.0))
+    tipl_amd.nifti_write(tmp_path / "s.nii.gz", m[:, ::-1, ::-1].copy(), (1.2, 1.0, 1.1),
+                         srow=np.array([1.2, 0, 0, -10, 0, 1.0, 0, -20, 0, 0, 1.1, -5], np.float32))
+    res = tipl_amd.register_files(ctx, tmp_path / "t.nii.gz", tmp_path / "s.nii.gz", nonlinear=True,
+                                  reverse=True, outputs=("warped",))
+    ff, fi = tipl_amd.nifti_read(tmp_path / "t.nii.gz")
+    mm, mi = tipl_amd.nifti_read(tmp_path / "s.nii.gz")
+    assert np.array_equal(mm, m) and mi["flip_swap"] == [0, 1]
+    want = tipl_amd.register(ctx, ff, mm, fixed_vs=tuple(fi["vs"]), moving_vs=tuple(mi["vs"]),
+                             preprocess="mm_reg", nonlinear=True, reverse=True, outputs=("warped",))
+    assert res["status"] == 0 and np.array_equal(res["T"], want["T"])
+    assert np.array_equal(res["field"], want["field"]) and np.array_equal(res["warped"], want["warped"])
+    assert res["r_nonlinear"] == want["r_nonlinear"]

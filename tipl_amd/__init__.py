@@ -40,7 +40,7 @@ EXPORTS = (
     "tiplhip_autotune", "tiplhip_displacement_to_mapping", "tiplhip_mapping_to_displacement",
     "tiplhip_inv_displacement_to_mapping", "tiplhip_compose_mapping",
     "tiplhip_jacobian_determinant_dis", "tiplhip_jacobian_determinant", "tiplhip_match_fov",
-    "tiplhip_correlation",
+    "tiplhip_correlation", "tiplhip_nifti_read_info", "tiplhip_nifti_read", "tiplhip_nifti_write",
 )
 
 
@@ -81,6 +81,14 @@ class RegResult(C.Structure):
                 ("queue_ms", C.c_double), ("run_ms", C.c_double), ("prep_ms", C.c_double),
                 ("affine_ms", C.c_double), ("cdm_ms", C.c_double), ("r_linear", C.c_float),
                 ("r_nonlinear", C.c_float)]
+
+
+class NiftiInfo(C.Structure):
+    _fields_ = [("dim", C.c_int * 8), ("pixdim", C.c_float * 8), ("srow", C.c_float * 12),
+                ("datatype", C.c_int), ("bitpix", C.c_int), ("scl_slope", C.c_float),
+                ("scl_inter", C.c_float), ("sform_code", C.c_int), ("qform_code", C.c_int),
+                ("is_mni", C.c_int), ("version", C.c_int), ("big_endian", C.c_int),
+                ("flip_swap", C.c_int * 6), ("flip_swap_n", C.c_int), ("descrip", C.c_char * 80)]
 
 
 class ServerStats(C.Structure):
@@ -160,6 +168,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "tiplhip_jacobian_determinant": (ip, [vp, vp, P(ip), vp]),
         "tiplhip_match_fov": (ip, [vp, vp, vp, P(ip)]),
         "tiplhip_correlation": (ip, [vp, vp, vp, ip, P(C.c_double)]),
+        "tiplhip_nifti_read_info": (ip, [C.c_char_p, P(NiftiInfo)]),
+        "tiplhip_nifti_read": (ip, [C.c_char_p, ip, ip, vp, P(NiftiInfo)]),
+        "tiplhip_nifti_write": (ip, [C.c_char_p, vp, ip, P(ip), vp, vp, ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -657,3 +668,60 @@ class Server:
 
     def __exit__(self, *exc):
         self.close()
+
+
+# -- NIfTI files (host) ---------------------------------------------------------
+def _info_dict(info: NiftiInfo) -> dict:
+    return {"dim": list(info.dim), "vs": np.array(info.pixdim[1:4], np.float32),
+            "srow": np.array(info.srow[:], np.float32), "datatype": info.datatype,
+            "bitpix": info.bitpix, "scl_slope": info.scl_slope, "scl_inter": info.scl_inter,
+            "sform_code": info.sform_code, "qform_code": info.qform_code,
+            "is_mni": bool(info.is_mni), "version": info.version, "big_endian": bool(info.big_endian),
+            "flip_swap": list(info.flip_swap[:info.flip_swap_n]),
+            "descrip": info.descrip.decode(errors="replace")}
+
+
+def nifti_info(path) -> dict:
+    """Header of a NIfTI / Analyze file after the reference's LPS reorientation."""
+    info = NiftiInfo()
+    _check(load_library().tiplhip_nifti_read_info(os.fsencode(str(path)), C.byref(info)))
+    return _info_dict(info)
+
+
+def nifti_read(path, volume: int = 0, dtype=np.float32):
+    """One volume as a (z, y, x) array in LPS orientation (tipl::io::nifti's
+    `>>`), and the header; dtype uint8, int16 or float32."""
+    info = NiftiInfo()
+    lib = load_library()
+    _check(lib.tiplhip_nifti_read_info(os.fsencode(str(path)), C.byref(info)))
+    dt = np.dtype(dtype)
+    code = {np.dtype(np.uint8): U8, np.dtype(np.int16): I16, np.dtype(np.float32): F32}[dt]
+    out = np.empty((info.dim[3], info.dim[2], info.dim[1]), dt)
+    _check(lib.tiplhip_nifti_read(os.fsencode(str(path)), int(volume), code, _ptr(out), C.byref(info)))
+    return out, _info_dict(info)
+
+
+def nifti_write(path, data: np.ndarray, vs=(1.0, 1.0, 1.0), srow=None, is_mni: bool = False):
+    """Writes a (z, y, x) uint8 / int16 / float32 volume as NIfTI-1 (gzip for .gz)."""
+    data = np.ascontiguousarray(data)
+    code = {np.dtype(np.uint8): U8, np.dtype(np.int16): I16, np.dtype(np.float32): F32}[data.dtype]
+    if srow is None:
+        srow = np.array([vs[0], 0, 0, 0, 0, vs[1], 0, 0, 0, 0, vs[2], 0], np.float32)
+    v = np.ascontiguousarray(vs, np.float32)
+    r = np.ascontiguousarray(srow, np.float32)
+    _check(load_library().tiplhip_nifti_write(os.fsencode(str(path)), _ptr(data), code,
+                                              _dims(data.shape), _ptr(v), _ptr(r), int(bool(is_mni))))
+
+
+def register_files(ctx: "Context", template_path, subject_path, volume: int = 0, **kw) -> dict:
+    """mm_reg's file entry (reg/mm_reg.hpp:100-152 load_reg_image, then the
+    registration): the template (fixed) and the subject (moving) read from
+    NIfTI files in LPS orientation with their voxel sizes, preprocessed as
+    mm_reg does (template_image_pre / subject_image_pre, on the device), then
+    tiplhip_register.  Extra keywords go to make_request."""
+    fixed, fi = nifti_read(template_path, volume)
+    moving, mi = nifti_read(subject_path, volume)
+    kw.setdefault("preprocess", "mm_reg")
+    res = register(ctx, fixed, moving, fixed_vs=tuple(fi["vs"]), moving_vs=tuple(mi["vs"]), **kw)
+    res["template_info"], res["subject_info"] = fi, mi
+    return res

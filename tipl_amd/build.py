@@ -15,7 +15,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtiplhip.so")
-SOURCES = ["api.hip", "affine.hip", "cdm.hip", "filter.hip", "prep.hip", "warp.hip", "server.cpp"]
+SOURCES = ["api.hip", "affine.hip", "cdm.hip", "filter.hip", "prep.hip", "warp.hip", "server.cpp", "nifti.cpp"]
 
 # -ffp-contract=off: the trilinear paths must round every FP32 operation in
 # source order to reproduce the reference's uint8 truncation bit for bit.
@@ -57,7 +57,7 @@ def build(verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
