@@ -468,6 +468,12 @@ typedef struct tiplhip_server_stats {
 
 int tiplhip_server_create(int device, int slots, size_t hbm_budget, tiplhip_server** out);
 /* Estimated device bytes a request holds while it runs (what admission counts). */
+/* The same with an explicit number of batch launcher threads (each with its
+ * own stream): -1 picks the default (one per four slots, 1 to 4), 0 runs no
+ * batcher -- every slot then launches its own candidate sets on its own
+ * stream. */
+int tiplhip_server_create_ex(int device, int slots, int launchers, size_t hbm_budget,
+                             tiplhip_server** out);
 size_t tiplhip_request_bytes(const tiplhip_reg_request* req);
 int tiplhip_server_submit(tiplhip_server* s, const tiplhip_reg_request* req, uint64_t* ticket);
 /* Blocks until the ticket is done; returns its status and forgets the ticket. */

@@ -36,7 +36,7 @@ EXPORTS = (
     "tiplhip_gaussian", "tiplhip_cost_batch_pairs", "tiplhip_volume_resample",
     "tiplhip_reg_request_default", "tiplhip_register", "tiplhip_server_create",
     "tiplhip_request_bytes", "tiplhip_server_submit", "tiplhip_server_wait",
-    "tiplhip_server_stats_get", "tiplhip_server_destroy", "tiplhip_volume_create_subject",
+    "tiplhip_server_stats_get", "tiplhip_server_destroy", "tiplhip_server_create_ex", "tiplhip_volume_create_subject",
     "tiplhip_autotune", "tiplhip_displacement_to_mapping", "tiplhip_mapping_to_displacement",
     "tiplhip_inv_displacement_to_mapping", "tiplhip_compose_mapping",
     "tiplhip_jacobian_determinant_dis", "tiplhip_jacobian_determinant", "tiplhip_match_fov",
@@ -153,6 +153,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "tiplhip_reg_request_default": (None, [P(RegRequest)]),
         "tiplhip_register": (ip, [vp, P(RegRequest), P(RegResult)]),
         "tiplhip_server_create": (ip, [ip, ip, C.c_size_t, P(vp)]),
+        "tiplhip_server_create_ex": (ip, [ip, ip, ip, C.c_size_t, P(vp)]),
         "tiplhip_request_bytes": (C.c_size_t, [P(RegRequest)]),
         "tiplhip_server_submit": (ip, [vp, P(RegRequest), P(C.c_uint64)]),
         "tiplhip_server_wait": (ip, [vp, C.c_uint64, P(RegResult)]),
@@ -633,9 +634,10 @@ def request_bytes(fixed, moving, **kw) -> int:
 class Server:
     """The serving loop (tiplhip_server_*): slots, HBM-budget admission, one cost batcher."""
 
-    def __init__(self, device: int = 0, slots: int = 8, hbm_budget: int = 64 << 30):
+    def __init__(self, device: int = 0, slots: int = 8, hbm_budget: int = 64 << 30, launchers: int = -1):
         h = C.c_void_p()
-        _check(load_library().tiplhip_server_create(device, slots, C.c_size_t(hbm_budget), C.byref(h)))
+        _check(load_library().tiplhip_server_create_ex(device, slots, launchers, C.c_size_t(hbm_budget),
+                                                       C.byref(h)))
         self.handle = h.value
         self._keep = {}
 

@@ -390,11 +390,16 @@ struct tiplhip_server {
     void worker(tipl::hip::context& ctx)
     {
         using namespace tiplhip;
+        // with a batcher, candidate sets of all slots are merged into shared
+        // launches; without one, each slot launches its own on its stream
         tipl::reg::detail::cost_evaluator eval =
-            [this](const tiplhip_volume* from, const float* fvs, const tiplhip_volume* to, const float* tvs,
-                   const float* params, int K, int cost, int bins, double* costs) {
-                batcher->evaluate(from, fvs, to, tvs, params, K, cost, bins, costs);
-            };
+            batcher ? tipl::reg::detail::cost_evaluator(
+                          [this](const tiplhip_volume* from, const float* fvs, const tiplhip_volume* to,
+                                 const float* tvs, const float* params, int K, int cost, int bins,
+                                 double* costs) {
+                              batcher->evaluate(from, fvs, to, tvs, params, K, cost, bins, costs);
+                          })
+                    : tipl::reg::detail::direct_evaluator(ctx);
         for (;;) {
             Ticket* t = nullptr;
             {
@@ -491,10 +496,17 @@ int tiplhip_register(tiplhip_ctx* ctx, const tiplhip_reg_request* req, tiplhip_r
 
 int tiplhip_server_create(int device, int slots, size_t hbm_budget, tiplhip_server** out)
 {
+    return tiplhip_server_create_ex(device, slots, -1, hbm_budget, out);
+}
+
+int tiplhip_server_create_ex(int device, int slots, int launchers, size_t hbm_budget,
+                             tiplhip_server** out)
+{
     return guarded([&] {
         TIPLHIP_REQUIRE(out != nullptr, "null output");
         *out = nullptr;
         TIPLHIP_REQUIRE(slots >= 1 && slots <= 64, "slots must be in [1, 64]");
+        TIPLHIP_REQUIRE(launchers >= -1 && launchers <= 16, "launchers must be in [-1, 16]");
         TIPLHIP_REQUIRE(hbm_budget > 0, "zero memory budget");
         int n = 0;
         rc_check(tiplhip_device_count(&n));
@@ -502,8 +514,9 @@ int tiplhip_server_create(int device, int slots, size_t hbm_budget, tiplhip_serv
         auto s = std::make_unique<tiplhip_server>();
         s->device = device;
         s->budget = hbm_budget;
-        // one launcher per 4 slots (at least 1, at most 4)
-        s->batcher = std::make_unique<Batcher>(device, std::min(4, std::max(1, slots / 4)));
+        // default: one launcher per 4 slots (at least 1, at most 4); 0: no batcher
+        if (launchers < 0) launchers = std::min(4, std::max(1, slots / 4));
+        if (launchers > 0) s->batcher = std::make_unique<Batcher>(device, launchers);
         for (int i = 0; i < slots; ++i) s->contexts.push_back(std::make_unique<tipl::hip::context>(device));
         for (int i = 0; i < slots; ++i)
             s->workers.emplace_back([p = s.get(), c = s->contexts[size_t(i)].get()] { p->worker(*c); });
@@ -570,7 +583,7 @@ int tiplhip_server_stats_get(tiplhip_server* s, tiplhip_server_stats* st)
             lat = s->latencies;
             if (s->completed > 0) st->wall_s = ms_between(s->first_submit, s->last_done) * 1e-3;
         }
-        s->batcher->counts(st->batch_launches, st->batched_pairs);
+        if (s->batcher) s->batcher->counts(st->batch_launches, st->batched_pairs);
         if (!lat.empty()) {
             std::sort(lat.begin(), lat.end());
             // nearest-rank percentiles
