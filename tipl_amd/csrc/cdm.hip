@@ -214,39 +214,78 @@ __device__ inline void regress_and_store(const uint8_t* __restrict__ Js, const F
     }
 }
 
+constexpr int kStage = (kPX * kPY + kTX * kTY - 1) / (kTX * kTY);  // staged bytes per thread
+
 __global__ __launch_bounds__(256) void gradient_r_box_kernel(const uint8_t* __restrict__ Js,
                                                              const uint8_t* __restrict__ It,
                                                              FieldGeo g, int accumulate,
                                                              float* __restrict__ nd,
-                                                             double* __restrict__ partials)
+                                                             double* __restrict__ partials, int zc)
 {
     __shared__ uint8_t pj[kPY][kPX], pi[kPY][kPX];
     __shared__ Sums5 xs[kPY][kTX];
     const int tx = int(threadIdx.x), ty = int(threadIdx.y);
     const int tid = ty * kTX + tx;
     const int x0 = int(blockIdx.x) * kTX, y0 = int(blockIdx.y) * kTY;
-    const int z0 = int(blockIdx.z) * kZC, z1 = min(z0 + kZC, g.d);
+    const int z0 = int(blockIdx.z) * zc, z1 = min(z0 + zc, g.d);
     Sums5 ring[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) ring[k] = Sums5{0u, 0u, 0u, 0u, 0u};
     float cost_acc = 0.0f;
     double cost_sum = 0.0;
+    // This thread's staging slots (plane-independent): LDS cell c = tid + 256 k
+    // of the 2-voxel-haloed tile, its in-plane offset, and whether it lies
+    // inside the volume (zero outside).  The next plane's bytes are loaded
+    // into registers while the current plane is summed.
+    uint32_t soff[kStage];
+    bool sin[kStage];
+#pragma unroll
+    for (int k = 0; k < kStage; ++k) {
+        const int c = tid + k * kTX * kTY;
+        const int cy = c / kPX, cx = c - cy * kPX;
+        const int gx = x0 - 2 + cx, gy = y0 - 2 + cy;
+        sin[k] = c < kPX * kPY && gx >= 0 && gx < g.w && gy >= 0 && gy < g.h;
+        soff[k] = sin[k] ? uint32_t(gy) * uint32_t(g.w) + uint32_t(gx) : 0u;
+    }
+    uint8_t na[kStage], nb[kStage];
+    {
+        const int pz = z0 - 2;
+#pragma unroll
+        for (int k = 0; k < kStage; ++k) {
+            const bool in = sin[k] && pz >= 0 && pz < g.d;
+            const uint32_t q = uint32_t(in ? pz : 0) * g.wh + soff[k];
+            na[k] = in ? Js[q] : uint8_t(0);
+            nb[k] = in ? It[q] : uint8_t(0);
+        }
+    }
     for (int pz = z0 - 2; pz < z1 + 2; ++pz) {
         Sums5 P{0u, 0u, 0u, 0u, 0u};
-        if (pz >= 0 && pz < g.d) {
+        const bool plane_in = pz >= 0 && pz < g.d;
+        if (plane_in) {
             // stage the plane with a 2-voxel halo (zero outside the volume)
-            for (int c = tid; c < kPX * kPY; c += kTX * kTY) {
-                const int cy = c / kPX, cx = c - cy * kPX;
-                const int gx = x0 - 2 + cx, gy = y0 - 2 + cy;
-                uint8_t a = 0, b = 0;
-                if (gx >= 0 && gx < g.w && gy >= 0 && gy < g.h) {
-                    const uint32_t q = uint32_t(pz) * g.wh + uint32_t(gy) * uint32_t(g.w) + uint32_t(gx);
-                    a = Js[q];
-                    b = It[q];
+#pragma unroll
+            for (int k = 0; k < kStage; ++k) {
+                const int c = tid + k * kTX * kTY;
+                if (c < kPX * kPY) {
+                    const int cy = c / kPX, cx = c - cy * kPX;
+                    pj[cy][cx] = na[k];
+                    pi[cy][cx] = nb[k];
                 }
-                pj[cy][cx] = a;
-                pi[cy][cx] = b;
             }
+        }
+        {
+            const int nz = pz + 1;
+            if (nz < z1 + 2) {
+#pragma unroll
+                for (int k = 0; k < kStage; ++k) {
+                    const bool in = sin[k] && nz >= 0 && nz < g.d;
+                    const uint32_t q = uint32_t(in ? nz : 0) * g.wh + soff[k];
+                    na[k] = in ? Js[q] : uint8_t(0);
+                    nb[k] = in ? It[q] : uint8_t(0);
+                }
+            }
+        }
+        if (plane_in) {
             __syncthreads();
             // x sums for the 8 staged rows (2 per thread)
             for (int r = ty; r < kPY; r += kTY) {
@@ -312,10 +351,20 @@ __global__ __launch_bounds__(256) void gradient_r_box_kernel(const uint8_t* __re
     }
 }
 
+// Planes per block of the box-sum gradient: kZC for large volumes; on
+// small pyramid levels fewer (down to 4), so about 1024 blocks share the
+// work instead of a few blocks marching long z chains.
+int box_zc(const FieldGeo& g)
+{
+    const long tiles = long((g.w + kTX - 1) / kTX) * long((g.h + kTY - 1) / kTY);
+    return int(std::min<long>(kZC, std::max<long>(4, long(g.d) * tiles / 1024)));
+}
+
 dim3 box_grid(const FieldGeo& g)
 {
+    const int zc = box_zc(g);
     return dim3(uint32_t((g.w + kTX - 1) / kTX), uint32_t((g.h + kTY - 1) / kTY),
-                uint32_t((g.d + kZC - 1) / kZC));
+                uint32_t((g.d + zc - 1) / zc));
 }
 
 // Sums the block partials in a fixed order and stores float(sum / n).
@@ -837,7 +886,7 @@ void launch_gradient(tiplhip_ctx* ctx, const uint8_t* Js, const uint8_t* It, con
     } else {
         const dim3 bg = box_grid(g);
         hipLaunchKernelGGL(gradient_r_box_kernel, bg, dim3(kTX, kTY), 0, ctx->stream, Js, It, g,
-                           accumulate, nd, partials);
+                           accumulate, nd, partials, box_zc(g));
         parts = bg.x * bg.y * bg.z;
     }
     TIPLHIP_CHECK(hipGetLastError());
