@@ -807,7 +807,7 @@ void launch_resample(tiplhip_ctx* ctx, const tiplhip_volume* src, const int* out
 // launch); costs[] receives them job after job.  T holds the jobs'
 // transforms in the same order.
 void cost_jobs(tiplhip_ctx* ctx, std::vector<PairJob> jobs, const float* T, int cost, int bins,
-               double* costs)
+               double* costs, const LaunchShape* force = nullptr)
 {
     TIPLHIP_REQUIRE(!jobs.empty(), "no image pairs");
     TIPLHIP_REQUIRE(T != nullptr && costs != nullptr, "null transform or cost buffer");
@@ -846,7 +846,7 @@ void cost_jobs(tiplhip_ctx* ctx, std::vector<PairJob> jobs, const float* T, int 
     }
     for (const PairJob& j : jobs) volume_ensure_packed(ctx, const_cast<tiplhip_volume*>(j.to));
     const Staged st = stage_jobs(ctx, jobs, Tk.data(), crop.empty() ? nullptr : crop.data(), cost,
-                                 bins);
+                                 bins, force);
     if (cost == TIPLHIP_COST_CORR) {
         launch_corr_sums(ctx, st);
         launch_corr_finalize(ctx, st, nullptr);
@@ -1051,7 +1051,12 @@ int tiplhip_cost_batch_pairs(tiplhip_ctx* ctx, const tiplhip_pair_costs* pairs, 
             T.insert(T.end(), p.T, p.T + 12 * size_t(p.K));
         }
         std::vector<double> costs(K);
-        cost_jobs(ctx, jobs, T.data(), cost, bins, costs.data());
+        // Multi-pair launches serve throughput (a batcher merging the candidate
+        // sets of many registrations): 256-bin histograms then use the fewest,
+        // fullest workgroups, which spend the least on clearing and flushing
+        // their 128 KB LDS histograms per sample.
+        const LaunchShape fullest{16u, 0u};
+        cost_jobs(ctx, jobs, T.data(), cost, bins, costs.data(), hist256(cost, bins) ? &fullest : nullptr);
         size_t base = 0;
         for (int i = 0; i < n; ++i) {
             std::copy_n(costs.begin() + std::ptrdiff_t(base), pairs[i].K, pairs[i].costs);

@@ -547,6 +547,7 @@ int tiplhip_ctx_create(int device, tiplhip_ctx** out)
             TIPLHIP_CHECK(hipEventCreate(&c->t1));
             TIPLHIP_CHECK(hipEventCreate(&c->k0));
             TIPLHIP_CHECK(hipEventCreate(&c->k1));
+            TIPLHIP_CHECK(hipEventCreateWithFlags(&c->e_read, hipEventDisableTiming));
         } catch (...) {
             if (c->stream) (void)hipStreamDestroy(c->stream);
             delete c;
@@ -564,7 +565,7 @@ int tiplhip_ctx_destroy(tiplhip_ctx* ctx)
         {
             DeviceGuard dg(ctx->device);
             (void)hipStreamSynchronize(ctx->stream);
-            for (hipEvent_t e : {ctx->t0, ctx->t1, ctx->k0, ctx->k1})
+            for (hipEvent_t e : {ctx->t0, ctx->t1, ctx->k0, ctx->k1, ctx->e_read})
                 if (e) (void)hipEventDestroy(e);
             ctx->d_T.release();
             ctx->d_hist.release();

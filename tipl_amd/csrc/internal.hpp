@@ -170,6 +170,7 @@ struct tiplhip_ctx {
     std::map<tiplhip::TuneKey, tiplhip::LaunchShape> tuned;
     // timing
     hipEvent_t t0 = nullptr, t1 = nullptr, k0 = nullptr, k1 = nullptr;
+    hipEvent_t e_read = nullptr;     // marks a device-to-host copy the host waits for
     bool kernel_timing = false;
     tiplhip_ctx* aux = nullptr;      // see aux_context
     double kernel_ms = 0.0;
