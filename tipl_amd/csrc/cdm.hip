@@ -743,202 +743,6 @@ __global__ __launch_bounds__(kQX * kQY) void poisson2_kernel(const float* __rest
 }
 
 // ---------------------------------------------------------------------------
-// Three Jacobi sweeps per launch (s_t -> s_t+1 -> s_t+2 -> s_t+3), the same
-// flat-index tiling and ring scheme one level deeper: per plane m the block
-// holds s_t on the tile grown by 3 (S ring), computes s_t+1 of plane m on
-// the tile grown by 2 (U ring), s_t+2 of plane m-1 on the tile grown by 1
-// (T ring) and s_t+3 of plane m-2 on the tile.  Thread (tx, ty) computes its
-// own voxel at every level; threads 0..147 also the inner ring (tile+1) of
-// the U and T levels, threads 0..155 the outer ring (tile+2) of the U level.
-// The new_d values a thread needs are loaded one plane ahead and kept in
-// registers for the two planes its later levels lag behind.  HBM traffic per
-// three sweeps is 36 B per voxel (new_d and s_t read, s_t+3 written): twelve
-// sweeps take four launches instead of six.
-// ---------------------------------------------------------------------------
-constexpr int kS3X = kQX + 6, kS3Y = kQY + 6;   // tile grown by 3
-constexpr int kS3N = kS3X * kS3Y;               // 980
-constexpr int kRing2 = kS2X * kS2Y - kS1X * kS1Y;  // 156 positions between tile+1 and tile+2
-
-// offset (i, j) of outer ring position r (-2 <= i <= kQX+1, -2 <= j <= kQY+1, outside tile+1)
-__device__ inline void ring2_offset(int r, int& i, int& j)
-{
-    if (r < kS2X) { i = r - 2; j = -2; }
-    else if (r < 2 * kS2X) { i = r - kS2X - 2; j = kQY + 1; }
-    else if (r < 2 * kS2X + kS1Y) { i = -2; j = r - 2 * kS2X - 1; }
-    else { i = kQX + 1; j = r - 2 * kS2X - kS1Y - 1; }
-}
-
-// planes per workgroup of the three-sweep kernel: about 512 workgroups, 3..16 planes
-int poisson3_zq(const FieldGeo& g)
-{
-    const long long tiles = (long long)((g.w + kQX - 1) / kQX) * ((g.h + kQY - 1) / kQY);
-    const long long want = std::max<long long>(1, (tiles * g.d + 511) / 512);
-    return int(std::min<long long>(kQZMax, std::max<long long>(3, want)));
-}
-
-template <bool FIRST, bool MAXLEN>
-__global__ __launch_bounds__(kQX * kQY) void poisson3_kernel(const float* __restrict__ nd,
-                                                             const float* __restrict__ a,
-                                                             FieldGeo g, int zq,
-                                                             float* __restrict__ b,
-                                                             float* __restrict__ maxparts)
-{
-    __shared__ float3 S[3][kS3Y][kS3X];
-    __shared__ float3 U[3][kS2Y][kS2X];
-    __shared__ float3 T[3][kS1Y][kS1X];
-    const int tx = int(threadIdx.x), ty = int(threadIdx.y), tid = ty * kQX + tx;
-    const int x0 = int(blockIdx.x) * kQX, y0 = int(blockIdx.y) * kQY;
-    const int z0 = int(blockIdx.z) * zq, z1 = min(z0 + zq, g.d);
-    const long long n = g.n, w = g.w, wh = g.wh;
-    const long long base = (long long)y0 * w + x0;
-    const float3 zero = make_float3(0.0f, 0.0f, 0.0f);
-
-    // this thread's S positions (two at most)
-    const int e0 = tid, e1 = tid + kQX * kQY;
-    const long long so0 = (long long)(e0 / kS3X - 3) * w + (e0 % kS3X - 3);
-    const long long so1 = (long long)(e1 / kS3X - 3) * w + (e1 % kS3X - 3);
-    const bool has1 = e1 < kS3N;
-    // own voxel, inner ring (tile+1) and outer ring (tile+2) positions
-    int ri = 0, rj = 0, qi = 0, qj = 0;
-    const bool has_r1 = tid < kRing, has_r2 = tid < kRing2;
-    if (has_r1) ring_offset(tid, ri, rj);
-    if (has_r2) ring2_offset(tid, qi, qj);
-    const long long io = (long long)ty * w + tx;
-    const long long r1o = (long long)rj * w + ri;
-    const long long r2o = (long long)qj * w + qi;
-
-    auto load_s = [&](long long q) -> float3 {
-        if (q < 0 || q >= n) return make_float3(0.0f, 0.0f, 0.0f);
-        const float3 t = ld3(FIRST ? nd : a, uint32_t(q));
-        if (FIRST) return make_float3(t.x * kInvD2, t.y * kInvD2, t.z * kInvD2);
-        return t;
-    };
-    auto load_r = [&](long long q) -> float3 {
-        if (q < 0 || q >= n) return make_float3(0.0f, 0.0f, 0.0f);
-        return ld3(nd, uint32_t(q));
-    };
-    auto valid = [&](long long q) { return q >= 0 && q < n; };
-    auto update = [](float3 r, float3 xm, float3 xp, float3 ym, float3 yp, float3 zm, float3 zp) {
-        r.x = r.x - xm.x; r.y = r.y - xm.y; r.z = r.z - xm.z;
-        r.x = r.x - xp.x; r.y = r.y - xp.y; r.z = r.z - xp.z;
-        r.x = r.x - ym.x; r.y = r.y - ym.y; r.z = r.z - ym.z;
-        r.x = r.x - yp.x; r.y = r.y - yp.y; r.z = r.z - yp.z;
-        r.x = r.x - zm.x; r.y = r.y - zm.y; r.z = r.z - zm.z;
-        r.x = r.x - zp.x; r.y = r.y - zp.y; r.z = r.z - zp.z;
-        return make_float3(r.x * kInvD2, r.y * kInvD2, r.z * kInvD2);
-    };
-    // ring slot of plane m (m >= z0 - 4 > -6)
-    auto slot = [](int m) { return (m + 6) % 3; };
-    auto store_s = [&](int m, float3 v0, float3 v1) {
-        float3(*dst)[kS3X] = S[slot(m)];
-        dst[e0 / kS3X][e0 % kS3X] = v0;
-        if (has1) dst[e1 / kS3X][e1 % kS3X] = v1;
-    };
-
-    // prologue: S planes z0-3, z0-2 in LDS; S(z0-1) and the new_d of plane z0-2 in registers
-    {
-        const long long p = (long long)(z0 - 3) * wh + base;
-        store_s(z0 - 3, load_s(p + so0), has1 ? load_s(p + so1) : zero);
-        const long long p1 = p + wh;
-        store_s(z0 - 2, load_s(p1 + so0), has1 ? load_s(p1 + so1) : zero);
-    }
-    long long pz = (long long)(z0 - 1) * wh + base;  // plane m+1 during iteration m
-    float3 s0 = load_s(pz + so0), s1 = has1 ? load_s(pz + so1) : zero;
-    const long long pm = pz - wh;  // plane z0-2
-    float3 r_in = load_r(pm + io), r_r1 = has_r1 ? load_r(pm + r1o) : zero,
-           r_r2 = has_r2 ? load_r(pm + r2o) : zero;
-    bool v_in = valid(pm + io), v_r1 = has_r1 && valid(pm + r1o), v_r2 = has_r2 && valid(pm + r2o);
-    // new_d of earlier planes for the lagging levels: own voxel at m-1 and
-    // m-2, inner ring at m-1
-    float3 in_m1 = zero, in_m2 = zero, r1_m1 = zero;
-    bool vin_m1 = false, vr1_m1 = false;
-    float len_max = 0.0f;
-
-    for (int m = z0 - 2; m <= z1 + 1; ++m) {
-        store_s(m + 1, s0, s1);
-        const float3 c_in = r_in, c_r1 = r_r1, c_r2 = r_r2;
-        const bool cv_in = v_in, cv_r1 = v_r1, cv_r2 = v_r2;
-        pz += wh;  // plane m+2
-        s0 = load_s(pz + so0);
-        s1 = has1 ? load_s(pz + so1) : zero;
-        const long long pn = pz - wh;  // plane m+1
-        r_in = load_r(pn + io);
-        r_r1 = has_r1 ? load_r(pn + r1o) : zero;
-        r_r2 = has_r2 ? load_r(pn + r2o) : zero;
-        v_in = valid(pn + io);
-        v_r1 = has_r1 && valid(pn + r1o);
-        v_r2 = has_r2 && valid(pn + r2o);
-        __syncthreads();
-        // s_t+1 of plane m on tile+2
-        {
-            float3(*sm)[kS3X] = S[slot(m - 1)];
-            float3(*sc)[kS3X] = S[slot(m)];
-            float3(*sp)[kS3X] = S[slot(m + 1)];
-            float3(*dst)[kS2X] = U[slot(m)];
-            auto at = [&](int i, int j, float3 r, bool v) {
-                const int ci = i + 3, cj = j + 3;
-                dst[j + 2][i + 2] = v ? update(r, sc[cj][ci - 1], sc[cj][ci + 1], sc[cj - 1][ci],
-                                               sc[cj + 1][ci], sm[cj][ci], sp[cj][ci])
-                                      : zero;
-            };
-            at(tx, ty, c_in, cv_in);
-            if (has_r1) at(ri, rj, c_r1, cv_r1);
-            if (has_r2) at(qi, qj, c_r2, cv_r2);
-        }
-        __syncthreads();
-        // s_t+2 of plane m-1 on tile+1
-        {
-            const int zz = m - 1;
-            float3(*um)[kS2X] = U[slot(zz - 1)];
-            float3(*uc)[kS2X] = U[slot(zz)];
-            float3(*up)[kS2X] = U[slot(zz + 1)];
-            float3(*dst)[kS1X] = T[slot(zz)];
-            auto at = [&](int i, int j, float3 r, bool v) {
-                const int ci = i + 2, cj = j + 2;
-                dst[j + 1][i + 1] = v ? update(r, uc[cj][ci - 1], uc[cj][ci + 1], uc[cj - 1][ci],
-                                               uc[cj + 1][ci], um[cj][ci], up[cj][ci])
-                                      : zero;
-            };
-            at(tx, ty, in_m1, vin_m1);
-            if (has_r1) at(ri, rj, r1_m1, vr1_m1);
-        }
-        __syncthreads();
-        // s_t+3 of plane m-2 at this thread's voxel
-        const int z = m - 2;
-        if (z >= z0 && z < z1 && x0 + tx < g.w && y0 + ty < g.h) {
-            float3(*tm)[kS1X] = T[slot(z - 1)];
-            float3(*tc)[kS1X] = T[slot(z)];
-            float3(*tp)[kS1X] = T[slot(z + 1)];
-            const int cj = ty + 1, ci = tx + 1;
-            const float3 r = update(in_m2, tc[cj][ci - 1], tc[cj][ci + 1], tc[cj - 1][ci],
-                                    tc[cj + 1][ci], tm[cj][ci], tp[cj][ci]);
-            st3(b, uint32_t((long long)z * wh + base + io), r);
-            if (MAXLEN) {
-                float s2 = r.x * r.x + r.y * r.y;
-                s2 = s2 + r.z * r.z;
-                len_max = fmaxf(len_max, float(sqrt(double(s2))));
-            }
-        }
-        in_m2 = in_m1;
-        in_m1 = c_in;
-        vin_m1 = cv_in;
-        r1_m1 = c_r1;
-        vr1_m1 = cv_r1;
-    }
-    if (MAXLEN) {
-        __shared__ float part[kQY];
-        const float mx = wave_max_f32(len_max);
-        if ((tid & 63) == 0) part[tid >> 6] = mx;
-        __syncthreads();
-        if (tid == 0) {
-            float r = part[0];
-            for (int i = 1; i < kQY; ++i) r = fmaxf(r, part[i]);
-            maxparts[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = r;
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
 // new_d *= k, then accumulate_displacement(cur, new_d) in place:
 // new(x) = m(x + new(x)) - x with m(y) = cur(y) + y by trilinear
 // interpolation; cur(x) where new(x) == 0 or the sample is outside.
@@ -1104,27 +908,9 @@ float* launch_poisson(tiplhip_ctx* ctx, const float* nd, float* a, float* b, con
     int done = 0;
     while (done < sweeps) {
         const bool first = done == 0;
-        const bool triple = sweeps - done >= 3;
-        const bool pair = !triple && sweeps - done == 2;
-        const bool last = done + (triple ? 3 : pair ? 2 : 1) == sweeps && maxlen;
-        if (triple) {
-            const int zq = poisson3_zq(g);
-            const dim3 grid = poisson2_grid(g, zq), block(kQX, kQY);
-            if (first && last)
-                hipLaunchKernelGGL((poisson3_kernel<true, true>), grid, block, 0, ctx->stream, nd, src,
-                                   g, zq, dst, maxparts);
-            else if (first)
-                hipLaunchKernelGGL((poisson3_kernel<true, false>), grid, block, 0, ctx->stream, nd,
-                                   src, g, zq, dst, maxparts);
-            else if (last)
-                hipLaunchKernelGGL((poisson3_kernel<false, true>), grid, block, 0, ctx->stream, nd,
-                                   src, g, zq, dst, maxparts);
-            else
-                hipLaunchKernelGGL((poisson3_kernel<false, false>), grid, block, 0, ctx->stream, nd,
-                                   src, g, zq, dst, maxparts);
-            last_blocks = grid.x * grid.y * grid.z;
-            done += 3;
-        } else if (pair) {
+        const bool pair = sweeps - done >= 2;
+        const bool last = done + (pair ? 2 : 1) == sweeps && maxlen;
+        if (pair) {
             const int zq = poisson2_zq(g);
             const dim3 grid = poisson2_grid(g, zq), block(kQX, kQY);
             if (first && last)
