@@ -359,13 +359,7 @@ __global__ __launch_bounds__(MiTraits<BINS>::kThreads) void mi_hist_kernel(
         if (!m.ok) continue;
         const ColumnSums cs = column_sums(T, float(m.x), float(m.y));
         uint32_t fidx = m.z0 * fwh + m.y * uint32_t(g.fw) + m.x;
-        for (uint32_t z = m.z0; z < m.z1; ++z, fidx += fwh) {
-            const uint32_t fv = load_u8(rf, fidx, 0);
-            const float fz = float(z);
-            const float px = (cs.hx + fz * T[2]) + T[9];
-            const float py = (cs.hy + fz * T[5]) + T[10];
-            const float pz = (cs.hz + fz * T[8]) + T[11];
-            const uint32_t tv = sample_u8_packed(r8, g.to, px, py, pz);
+        auto count = [&](uint32_t fv, uint32_t tv) {
             const uint32_t bin = ((fv >> Tr::kShift) << Tr::kBand) | (tv >> Tr::kShift);
             if constexpr (Tr::kAggregate) {
                 const uint32_t b0 = __builtin_amdgcn_readfirstlane(bin);
@@ -381,6 +375,30 @@ __global__ __launch_bounds__(MiTraits<BINS>::kThreads) void mi_hist_kernel(
             } else {
                 atomicAdd(&lds[Tr::word(bin)], Tr::inc(bin));
             }
+        };
+        uint32_t z = m.z0;
+        // two planes per step: both samples' gathers are in flight together
+        for (; z + 1 < m.z1; z += 2, fidx += 2 * fwh) {
+            const uint32_t fva = load_u8(rf, fidx, 0), fvb = load_u8(rf, fidx + fwh, 0);
+            const float fza = float(z), fzb = float(z + 1);
+            const float pxa = (cs.hx + fza * T[2]) + T[9];
+            const float pya = (cs.hy + fza * T[5]) + T[10];
+            const float pza = (cs.hz + fza * T[8]) + T[11];
+            const float pxb = (cs.hx + fzb * T[2]) + T[9];
+            const float pyb = (cs.hy + fzb * T[5]) + T[10];
+            const float pzb = (cs.hz + fzb * T[8]) + T[11];
+            uint32_t tva, tvb;
+            sample2_u8_packed(r8, g.to, pxa, pya, pza, pxb, pyb, pzb, tva, tvb);
+            count(fva, tva);
+            count(fvb, tvb);
+        }
+        if (z < m.z1) {
+            const uint32_t fv = load_u8(rf, fidx, 0);
+            const float fz = float(z);
+            const float px = (cs.hx + fz * T[2]) + T[9];
+            const float py = (cs.hy + fz * T[5]) + T[10];
+            const float pz = (cs.hz + fz * T[8]) + T[11];
+            count(fv, sample_u8_packed(r8, g.to, px, py, pz));
         }
     }
     __syncthreads();

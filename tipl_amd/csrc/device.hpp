@@ -126,6 +126,54 @@ __device__ inline uint32_t sample_u8_packed(__amdgpu_buffer_rsrc_t r8, const Vol
     return uint32_t(acc);
 }
 
+// Two samples of sample_u8_packed with both cells' loads issued before
+// either is used (two independent chains per thread hide more of the
+// gather latency).  A sample outside the volume reads past the end of the
+// buffer (the raw buffer load returns 0 there) and is forced to 0, so no
+// lane diverges; samples inside compute exactly as sample_u8_packed.
+__device__ inline void sample2_u8_packed(__amdgpu_buffer_rsrc_t r8, const VolGeo& g, float xa,
+                                         float ya, float za, float xb, float yb, float zb,
+                                         uint32_t& ta, uint32_t& tb)
+{
+    const bool ina = (__float_as_uint(xa) < __float_as_uint(g.wm1)) &
+                     (__float_as_uint(ya) < __float_as_uint(g.hm1)) &
+                     (__float_as_uint(za) < __float_as_uint(g.dm1));
+    const bool inb = (__float_as_uint(xb) < __float_as_uint(g.wm1)) &
+                     (__float_as_uint(yb) < __float_as_uint(g.hm1)) &
+                     (__float_as_uint(zb) < __float_as_uint(g.dm1));
+    const uint32_t oa = (__umul24(uint32_t(za), g.wh) + __umul24(uint32_t(ya), uint32_t(g.w)) +
+                         uint32_t(xa)) * 8u;
+    const uint32_t ob = (__umul24(uint32_t(zb), g.wh) + __umul24(uint32_t(yb), uint32_t(g.w)) +
+                         uint32_t(xb)) * 8u;
+    const auto qa = __builtin_amdgcn_raw_buffer_load_b64(r8, int(ina ? oa : 0xFFFFFFF8u), 0, 0);
+    const auto qb = __builtin_amdgcn_raw_buffer_load_b64(r8, int(inb ? ob : 0xFFFFFFF8u), 0, 0);
+    auto blend = [](uint32_t lo, uint32_t hi, float x, float y, float z) {
+        const float p0 = __builtin_amdgcn_fractf(x);
+        const float p1 = __builtin_amdgcn_fractf(y);
+        const float p2 = __builtin_amdgcn_fractf(z);
+        const float n0 = 1.0f - p0;
+        const float n1 = 1.0f - p1;
+        const float n2 = 1.0f - p2;
+        const float n0n1 = n0 * n1;
+        const float p0n1 = p0 * n1;
+        const float n0p1 = n0 * p1;
+        const float p0p1 = p0 * p1;
+        float acc = float(lo & 0xffu) * (n0n1 * n2);
+        acc = acc + float((lo >> 8) & 0xffu) * (p0n1 * n2);
+        acc = acc + float((lo >> 16) & 0xffu) * (n0p1 * n2);
+        acc = acc + float(lo >> 24) * (p0p1 * n2);
+        acc = acc + float(hi & 0xffu) * (n0n1 * p2);
+        acc = acc + float((hi >> 8) & 0xffu) * (p0n1 * p2);
+        acc = acc + float((hi >> 16) & 0xffu) * (n0p1 * p2);
+        acc = acc + float(hi >> 24) * (p0p1 * p2);
+        return uint32_t(acc);
+    };
+    const uint32_t va = blend(qa[0], qa[1], xa, ya, za);
+    const uint32_t vb = blend(qb[0], qb[1], xb, yb, zb);
+    ta = ina ? va : 0u;
+    tb = inb ? vb : 0u;
+}
+
 // Hardware dispatches workgroup b of a 1-D grid to XCD b % 8.  This remap
 // gives each XCD a contiguous range of logical workgroups so that the
 // volume region a range touches stays in that XCD's L2.  Bijective on
