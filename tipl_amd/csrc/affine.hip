@@ -217,26 +217,40 @@ __global__ __launch_bounds__(256) void corr_sums_kernel(const Segment* __restric
         if (!m.ok) continue;
         const ColumnSums cs = column_sums(T, float(m.x), float(m.y));
         uint32_t fidx = m.z0 * fwh + m.y * uint32_t(g.fw) + m.x;
-        for (uint32_t z = m.z0; z < m.z1; ++z, fidx += fwh) {
-            const uint32_t xv = load_u8(rf, fidx, 0);
-            uint32_t yv;
-            if (cr.x) {
-                int sx = int(m.x) + cr.y, s_y = int(m.y) + cr.z, sz = int(z) + cr.w;
-                bool in = sx >= 0 && sx < g.to.w && s_y >= 0 && s_y < g.to.h && sz >= 0 &&
-                          sz < g.to.d;
-                yv = in ? load_u8(rt, uint32_t(sz) * g.to.wh + uint32_t(s_y) * uint32_t(g.to.w) +
-                                          uint32_t(sx), 0)
-                        : 0u;
-            } else {
-                const float fz = float(z);
-                const float px = (cs.hx + fz * T[2]) + T[9];
-                const float py = (cs.hy + fz * T[5]) + T[10];
-                const float pz = (cs.hz + fz * T[8]) + T[11];
-                yv = sample_u8_packed(r8, g.to, px, py, pz);
-            }
+        auto add = [&](uint32_t xv, uint32_t yv) {
             sy += yv;
             syy += yv * yv;
             sxy += xv * yv;
+        };
+        if (cr.x) {
+            for (uint32_t z = m.z0; z < m.z1; ++z, fidx += fwh) {
+                int sx = int(m.x) + cr.y, s_y = int(m.y) + cr.z, sz = int(z) + cr.w;
+                bool in = sx >= 0 && sx < g.to.w && s_y >= 0 && s_y < g.to.h && sz >= 0 &&
+                          sz < g.to.d;
+                add(load_u8(rf, fidx, 0),
+                    in ? load_u8(rt, uint32_t(sz) * g.to.wh + uint32_t(s_y) * uint32_t(g.to.w) +
+                                         uint32_t(sx), 0)
+                       : 0u);
+            }
+            continue;
+        }
+        uint32_t z = m.z0;
+        // two planes per step: both samples' gathers are in flight together
+        for (; z + 1 < m.z1; z += 2, fidx += 2 * fwh) {
+            const uint32_t xa = load_u8(rf, fidx, 0), xb = load_u8(rf, fidx + fwh, 0);
+            const float fza = float(z), fzb = float(z + 1);
+            uint32_t ya, yb;
+            sample2_u8_packed(r8, g.to, (cs.hx + fza * T[2]) + T[9], (cs.hy + fza * T[5]) + T[10],
+                              (cs.hz + fza * T[8]) + T[11], (cs.hx + fzb * T[2]) + T[9],
+                              (cs.hy + fzb * T[5]) + T[10], (cs.hz + fzb * T[8]) + T[11], ya, yb);
+            add(xa, ya);
+            add(xb, yb);
+        }
+        if (z < m.z1) {
+            const float fz = float(z);
+            add(load_u8(rf, fidx, 0), sample_u8_packed(r8, g.to, (cs.hx + fz * T[2]) + T[9],
+                                                       (cs.hy + fz * T[5]) + T[10],
+                                                       (cs.hz + fz * T[8]) + T[11]));
         }
     }
     __shared__ unsigned long long red[3][4];
