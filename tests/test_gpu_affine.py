@@ -118,6 +118,32 @@ def test_mi_cost_batch(ctx, oracle, bins):
     np.testing.assert_allclose(got, want, rtol=COST_RTOL)
 
 
+def test_mi_cost_sequence_reuses_cleared_histograms(ctx, oracle):
+    """The entropy kernels leave the histogram buffer zero for the next
+    launch; launches of growing and shrinking size, both bin counts, and the
+    raw histogram and autotune entry points (which do not clear it) in
+    between must not change any cost."""
+    rng = np.random.default_rng(24)
+    A, B = smooth_volume((18, 22, 26), 25), smooth_volume((20, 20, 24), 26)
+    va, vb = ctx.volume(A), ctx.volume(B)
+    Ts = np.stack([_T(oracle, p, A.shape, B.shape) for p in random_params(rng, 40)])
+    want = {bins: np.array([oracle.mi_cost(A, B, T, bins=bins) for T in Ts]) for bins in (256, 64)}
+    steps = [(256, 3), (256, 40), (64, 7), (256, 1), "hist", (256, 12), (64, 40), "tune",
+             (64, 5), (256, 40), (256, 2)]
+    for step in steps:
+        if step == "hist":
+            ctx.mi_hist(va, vb, Ts[:9], bins=256)
+            continue
+        if step == "tune":
+            ctx.autotune(va, vb, Ts[:6], "mi", 256, reps=1)
+            continue
+        bins, k = step
+        got = ctx.cost_batch(va, vb, Ts[:k], cost="mi", bins=bins)
+        np.testing.assert_allclose(got, want[bins][:k], rtol=COST_RTOL)
+        again = ctx.cost_batch(va, vb, Ts[:k], cost="mi", bins=bins)
+        assert np.array_equal(got, again)
+
+
 def test_batch_equals_single(ctx, oracle):
     rng = np.random.default_rng(17)
     A, B = smooth_volume((24, 20, 28), 18), smooth_volume((24, 20, 28), 19)

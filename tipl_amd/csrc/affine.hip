@@ -272,7 +272,8 @@ __global__ __launch_bounds__(256) void corr_sums_kernel(const Segment* __restric
 // 407-443) in FP64, then `float c; 1.0f - c*c` (reg/linear.hpp:34-35).
 __global__ void corr_finalize_kernel(const Segment* __restrict__ segs, uint32_t nseg,
                                      const unsigned long long* __restrict__ sums, int K,
-                                     double* __restrict__ costs, long long* __restrict__ sums_out)
+                                     double* __restrict__ costs, double* __restrict__ host_costs,
+                                     long long* __restrict__ sums_out)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= K) return;
@@ -295,6 +296,7 @@ __global__ void corr_finalize_kernel(const Segment* __restrict__ segs, uint32_t 
     }
     float c = float(r);
     costs[k] = double(1.0f - c * c);
+    if (host_costs) host_costs[k] = double(1.0f - c * c);
     if (sums_out) {
         sums_out[k * 5 + 0] = sx;
         sums_out[k * 5 + 1] = sxx;
@@ -464,8 +466,10 @@ __global__ __launch_bounds__(MiTraits<BINS>::kThreads) void mi_hist_kernel(
 // histogram kernel.  Each candidate's bins are split into slices of 4096
 // (16 per thread), one workgroup per slice, so even a single candidate
 // fills many CUs; the slice sums are combined by mi_entropy_final_kernel
-// in a fixed tree.  The result is deterministic (the reference's sum is
-// sequential in row-major order; the two agree to rounding).
+// in a fixed tree.  Each wave first packs its non-zero bins (a ballot and a
+// lane rank per bin slot) so the FP64 log terms run on dense lanes only.
+// The result is deterministic (the reference's sum is sequential in
+// row-major order; the two agree to rounding).
 constexpr int kEntropyThreads = 256;
 constexpr int kEntropySlice = 4096;
 
@@ -474,7 +478,7 @@ constexpr int entropy_slices() { return BINS * BINS / kEntropySlice; }
 
 template <int BINS>
 __global__ __launch_bounds__(kEntropyThreads) void mi_entropy_part_kernel(
-    const Segment* __restrict__ segs, uint32_t nseg, const int* __restrict__ hist,
+    const Segment* __restrict__ segs, uint32_t nseg, int* hist,
     const uint32_t* __restrict__ to_hist, double* __restrict__ part)
 {
     constexpr int G = entropy_slices<BINS>();
@@ -485,27 +489,47 @@ __global__ __launch_bounds__(kEntropyThreads) void mi_entropy_part_kernel(
         segs[find_segment(segs, nseg, cand, ByCandidate{})].from_hist;
     __shared__ uint32_t to_h[BINS];
     __shared__ double wpart[kWaves];
+    // each wave's non-zero bins, packed in (slot, lane) order: at most 64 x 4 x kVec a wave
+    constexpr int kPerWave = 64 * 4 * kVec;
+    __shared__ int s_mu[kEntropySlice];
+    __shared__ uint16_t s_bin[kEntropySlice];
     for (int i = threadIdx.x; i < BINS; i += kEntropyThreads) to_h[i] = to_hist[size_t(cand) * BINS + i];
     // four bins per 16-byte load, all of a thread's loads issued before use
-    const int4* h4 =
-        reinterpret_cast<const int4*>(hist + size_t(cand) * BINS * BINS + size_t(slice) * kEntropySlice);
+    int4* h4 = reinterpret_cast<int4*>(hist + size_t(cand) * BINS * BINS + size_t(slice) * kEntropySlice);
     int4 q[kVec];
 #pragma unroll
     for (int u = 0; u < kVec; ++u) q[u] = h4[threadIdx.x + u * kEntropyThreads];
-    __syncthreads();
-    double acc = 0.0;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    int* w_mu = s_mu + wave * kPerWave;
+    uint16_t* w_bin = s_bin + wave * kPerWave;
+    uint32_t n = 0;  // wave-uniform count so far
 #pragma unroll
     for (int u = 0; u < kVec; ++u) {
-        const int i0 = int(slice) * kEntropySlice + 4 * (int(threadIdx.x) + u * kEntropyThreads);
         const int mus[4] = {q[u].x, q[u].y, q[u].z, q[u].w};
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const int mu = mus[c];
-            if (mu == 0) continue;
-            const int i = i0 + c;
-            const double m = double(mu);
-            acc += m * log(m / double(from_hist[i / BINS]) / double(to_h[i % BINS]));
+            const bool nz = mus[c] != 0;
+            const unsigned long long mk = __ballot(nz);
+            if (nz) {
+                const uint32_t r = __builtin_amdgcn_mbcnt_hi(uint32_t(mk >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo(uint32_t(mk), 0u));
+                w_mu[n + r] = mus[c];
+                w_bin[n + r] = uint16_t(4 * (threadIdx.x + u * kEntropyThreads) + c);
+            }
+            n += uint32_t(__popcll(mk));
         }
+    }
+    __syncthreads();
+    // the slice is read (its values are in registers and LDS): leave it zero
+    // for the next launch, which then needs no clearing pass
+#pragma unroll
+    for (int u = 0; u < kVec; ++u) h4[threadIdx.x + u * kEntropyThreads] = make_int4(0, 0, 0, 0);
+    // the log terms of the wave's non-zero bins only, lane l taking entries l, l + 64, ...
+    double acc = 0.0;
+    for (uint32_t e = lane; e < n; e += 64) {
+        const int i = int(slice) * kEntropySlice + int(w_bin[e]);
+        const double m = double(w_mu[e]);
+        acc += m * log(m / double(from_hist[i / BINS]) / double(to_h[i % BINS]));
     }
     acc = wave_sum_f64(acc);
     if ((threadIdx.x & 63) == 0) wpart[threadIdx.x >> 6] = acc;
@@ -519,21 +543,31 @@ __global__ __launch_bounds__(kEntropyThreads) void mi_entropy_part_kernel(
     }
 }
 
+// Also clears the block's candidates' to_hist rows (every slice has read
+// them), so the whole histogram buffer is zero again after this launch.
 template <int BINS>
 __global__ __launch_bounds__(256) void mi_entropy_final_kernel(const double* __restrict__ part,
-                                                               uint32_t K, double* __restrict__ costs)
+                                                               uint32_t K, double* __restrict__ costs,
+                                                               double* __restrict__ host_costs,
+                                                               uint32_t* to_hist)
 {
     constexpr int G = entropy_slices<BINS>();
-    const uint32_t cand = blockIdx.x * 256u + threadIdx.x;
-    if (cand >= K) return;
-    double t[G];
+    const uint32_t first = blockIdx.x * 256u, cand = first + threadIdx.x;
+    if (cand < K) {
+        double t[G];
 #pragma unroll
-    for (int i = 0; i < G; ++i) t[i] = part[size_t(cand) * G + i];
+        for (int i = 0; i < G; ++i) t[i] = part[size_t(cand) * G + i];
 #pragma unroll
-    for (int w = G / 2; w > 0; w /= 2)
+        for (int w = G / 2; w > 0; w /= 2)
 #pragma unroll
-        for (int i = 0; i < w; ++i) t[i] = t[2 * i] + t[2 * i + 1];
-    costs[cand] = -(t[0] / double(float(BINS * BINS)));
+            for (int i = 0; i < w; ++i) t[i] = t[2 * i] + t[2 * i + 1];
+        const double c = -(t[0] / double(float(BINS * BINS)));
+        costs[cand] = c;
+        if (host_costs) host_costs[cand] = c;
+    }
+    const uint32_t words4 = (min(K - first, 256u) * uint32_t(BINS)) / 4u;
+    uint4* row4 = reinterpret_cast<uint4*>(to_hist + size_t(first) * BINS);
+    for (uint32_t i = threadIdx.x; i < words4; i += 256u) row4[i] = make_uint4(0u, 0u, 0u, 0u);
 }
 
 struct TransformArg {
@@ -759,12 +793,13 @@ void launch_corr_sums(tiplhip_ctx* ctx, const Staged& st)
     timing_end(ctx);
 }
 
-void launch_corr_finalize(tiplhip_ctx* ctx, const Staged& st, long long* sums_out)
+void launch_corr_finalize(tiplhip_ctx* ctx, const Staged& st, long long* sums_out,
+                          double* host_costs = nullptr)
 {
     ctx->d_costs.reserve(sizeof(double) * size_t(st.K));
     hipLaunchKernelGGL(corr_finalize_kernel, dim3((st.K + 63) / 64), dim3(64), 0, ctx->stream,
                        st.segs, st.nseg, ctx->d_sums.as<unsigned long long>(), int(st.K),
-                       ctx->d_costs.as<double>(), sums_out);
+                       ctx->d_costs.as<double>(), host_costs, sums_out);
     TIPLHIP_CHECK(hipGetLastError());
 }
 
@@ -775,11 +810,23 @@ uint32_t* to_hist_of(tiplhip_ctx* ctx, const Staged& st, int bins)
     return reinterpret_cast<uint32_t*>(ctx->d_hist.as<int>() + size_t(bins) * bins * size_t(st.K));
 }
 
+size_t hist_bytes(const Staged& st, int bins)
+{
+    return sizeof(int) * (size_t(bins) * bins + size_t(bins)) * size_t(st.K);
+}
+
+// The histogram kernel accumulates into a zeroed buffer.  The entropy
+// kernels leave what they read zero again, so a clearing pass runs only
+// after the buffer grows or after a launch whose histograms were not
+// consumed by them (the raw histogram entry point, the autotuner).
 void launch_mi_hist(tiplhip_ctx* ctx, const Staged& st, int bins)
 {
-    size_t hb = sizeof(int) * (size_t(bins) * bins + size_t(bins)) * size_t(st.K);
+    const size_t hb = hist_bytes(st, bins);
+    const size_t had = ctx->d_hist.bytes;
     ctx->d_hist.reserve(hb);
-    zero_bytes16(ctx, ctx->d_hist.p, hb);
+    if (ctx->d_hist.bytes != had) ctx->hist_zero = 0;
+    if (ctx->hist_zero < hb) zero_bytes16(ctx, ctx->d_hist.p, hb);
+    ctx->hist_zero = 0;
     timing_begin(ctx);
     if (bins == 256)
         hipLaunchKernelGGL(mi_hist_kernel<256>, dim3(st.total_blocks), dim3(1024), 0, ctx->stream,
@@ -795,7 +842,7 @@ void launch_mi_hist(tiplhip_ctx* ctx, const Staged& st, int bins)
 
 // d_costs holds the K costs, then K x slices partial sums.
 template <int BINS>
-void launch_mi_entropy_t(tiplhip_ctx* ctx, const Staged& st)
+void launch_mi_entropy_t(tiplhip_ctx* ctx, const Staged& st, double* host_costs)
 {
     constexpr int G = entropy_slices<BINS>();
     ctx->d_costs.reserve(sizeof(double) * size_t(st.K) * (1 + G));
@@ -806,16 +853,19 @@ void launch_mi_entropy_t(tiplhip_ctx* ctx, const Staged& st)
                        to_hist_of(ctx, st, BINS), part);
     TIPLHIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(mi_entropy_final_kernel<BINS>, dim3((st.K + 255) / 256), dim3(256), 0,
-                       ctx->stream, part, st.K, costs);
+                       ctx->stream, part, st.K, costs, host_costs, to_hist_of(ctx, st, BINS));
     TIPLHIP_CHECK(hipGetLastError());
+    ctx->hist_zero = hist_bytes(st, BINS);
 }
 
-void launch_mi_entropy(tiplhip_ctx* ctx, const Staged& st, int bins)
+// Costs into ctx->d_costs and, when host_costs is set (device-visible host
+// memory), straight into host memory as well.
+void launch_mi_entropy(tiplhip_ctx* ctx, const Staged& st, int bins, double* host_costs)
 {
     if (bins == 256)
-        launch_mi_entropy_t<256>(ctx, st);
+        launch_mi_entropy_t<256>(ctx, st, host_costs);
     else
-        launch_mi_entropy_t<64>(ctx, st);
+        launch_mi_entropy_t<64>(ctx, st, host_costs);
 }
 
 // tipl::resample of src into a device grid of shape out_dim.
@@ -879,14 +929,19 @@ void cost_jobs(tiplhip_ctx* ctx, std::vector<PairJob> jobs, const float* T, int 
     for (const PairJob& j : jobs) volume_ensure_packed(ctx, const_cast<tiplhip_volume*>(j.to));
     const Staged st = stage_jobs(ctx, jobs, Tk.data(), crop.empty() ? nullptr : crop.data(), cost,
                                  bins, force);
+    // the finishing kernel writes the costs into coherent host memory: no
+    // copy back, the host only waits for the stream
+    ctx->h_costs.reserve(sizeof(double) * K);
+    double* hc = static_cast<double*>(ctx->h_costs.dev);
     if (cost == TIPLHIP_COST_CORR) {
         launch_corr_sums(ctx, st);
-        launch_corr_finalize(ctx, st, nullptr);
+        launch_corr_finalize(ctx, st, nullptr, hc);
     } else {
         launch_mi_hist(ctx, st, bins);
-        launch_mi_entropy(ctx, st, bins);
+        launch_mi_entropy(ctx, st, bins, hc);
     }
-    read_back(ctx, ctx->d_costs.p, costs, sizeof(double) * K);
+    TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
+    std::memcpy(costs, ctx->h_costs.p, sizeof(double) * K);
 }
 
 }  // namespace

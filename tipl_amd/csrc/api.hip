@@ -537,6 +537,9 @@ int tiplhip_ctx_create(int device, tiplhip_ctx** out)
         TIPLHIP_REQUIRE(device >= 0 && device < n, "device index out of range");
         auto* c = new tiplhip_ctx();
         c->device = device;
+        // costs written by kernels straight into host memory: uncached on the
+        // device side, so a finished kernel's writes are already visible
+        c->h_costs.flags = hipHostMallocMapped | hipHostMallocCoherent;
         try {
             DeviceGuard dg(device);
             hipDeviceProp_t prop;
@@ -575,6 +578,7 @@ int tiplhip_ctx_destroy(tiplhip_ctx* ctx)
             ctx->d_tmp2.release();
             ctx->h_T.release();
             ctx->h_out.release();
+            ctx->h_costs.release();
             (void)hipStreamDestroy(ctx->stream);
         }
         delete ctx;

@@ -96,7 +96,9 @@ struct DeviceBuffer {
 // Pinned host allocation that only grows.
 struct PinnedBuffer {
     void* p = nullptr;
+    void* dev = nullptr;  // the same memory as kernels address it
     size_t bytes = 0;
+    unsigned flags = hipHostMallocDefault;
     PinnedBuffer() = default;
     PinnedBuffer(const PinnedBuffer&) = delete;
     PinnedBuffer& operator=(const PinnedBuffer&) = delete;
@@ -105,13 +107,15 @@ struct PinnedBuffer {
     {
         if (n <= bytes) return;
         release();
-        TIPLHIP_CHECK(hipHostMalloc(&p, n, hipHostMallocDefault));
+        TIPLHIP_CHECK(hipHostMalloc(&p, n, flags));
         bytes = n;
+        TIPLHIP_CHECK(hipHostGetDevicePointer(&dev, p, 0));
     }
     void release()
     {
         if (p) (void)hipHostFree(p);
         p = nullptr;
+        dev = nullptr;
         bytes = 0;
     }
     template <class T>
@@ -166,6 +170,8 @@ struct tiplhip_ctx {
     tiplhip::DeviceBuffer d_tmp2;
     tiplhip::PinnedBuffer h_T;
     tiplhip::PinnedBuffer h_out;
+    tiplhip::PinnedBuffer h_costs;   // coherent: cost kernels write the K costs here directly
+    size_t hist_zero = 0;            // leading bytes of d_hist that are zero once queued work ends
     // autotuned launch shapes of single-pair cost launches
     std::map<tiplhip::TuneKey, tiplhip::LaunchShape> tuned;
     // timing
