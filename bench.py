@@ -339,7 +339,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cdm-iters", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--serve-requests", type=int, default=48,
+    ap.add_argument("--serve-requests", type=int, default=96,
                     help="requests in the serving leg (0 skips it)")
     ap.add_argument("--serve-slots", type=int, default=16)
     ap.add_argument("--batch-pairs", type=int, default=32,

@@ -144,6 +144,30 @@ def test_mi_cost_sequence_reuses_cleared_histograms(ctx, oracle):
         assert np.array_equal(got, again)
 
 
+def test_large_launch_z_table_is_bit_identical(ctx, oracle):
+    """Launches of at least 2^25 samples read the z terms of the transformed
+    positions from a table built by a separate kernel; smaller ones compute
+    them inline.  130 candidates on a 64^3 pair (34M samples) take the table
+    path, batches of 10 do not: costs and histograms must agree bit for bit,
+    and with the oracle."""
+    rng = np.random.default_rng(27)
+    A, B = smooth_volume((64, 64, 64), 28), smooth_volume((64, 64, 64), 29)
+    va, vb = ctx.volume(A), ctx.volume(B)
+    Ts = np.stack([_T(oracle, p, A.shape, B.shape) for p in random_params(rng, 130)])
+    assert 130 * A.size >= 1 << 25 and 10 * A.size < 1 << 25
+    for cost, bins in (("mi", 64), ("mi", 256), ("corr", 256)):
+        big = ctx.cost_batch(va, vb, Ts, cost=cost, bins=bins)
+        small = np.concatenate([ctx.cost_batch(va, vb, Ts[i:i + 10], cost=cost, bins=bins)
+                                for i in range(0, len(Ts), 10)])
+        assert np.array_equal(big.view(np.uint64), small.view(np.uint64)), (cost, bins)
+    h = ctx.mi_hist(va, vb, Ts, bins=64)
+    for k in (0, 77, 129):
+        assert np.array_equal(h[k], oracle.mi_hist(A, B, Ts[k], bins=64))
+    s = ctx.corr_sums(va, vb, Ts)
+    for k in (3, 128):
+        assert np.array_equal(s[k], oracle.corr_sums(A, oracle.resample(B, A.shape, Ts[k], crop=False)))
+
+
 def test_batch_equals_single(ctx, oracle):
     rng = np.random.default_rng(17)
     A, B = smooth_volume((24, 20, 28), 18), smooth_volume((24, 20, 28), 19)

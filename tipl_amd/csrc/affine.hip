@@ -180,16 +180,46 @@ __device__ inline Pos apply(const float* T, float fx, float fy, float fz)
     return p;
 }
 
+// fz * T[2], fz * T[5], fz * T[8] of plane z (the z terms of the transformed
+// position).  Large launches take them from a table built once per
+// candidate and plane by ztab_kernel -- the same FP32 products, so the
+// positions are bit-identical -- and read it with scalar loads, since z is
+// uniform across a wave: four fewer VALU instructions per sample.
+template <bool ZT>
+__device__ inline float4 z_products(const float4* __restrict__ ztab, uint32_t row, uint32_t z,
+                                    const float* T)
+{
+    if constexpr (ZT) {
+        return ztab[row + z];
+    } else {
+        const float fz = float(z);
+        return make_float4(fz * T[2], fz * T[5], fz * T[8], 0.0f);
+    }
+}
+
+__global__ __launch_bounds__(256) void ztab_kernel(const float* __restrict__ Ts, uint32_t K,
+                                                   uint32_t zstride, float4* __restrict__ ztab)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= K * zstride) return;
+    const uint32_t k = i / zstride;
+    const float fz = float(i % zstride);
+    ztab[i] = make_float4(fz * Ts[12 * k + 2], fz * Ts[12 * k + 5], fz * Ts[12 * k + 8], 0.0f);
+}
+
 // ---------------------------------------------------------------------------
 // Correlation: per candidate the exact sums Sy, Syy, Sxy (Sx, Sxx of `from`
 // do not depend on the transform and are kept with the volume).
 // crop[k] = {flag, sx, sy, sz}: flag 1 reproduces the CPU resample's exact
 // crop for an integer-shift transform (numerical/resampling.hpp:1244-1262).
 // ---------------------------------------------------------------------------
+template <bool ZT>
 __global__ __launch_bounds__(256) void corr_sums_kernel(const Segment* __restrict__ segs,
                                                         uint32_t nseg, uint32_t total_blocks,
                                                         const float* __restrict__ Ts,
                                                         const int4* __restrict__ crop,
+                                                        const float4* __restrict__ ztab,
+                                                        uint32_t zstride,
                                                         unsigned long long* __restrict__ sums)
 {
     const uint32_t L = xcd_remap(blockIdx.x, total_blocks);
@@ -202,6 +232,7 @@ __global__ __launch_bounds__(256) void corr_sums_kernel(const Segment* __restric
 #pragma unroll
     for (int i = 0; i < 12; ++i) T[i] = Ts[12 * cand + i];
     const int4 cr = crop[cand];
+    const uint32_t zrow = cand * zstride;
     const __amdgpu_buffer_rsrc_t rf = make_rsrc(sg.from, g.from_bytes);
     const __amdgpu_buffer_rsrc_t rt = make_rsrc(sg.to, g.to_bytes8 / 8u);
     const __amdgpu_buffer_rsrc_t r8 = make_rsrc(sg.to8, g.to_bytes8);
@@ -238,19 +269,18 @@ __global__ __launch_bounds__(256) void corr_sums_kernel(const Segment* __restric
         // two planes per step: both samples' gathers are in flight together
         for (; z + 1 < m.z1; z += 2, fidx += 2 * fwh) {
             const uint32_t xa = load_u8(rf, fidx, 0), xb = load_u8(rf, fidx + fwh, 0);
-            const float fza = float(z), fzb = float(z + 1);
+            const float4 za = z_products<ZT>(ztab, zrow, z, T), zb = z_products<ZT>(ztab, zrow, z + 1, T);
             uint32_t ya, yb;
-            sample2_u8_packed(r8, g.to, (cs.hx + fza * T[2]) + T[9], (cs.hy + fza * T[5]) + T[10],
-                              (cs.hz + fza * T[8]) + T[11], (cs.hx + fzb * T[2]) + T[9],
-                              (cs.hy + fzb * T[5]) + T[10], (cs.hz + fzb * T[8]) + T[11], ya, yb);
+            sample2_u8_packed(r8, g.to, (cs.hx + za.x) + T[9], (cs.hy + za.y) + T[10],
+                              (cs.hz + za.z) + T[11], (cs.hx + zb.x) + T[9], (cs.hy + zb.y) + T[10],
+                              (cs.hz + zb.z) + T[11], ya, yb);
             add(xa, ya);
             add(xb, yb);
         }
         if (z < m.z1) {
-            const float fz = float(z);
-            add(load_u8(rf, fidx, 0), sample_u8_packed(r8, g.to, (cs.hx + fz * T[2]) + T[9],
-                                                       (cs.hy + fz * T[5]) + T[10],
-                                                       (cs.hz + fz * T[8]) + T[11]));
+            const float4 zp = z_products<ZT>(ztab, zrow, z, T);
+            add(load_u8(rf, fidx, 0), sample_u8_packed(r8, g.to, (cs.hx + zp.x) + T[9],
+                                                       (cs.hy + zp.y) + T[10], (cs.hz + zp.z) + T[11]));
         }
     }
     __shared__ unsigned long long red[3][4];
@@ -343,10 +373,11 @@ struct MiTraits<64> {
     __device__ static uint32_t inc(uint32_t) { return 1u; }
 };
 
-template <int BINS>
+template <int BINS, bool ZT>
 __global__ __launch_bounds__(MiTraits<BINS>::kThreads) void mi_hist_kernel(
     const Segment* __restrict__ segs, uint32_t nseg, uint32_t total_blocks,
-    const float* __restrict__ Ts, int* __restrict__ hist, uint32_t* __restrict__ to_hist)
+    const float* __restrict__ Ts, const float4* __restrict__ ztab, uint32_t zstride,
+    int* __restrict__ hist, uint32_t* __restrict__ to_hist)
 {
     using Tr = MiTraits<BINS>;
     __shared__ uint32_t lds[Tr::kWords];
@@ -362,6 +393,7 @@ __global__ __launch_bounds__(MiTraits<BINS>::kThreads) void mi_hist_kernel(
     float T[12];
 #pragma unroll
     for (int i = 0; i < 12; ++i) T[i] = Ts[12 * cand + i];
+    const uint32_t zrow = cand * zstride;
     const __amdgpu_buffer_rsrc_t rf = make_rsrc(sg.from, g.from_bytes);
     const __amdgpu_buffer_rsrc_t r8 = make_rsrc(sg.to8, g.to_bytes8);
     const uint32_t lane = threadIdx.x & 63u;
@@ -396,13 +428,13 @@ __global__ __launch_bounds__(MiTraits<BINS>::kThreads) void mi_hist_kernel(
         // two planes per step: both samples' gathers are in flight together
         for (; z + 1 < m.z1; z += 2, fidx += 2 * fwh) {
             const uint32_t fva = load_u8(rf, fidx, 0), fvb = load_u8(rf, fidx + fwh, 0);
-            const float fza = float(z), fzb = float(z + 1);
-            const float pxa = (cs.hx + fza * T[2]) + T[9];
-            const float pya = (cs.hy + fza * T[5]) + T[10];
-            const float pza = (cs.hz + fza * T[8]) + T[11];
-            const float pxb = (cs.hx + fzb * T[2]) + T[9];
-            const float pyb = (cs.hy + fzb * T[5]) + T[10];
-            const float pzb = (cs.hz + fzb * T[8]) + T[11];
+            const float4 za = z_products<ZT>(ztab, zrow, z, T), zb = z_products<ZT>(ztab, zrow, z + 1, T);
+            const float pxa = (cs.hx + za.x) + T[9];
+            const float pya = (cs.hy + za.y) + T[10];
+            const float pza = (cs.hz + za.z) + T[11];
+            const float pxb = (cs.hx + zb.x) + T[9];
+            const float pyb = (cs.hy + zb.y) + T[10];
+            const float pzb = (cs.hz + zb.z) + T[11];
             uint32_t tva, tvb;
             sample2_u8_packed(r8, g.to, pxa, pya, pza, pxb, pyb, pzb, tva, tvb);
             count(fva, tva);
@@ -410,11 +442,9 @@ __global__ __launch_bounds__(MiTraits<BINS>::kThreads) void mi_hist_kernel(
         }
         if (z < m.z1) {
             const uint32_t fv = load_u8(rf, fidx, 0);
-            const float fz = float(z);
-            const float px = (cs.hx + fz * T[2]) + T[9];
-            const float py = (cs.hy + fz * T[5]) + T[10];
-            const float pz = (cs.hz + fz * T[8]) + T[11];
-            count(fv, sample_u8_packed(r8, g.to, px, py, pz));
+            const float4 zp = z_products<ZT>(ztab, zrow, z, T);
+            count(fv, sample_u8_packed(r8, g.to, (cs.hx + zp.x) + T[9], (cs.hy + zp.y) + T[10],
+                                       (cs.hz + zp.z) + T[11]));
         }
     }
     __syncthreads();
@@ -683,7 +713,14 @@ struct Staged {
     const Segment* segs = nullptr;
     const float* T = nullptr;
     const int4* crop = nullptr;
+    double samples = 0.0;   // voxel-candidate samples of the launch
+    uint32_t zstride = 0;   // the largest `from` depth: rows of the z-product table
 };
+
+// Launches of at least this many samples get a z-product table: its build
+// (one small kernel) costs less than the four VALU instructions per sample
+// it saves.
+constexpr double kZTableSamples = double(1 << 25);
 
 bool hist256(int cost, int bins) { return cost == TIPLHIP_COST_MI && bins == 256; }
 
@@ -752,6 +789,8 @@ Staged stage_jobs(tiplhip_ctx* ctx, const std::vector<PairJob>& jobs, const floa
         sg.cand_begin = st.K;
         blocks += sg.g.total_blocks;
         st.K += uint32_t(jobs[j].K);
+        st.samples += double(jobs[j].from->n) * double(jobs[j].K);
+        st.zstride = std::max(st.zstride, uint32_t(jobs[j].from->dim[2]));
     }
     TIPLHIP_REQUIRE(blocks < (uint64_t(1) << 31), "launch exceeds 2^31 workgroups");
     st.nseg = uint32_t(segs.size());
@@ -779,16 +818,35 @@ Staged stage_jobs(tiplhip_ctx* ctx, const std::vector<PairJob>& jobs, const floa
     return st;
 }
 
+// The z-product table of a large launch (nullptr for a small one).
+const float4* stage_ztab(tiplhip_ctx* ctx, const Staged& st)
+{
+    if (st.samples < kZTableSamples) return nullptr;
+    const size_t n = size_t(st.K) * st.zstride;
+    TIPLHIP_REQUIRE(n < (size_t(1) << 32), "z-product table too large");
+    ctx->d_ztab.reserve(sizeof(float4) * n);
+    hipLaunchKernelGGL(ztab_kernel, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                       st.T, st.K, st.zstride, ctx->d_ztab.as<float4>());
+    TIPLHIP_CHECK(hipGetLastError());
+    return ctx->d_ztab.as<float4>();
+}
+
 // Correlation sums Sy, Syy, Sxy of every staged candidate into ctx->d_sums (K x 4 u64).
 void launch_corr_sums(tiplhip_ctx* ctx, const Staged& st)
 {
     ctx->d_sums.reserve(sizeof(unsigned long long) * 4 * size_t(st.K));
     zero_u64(ctx, ctx->d_sums.p, 4 * size_t(st.K));
+    const float4* zt = stage_ztab(ctx, st);
     // per-lane uint32 sums of squares stay below 2^32 for < 65536 voxels a lane
     timing_begin(ctx);
-    hipLaunchKernelGGL(corr_sums_kernel, dim3(st.total_blocks), dim3(256), 0, ctx->stream, st.segs,
-                       st.nseg, st.total_blocks, st.T, st.crop,
-                       ctx->d_sums.as<unsigned long long>());
+    if (zt)
+        hipLaunchKernelGGL(corr_sums_kernel<true>, dim3(st.total_blocks), dim3(256), 0, ctx->stream,
+                           st.segs, st.nseg, st.total_blocks, st.T, st.crop, zt, st.zstride,
+                           ctx->d_sums.as<unsigned long long>());
+    else
+        hipLaunchKernelGGL(corr_sums_kernel<false>, dim3(st.total_blocks), dim3(256), 0, ctx->stream,
+                           st.segs, st.nseg, st.total_blocks, st.T, st.crop, nullptr, 0u,
+                           ctx->d_sums.as<unsigned long long>());
     TIPLHIP_CHECK(hipGetLastError());
     timing_end(ctx);
 }
@@ -827,15 +885,22 @@ void launch_mi_hist(tiplhip_ctx* ctx, const Staged& st, int bins)
     if (ctx->d_hist.bytes != had) ctx->hist_zero = 0;
     if (ctx->hist_zero < hb) zero_bytes16(ctx, ctx->d_hist.p, hb);
     ctx->hist_zero = 0;
+    const float4* zt = stage_ztab(ctx, st);
+    int* h = ctx->d_hist.as<int>();
+    uint32_t* th = to_hist_of(ctx, st, bins);
     timing_begin(ctx);
-    if (bins == 256)
-        hipLaunchKernelGGL(mi_hist_kernel<256>, dim3(st.total_blocks), dim3(1024), 0, ctx->stream,
-                           st.segs, st.nseg, st.total_blocks, st.T, ctx->d_hist.as<int>(),
-                           to_hist_of(ctx, st, bins));
+    if (bins == 256 && zt)
+        hipLaunchKernelGGL((mi_hist_kernel<256, true>), dim3(st.total_blocks), dim3(1024), 0,
+                           ctx->stream, st.segs, st.nseg, st.total_blocks, st.T, zt, st.zstride, h, th);
+    else if (bins == 256)
+        hipLaunchKernelGGL((mi_hist_kernel<256, false>), dim3(st.total_blocks), dim3(1024), 0,
+                           ctx->stream, st.segs, st.nseg, st.total_blocks, st.T, nullptr, 0u, h, th);
+    else if (zt)
+        hipLaunchKernelGGL((mi_hist_kernel<64, true>), dim3(st.total_blocks), dim3(256), 0,
+                           ctx->stream, st.segs, st.nseg, st.total_blocks, st.T, zt, st.zstride, h, th);
     else
-        hipLaunchKernelGGL(mi_hist_kernel<64>, dim3(st.total_blocks), dim3(256), 0, ctx->stream,
-                           st.segs, st.nseg, st.total_blocks, st.T, ctx->d_hist.as<int>(),
-                           to_hist_of(ctx, st, bins));
+        hipLaunchKernelGGL((mi_hist_kernel<64, false>), dim3(st.total_blocks), dim3(256), 0,
+                           ctx->stream, st.segs, st.nseg, st.total_blocks, st.T, nullptr, 0u, h, th);
     TIPLHIP_CHECK(hipGetLastError());
     timing_end(ctx);
 }

@@ -574,6 +574,7 @@ int tiplhip_ctx_destroy(tiplhip_ctx* ctx)
             ctx->d_hist.release();
             ctx->d_sums.release();
             ctx->d_costs.release();
+            ctx->d_ztab.release();
             ctx->d_tmp.release();
             ctx->d_tmp2.release();
             ctx->h_T.release();

@@ -128,9 +128,21 @@ __device__ inline uint32_t sample_u8_packed(__amdgpu_buffer_rsrc_t r8, const Vol
 
 // Two samples of sample_u8_packed with both cells' loads issued before
 // either is used (two independent chains per thread hide more of the
-// gather latency).  A sample outside the volume reads past the end of the
-// buffer (the raw buffer load returns 0 there) and is forced to 0, so no
-// lane diverges; samples inside compute exactly as sample_u8_packed.
+// gather latency).  A sample outside the volume still loads: its offset is
+// whatever the truncated coordinates give, and the buffer's range check
+// makes any offset safe (out of range reads return 0).  Its value is then
+// forced to 0, so no lane diverges; samples inside compute exactly as
+// sample_u8_packed.
+// v_cvt_u32_f32: truncation for coordinates inside the volume; any other
+// input (negative, too large, NaN) gives a defined, saturated value rather
+// than the undefined result of a C++ conversion.
+__device__ inline uint32_t cvt_u32_sat(float x)
+{
+    uint32_t r;
+    asm("v_cvt_u32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
 __device__ inline void sample2_u8_packed(__amdgpu_buffer_rsrc_t r8, const VolGeo& g, float xa,
                                          float ya, float za, float xb, float yb, float zb,
                                          uint32_t& ta, uint32_t& tb)
@@ -141,12 +153,12 @@ __device__ inline void sample2_u8_packed(__amdgpu_buffer_rsrc_t r8, const VolGeo
     const bool inb = (__float_as_uint(xb) < __float_as_uint(g.wm1)) &
                      (__float_as_uint(yb) < __float_as_uint(g.hm1)) &
                      (__float_as_uint(zb) < __float_as_uint(g.dm1));
-    const uint32_t oa = (__umul24(uint32_t(za), g.wh) + __umul24(uint32_t(ya), uint32_t(g.w)) +
-                         uint32_t(xa)) * 8u;
-    const uint32_t ob = (__umul24(uint32_t(zb), g.wh) + __umul24(uint32_t(yb), uint32_t(g.w)) +
-                         uint32_t(xb)) * 8u;
-    const auto qa = __builtin_amdgcn_raw_buffer_load_b64(r8, int(ina ? oa : 0xFFFFFFF8u), 0, 0);
-    const auto qb = __builtin_amdgcn_raw_buffer_load_b64(r8, int(inb ? ob : 0xFFFFFFF8u), 0, 0);
+    const uint32_t oa = (__umul24(cvt_u32_sat(za), g.wh) + __umul24(cvt_u32_sat(ya), uint32_t(g.w)) +
+                         cvt_u32_sat(xa)) * 8u;
+    const uint32_t ob = (__umul24(cvt_u32_sat(zb), g.wh) + __umul24(cvt_u32_sat(yb), uint32_t(g.w)) +
+                         cvt_u32_sat(xb)) * 8u;
+    const auto qa = __builtin_amdgcn_raw_buffer_load_b64(r8, int(oa), 0, 0);
+    const auto qb = __builtin_amdgcn_raw_buffer_load_b64(r8, int(ob), 0, 0);
     auto blend = [](uint32_t lo, uint32_t hi, float x, float y, float z) {
         const float p0 = __builtin_amdgcn_fractf(x);
         const float p1 = __builtin_amdgcn_fractf(y);

@@ -166,6 +166,7 @@ struct tiplhip_ctx {
     tiplhip::DeviceBuffer d_hist;    // K x bins^2 int32
     tiplhip::DeviceBuffer d_sums;    // K x 4 uint64
     tiplhip::DeviceBuffer d_costs;   // K doubles
+    tiplhip::DeviceBuffer d_ztab;    // K x depth z products of large cost launches
     tiplhip::DeviceBuffer d_tmp;     // volume output / misc
     tiplhip::DeviceBuffer d_tmp2;
     tiplhip::PinnedBuffer h_T;
