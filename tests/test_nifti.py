@@ -164,6 +164,26 @@ def test_errors(tmp_path):
     (tmp_path / "short.nii").write_bytes(data[:-100])
     with pytest.raises(tipl_amd.TiplHipError):
         tipl_amd.nifti_read(tmp_path / "short.nii")
+    # malformed headers are refused, never read past the file
+    import struct
+    nifti_files.write(tmp_path / "ok.nii", _vol((2, 5, 6, 7)), datatype=16)
+    good = bytearray((tmp_path / "ok.nii").read_bytes())
+    for name, off, fmt, val in (("bitpix", 72, "<h", 8),           # float32 data, 8-bit bitpix
+                                ("neg_offset", 108, "<f", -4096.0),
+                                ("nan_offset", 108, "<f", float("nan")),
+                                ("huge_offset", 108, "<f", 3.0e38)):
+        bad = bytearray(good)
+        struct.pack_into(fmt, bad, off, val)
+        (tmp_path / (name + ".nii")).write_bytes(bytes(bad))
+        for volume in (0, 1):
+            with pytest.raises(tipl_amd.TiplHipError):
+                tipl_amd.nifti_read(tmp_path / (name + ".nii"), volume)
+    big = bytearray(good)  # dims far beyond the voxels the file holds
+    struct.pack_into("<8h", big, 40, 4, 512, 512, 512, 2, 1, 1, 1)
+    (tmp_path / "big.nii").write_bytes(bytes(big))
+    for volume in (0, 1):
+        with pytest.raises(tipl_amd.TiplHipError):
+            tipl_amd.nifti_read(tmp_path / "big.nii", volume, np.uint8)
 
 
 def test_cpp_mirror_reads_and_writes(tmp_path, ref):

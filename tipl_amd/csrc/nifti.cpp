@@ -296,6 +296,7 @@ Parsed parse(const std::string& path)
         std::memcpy(h.intent_name, f.data() + 508, 16);
         h.dim_info = f[524];
         p.version = 2;
+        TIPLHIP_REQUIRE(vox_offset >= 0, "invalid NIFTI format: negative vox_offset");
         p.data_begin = size_t(vox_offset);
         p.file.swap(f);
     } else {
@@ -305,6 +306,8 @@ Parsed parse(const std::string& path)
                             (h.magic[1] == 'i' || h.magic[1] == '+');
         if (is_nii && h.magic[1] == '+') {
             p.version = 1;
+            TIPLHIP_REQUIRE(h.vox_offset >= 0.0f && h.vox_offset < 4.0e18f,
+                            "invalid NIFTI format: vox_offset out of range");
             p.data_begin = size_t(h.vox_offset);
             p.file.swap(f);
         } else {
@@ -350,8 +353,16 @@ void read_voxels(const Parsed& p, size_t volume, Out* out)
     TIPLHIP_REQUIRE(n != 0, "invalid dim value");
     // select_volume (io/nifti.hpp:767-775) seeks from vox_offset for every
     // volume; the first volume is read where open() left the stream
-    const size_t begin = volume == 0 ? p.data_begin : size_t(h.vox_offset) + volume * bpp * n;
-    TIPLHIP_REQUIRE(begin + bpp * n <= p.file.size(), "end of file reached. The file is smaller than expected.");
+    size_t bytes = 0, skip = 0, begin = p.data_begin;
+    bool over = __builtin_mul_overflow(bpp, n, &bytes);
+    if (volume != 0) {
+        TIPLHIP_REQUIRE(h.vox_offset >= 0.0f && h.vox_offset < 4.0e18f, "vox_offset out of range");
+        over = over || __builtin_mul_overflow(bytes, volume, &skip) ||
+               __builtin_add_overflow(size_t(h.vox_offset), skip, &begin);
+    }
+    size_t end = 0;
+    over = over || __builtin_add_overflow(begin, bytes, &end);
+    TIPLHIP_REQUIRE(!over && end <= p.file.size(), "end of file reached. The file is smaller than expected.");
     const char* src = p.file.data() + begin;
     constexpr int kOut = std::is_same<Out, uint8_t>::value ? 2 : std::is_same<Out, int16_t>::value ? 4 : 16;
     if (compatible(kOut, h.datatype) && bpp == sizeof(Out)) {
@@ -361,6 +372,14 @@ void read_voxels(const Parsed& p, size_t volume, Out* out)
         return;
     }
     const bool sw = p.big_endian;
+    // every conversion below reads n elements of the datatype's size: that
+    // size must be the checked bytes per voxel (a malformed header could
+    // otherwise make it read past the file)
+    const int dt = h.datatype;
+    const size_t elem = dt == 2 || dt == 256 ? 1 : dt == 4 || dt == 512 ? 2 : dt == 8 || dt == 16 || dt == 768 ? 4
+                        : dt == 64 || dt == 1024 || dt == 1280 ? 8 : 0;
+    TIPLHIP_REQUIRE(elem == 0 || elem == bpp, "invalid NIFTI format: bitpix " + std::to_string(h.bitpix) +
+                                                  " does not match datatype " + std::to_string(dt));
     switch (h.datatype) {
     case 2: convert<Out, uint8_t>(src, n, sw, out); break;
     case 4: convert<Out, int16_t>(src, n, sw, out); break;
