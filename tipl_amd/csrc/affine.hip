@@ -874,17 +874,21 @@ size_t hist_bytes(const Staged& st, int bins)
 }
 
 // The histogram kernel accumulates into a zeroed buffer.  The entropy
-// kernels leave what they read zero again, so a clearing pass runs only
-// after the buffer grows or after a launch whose histograms were not
-// consumed by them (the raw histogram entry point, the autotuner).
+// kernels leave what they read zero again, so clearing passes run only over
+// bytes never cleared since the buffer was allocated and after a launch
+// whose histograms were not consumed by them (the raw histogram entry
+// point, the autotuner).
 void launch_mi_hist(tiplhip_ctx* ctx, const Staged& st, int bins)
 {
-    const size_t hb = hist_bytes(st, bins);
+    const size_t hb = hist_bytes(st, bins);  // a multiple of 16
     const size_t had = ctx->d_hist.bytes;
     ctx->d_hist.reserve(hb);
-    if (ctx->d_hist.bytes != had) ctx->hist_zero = 0;
-    if (ctx->hist_zero < hb) zero_bytes16(ctx, ctx->d_hist.p, hb);
-    ctx->hist_zero = 0;
+    if (ctx->d_hist.bytes != had) ctx->hist_dirty = ctx->hist_zero = 0;
+    char* base = static_cast<char*>(ctx->d_hist.p);
+    if (ctx->hist_dirty > 0) zero_bytes16(ctx, base, ctx->hist_dirty);
+    if (ctx->hist_zero < hb) zero_bytes16(ctx, base + ctx->hist_zero, hb - ctx->hist_zero);
+    ctx->hist_zero = std::max(ctx->hist_zero, hb);
+    ctx->hist_dirty = hb;
     const float4* zt = stage_ztab(ctx, st);
     int* h = ctx->d_hist.as<int>();
     uint32_t* th = to_hist_of(ctx, st, bins);
@@ -920,7 +924,7 @@ void launch_mi_entropy_t(tiplhip_ctx* ctx, const Staged& st, double* host_costs)
     hipLaunchKernelGGL(mi_entropy_final_kernel<BINS>, dim3((st.K + 255) / 256), dim3(256), 0,
                        ctx->stream, part, st.K, costs, host_costs, to_hist_of(ctx, st, BINS));
     TIPLHIP_CHECK(hipGetLastError());
-    ctx->hist_zero = hist_bytes(st, BINS);
+    ctx->hist_dirty = 0;
 }
 
 // Costs into ctx->d_costs and, when host_costs is set (device-visible host

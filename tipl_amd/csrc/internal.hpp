@@ -172,7 +172,9 @@ struct tiplhip_ctx {
     tiplhip::PinnedBuffer h_T;
     tiplhip::PinnedBuffer h_out;
     tiplhip::PinnedBuffer h_costs;   // coherent: cost kernels write the K costs here directly
-    size_t hist_zero = 0;            // leading bytes of d_hist that are zero once queued work ends
+    // d_hist once queued work ends: [0, hist_dirty) holds counts no entropy
+    // pass consumed, [hist_dirty, hist_zero) is zero
+    size_t hist_dirty = 0, hist_zero = 0;
     // autotuned launch shapes of single-pair cost launches
     std::map<tiplhip::TuneKey, tiplhip::LaunchShape> tuned;
     // timing

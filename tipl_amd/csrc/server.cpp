@@ -24,6 +24,7 @@
 #include <cmath>
 #include <condition_variable>
 #include <deque>
+#include <exception>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -93,6 +94,17 @@ struct Joiner {
     ~Joiner()
     {
         if (t.joinable()) t.join();
+    }
+};
+
+// When the scope unwinds on an exception, waits for ctx's stream: buffers
+// declared before it (some owned by another stream, whose frees are not
+// ordered after this stream's work) may still be read by kernels queued here.
+struct DrainOnThrow {
+    tiplhip_ctx* ctx;
+    ~DrainOnThrow()
+    {
+        if (std::uncaught_exceptions() > 0) (void)hipStreamSynchronize(ctx->stream);
     }
 };
 
@@ -181,7 +193,8 @@ void register_pair(tipl::hip::context& ctx, const tiplhip_reg_request& req,
     const size_t fb = sizeof(float) * 3 * nf;
     DevMem t2f(c, fb);
     tiplhip_ctx* aux = req.reverse ? aux_context(c) : nullptr;
-    std::unique_ptr<DevMem> f2t;
+    std::unique_ptr<DevMem> f2t;  // on the auxiliary stream, read on this one below
+    DrainOnThrow drain{c};
     int rev_rc = TIPLHIP_OK;
     std::string rev_err;
     {
