@@ -85,6 +85,17 @@ int tiplhip_ctx_synchronize(tiplhip_ctx* ctx);
 int tiplhip_ctx_set_kernel_timing(tiplhip_ctx* ctx, int enable);
 int tiplhip_ctx_kernel_time(tiplhip_ctx* ctx, double* total_ms, int64_t* launches);
 
+/* HIP graph replay of cost evaluations (default off).  When enabled, a
+ * cost launch (tiplhip_cost_batch, _params, _pairs) whose launch shape --
+ * cost, bins, candidate count, segment and workgroup counts, staged bytes --
+ * was seen before on this context replays a captured graph of its upload and
+ * kernels instead of enqueuing them one by one; only the staged transforms
+ * and segment table change between replays.  Results are bit-identical to
+ * the eager path.  Ignored while kernel timing is on.  No counterpart in the
+ * reference (the launch pattern of its line search, reg/linear.hpp:255-338,
+ * re-evaluates the same shapes many times). */
+int tiplhip_ctx_set_graphs(tiplhip_ctx* ctx, int enable);
+
 /* Stream-level timer: events recorded on the context stream. */
 int tiplhip_timer_start(tiplhip_ctx* ctx);
 int tiplhip_timer_stop(tiplhip_ctx* ctx, float* ms);

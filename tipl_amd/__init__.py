@@ -25,6 +25,7 @@ COST_CORR, COST_MI = 0, 1
 EXPORTS = (
     "tiplhip_last_error", "tiplhip_device_count", "tiplhip_ctx_create", "tiplhip_ctx_destroy",
     "tiplhip_ctx_synchronize", "tiplhip_ctx_set_kernel_timing", "tiplhip_ctx_kernel_time",
+    "tiplhip_ctx_set_graphs",
     "tiplhip_timer_start", "tiplhip_timer_stop", "tiplhip_stream_copy", "tiplhip_volume_create",
     "tiplhip_volume_create_device", "tiplhip_volume_destroy", "tiplhip_volume_dims",
     "tiplhip_volume_read", "tiplhip_volume_stats", "tiplhip_volume_downsample",
@@ -120,6 +121,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "tiplhip_ctx_destroy": (ip, [vp]),
         "tiplhip_ctx_synchronize": (ip, [vp]),
         "tiplhip_ctx_set_kernel_timing": (ip, [vp, ip]),
+        "tiplhip_ctx_set_graphs": (ip, [vp, ip]),
         "tiplhip_ctx_kernel_time": (ip, [vp, P(C.c_double), P(C.c_int64)]),
         "tiplhip_timer_start": (ip, [vp]),
         "tiplhip_timer_stop": (ip, [vp, P(C.c_float)]),
@@ -282,6 +284,10 @@ class Context:
 
     def synchronize(self):
         _check(load_library().tiplhip_ctx_synchronize(self.handle))
+
+    def set_graphs(self, enable: bool):
+        """Replays captured HIP graphs for cost launches of a shape seen before."""
+        _check(load_library().tiplhip_ctx_set_graphs(self.handle, int(enable)))
 
     # -- timing ------------------------------------------------------------
     def set_kernel_timing(self, enable: bool):

@@ -715,6 +715,8 @@ struct Staged {
     const int4* crop = nullptr;
     double samples = 0.0;   // voxel-candidate samples of the launch
     uint32_t zstride = 0;   // the largest `from` depth: rows of the z-product table
+    size_t bytes = 0;       // staged bytes (transforms, crop flags, segment table)
+    bool grew = false;      // the device staging buffer was reallocated
 };
 
 // Launches of at least this many samples get a z-product table: its build
@@ -723,6 +725,9 @@ struct Staged {
 constexpr double kZTableSamples = double(1 << 25);
 
 bool hist256(int cost, int bins) { return cost == TIPLHIP_COST_MI && bins == 256; }
+
+// Cached cost-launch graphs per context (cleared when full).
+constexpr size_t kMaxGraphs = 64;
 
 // Work items (16 x 4 voxel columns of up to zc planes) per workgroup above
 // which a launch gets more workgroups: about 256 items, 256K samples at zc = 16.
@@ -747,7 +752,8 @@ TuneKey tune_key(const tiplhip_volume* from, const tiplhip_volume* to, int K, in
 // in one copy.  Each pair gets a share of about cu_count x per_cu workgroups
 // in proportion to its voxel-candidate work.
 Staged stage_jobs(tiplhip_ctx* ctx, const std::vector<PairJob>& jobs, const float* T,
-                  const int4* crop, int cost, int bins, const LaunchShape* force = nullptr)
+                  const int4* crop, int cost, int bins, const LaunchShape* force = nullptr,
+                  bool upload = true)
 {
     Staged st;
     std::vector<Segment> segs(jobs.size());
@@ -799,7 +805,10 @@ Staged stage_jobs(tiplhip_ctx* ctx, const std::vector<PairJob>& jobs, const floa
     const size_t cb = crop ? sizeof(int4) * size_t(st.K) : 0;
     const size_t sb = sizeof(Segment) * segs.size();
     ctx->h_T.reserve(tb + cb + sb);
+    const size_t had = ctx->d_T.bytes;
     ctx->d_T.reserve(tb + cb + sb);
+    st.grew = ctx->d_T.bytes != had;
+    st.bytes = tb + cb + sb;
     char* h = static_cast<char*>(ctx->h_T.p);
     std::memcpy(h, T, tb);
     // a zero shift component is stored as +0.0 (the packed sampler's bounds
@@ -810,12 +819,18 @@ Staged stage_jobs(tiplhip_ctx* ctx, const std::vector<PairJob>& jobs, const floa
             if (hT[12 * k + size_t(i)] == 0.0f) hT[12 * k + size_t(i)] = 0.0f;
     if (crop) std::memcpy(h + tb, crop, cb);
     std::memcpy(h + tb + cb, segs.data(), sb);
-    TIPLHIP_CHECK(hipMemcpyAsync(ctx->d_T.p, h, tb + cb + sb, hipMemcpyHostToDevice, ctx->stream));
+    if (upload) TIPLHIP_CHECK(hipMemcpyAsync(ctx->d_T.p, h, st.bytes, hipMemcpyHostToDevice, ctx->stream));
     char* d = static_cast<char*>(ctx->d_T.p);
     st.T = reinterpret_cast<const float*>(d);
     st.crop = crop ? reinterpret_cast<const int4*>(d + tb) : nullptr;
     st.segs = reinterpret_cast<const Segment*>(d + tb + cb);
     return st;
+}
+
+// The staged bytes' copy to the device (when stage_jobs was told not to).
+void upload_staged(tiplhip_ctx* ctx, const Staged& st)
+{
+    TIPLHIP_CHECK(hipMemcpyAsync(ctx->d_T.p, ctx->h_T.p, st.bytes, hipMemcpyHostToDevice, ctx->stream));
 }
 
 // The z-product table of a large launch (nullptr for a small one).
@@ -878,16 +893,31 @@ size_t hist_bytes(const Staged& st, int bins)
 // bytes never cleared since the buffer was allocated and after a launch
 // whose histograms were not consumed by them (the raw histogram entry
 // point, the autotuner).
+// Returns true when the buffer was reallocated.
+bool reserve_hist(tiplhip_ctx* ctx, size_t hb)
+{
+    const size_t had = ctx->d_hist.bytes;
+    ctx->d_hist.reserve(hb);
+    if (ctx->d_hist.bytes == had) return false;
+    ctx->hist_dirty = ctx->hist_zero = 0;
+    return true;
+}
+
+// Queues the clearing passes that make the first hb bytes zero.
+void make_hist_clean(tiplhip_ctx* ctx, size_t hb)
+{
+    char* base = static_cast<char*>(ctx->d_hist.p);
+    if (ctx->hist_dirty > 0) zero_bytes16(ctx, base, ctx->hist_dirty);
+    ctx->hist_dirty = 0;
+    if (ctx->hist_zero < hb) zero_bytes16(ctx, base + ctx->hist_zero, hb - ctx->hist_zero);
+    ctx->hist_zero = std::max(ctx->hist_zero, hb);
+}
+
 void launch_mi_hist(tiplhip_ctx* ctx, const Staged& st, int bins)
 {
     const size_t hb = hist_bytes(st, bins);  // a multiple of 16
-    const size_t had = ctx->d_hist.bytes;
-    ctx->d_hist.reserve(hb);
-    if (ctx->d_hist.bytes != had) ctx->hist_dirty = ctx->hist_zero = 0;
-    char* base = static_cast<char*>(ctx->d_hist.p);
-    if (ctx->hist_dirty > 0) zero_bytes16(ctx, base, ctx->hist_dirty);
-    if (ctx->hist_zero < hb) zero_bytes16(ctx, base + ctx->hist_zero, hb - ctx->hist_zero);
-    ctx->hist_zero = std::max(ctx->hist_zero, hb);
+    reserve_hist(ctx, hb);
+    make_hist_clean(ctx, hb);
     ctx->hist_dirty = hb;
     const float4* zt = stage_ztab(ctx, st);
     int* h = ctx->d_hist.as<int>();
@@ -997,17 +1027,89 @@ void cost_jobs(tiplhip_ctx* ctx, std::vector<PairJob> jobs, const float* T, int 
     }
     for (const PairJob& j : jobs) volume_ensure_packed(ctx, const_cast<tiplhip_volume*>(j.to));
     const Staged st = stage_jobs(ctx, jobs, Tk.data(), crop.empty() ? nullptr : crop.data(), cost,
-                                 bins, force);
+                                 bins, force, /*upload=*/false);
+    // every buffer the launch touches, reserved up front (a graph capture may
+    // not allocate); a reallocation invalidates the cached graphs
+    bool grew = st.grew;
+    auto grow = [&](DeviceBuffer& b, size_t n) {
+        const size_t had = b.bytes;
+        b.reserve(n);
+        grew = grew || b.bytes != had;
+    };
+    const bool mi = cost == TIPLHIP_COST_MI;
+    if (mi) {
+        grew = reserve_hist(ctx, hist_bytes(st, bins)) || grew;
+        grow(ctx->d_costs, sizeof(double) * K * size_t(1 + (bins == 256 ? entropy_slices<256>()
+                                                                        : entropy_slices<64>())));
+    } else {
+        grow(ctx->d_sums, sizeof(unsigned long long) * 4 * K);
+        grow(ctx->d_costs, sizeof(double) * K);
+    }
+    if (st.samples >= kZTableSamples) grow(ctx->d_ztab, sizeof(float4) * K * st.zstride);
     // the finishing kernel writes the costs into coherent host memory: no
     // copy back, the host only waits for the stream
+    const size_t had_hc = ctx->h_costs.bytes;
     ctx->h_costs.reserve(sizeof(double) * K);
+    grew = grew || ctx->h_costs.bytes != had_hc;
     double* hc = static_cast<double*>(ctx->h_costs.dev);
-    if (cost == TIPLHIP_COST_CORR) {
-        launch_corr_sums(ctx, st);
-        launch_corr_finalize(ctx, st, nullptr, hc);
+    auto enqueue = [&] {
+        upload_staged(ctx, st);
+        if (mi) {
+            launch_mi_hist(ctx, st, bins);
+            launch_mi_entropy(ctx, st, bins, hc);
+        } else {
+            launch_corr_sums(ctx, st);
+            launch_corr_finalize(ctx, st, nullptr, hc);
+        }
+    };
+    if (grew && !ctx->graphs.empty()) {
+        TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
+        clear_graphs(ctx);
+    }
+    if (ctx->use_graphs && !ctx->kernel_timing) {
+        // the same launch shape replays one graph; only the staged bytes
+        // (transforms, crop flags, segment table) are new
+        if (mi) make_hist_clean(ctx, hist_bytes(st, bins));  // outside the graph
+        // (the buffers' addresses are part of the key: a graph is replayed
+        // only on the buffers it was captured with, each reserved above for
+        // this launch's sizes)
+        const uint64_t key_fields[] = {uint64_t(cost), uint64_t(mi ? bins : 0), uint64_t(K),
+                                       uint64_t(st.nseg), uint64_t(st.total_blocks), uint64_t(st.bytes),
+                                       uint64_t(st.samples >= kZTableSamples), uint64_t(st.zstride),
+                                       uint64_t(st.crop != nullptr),
+                                       uint64_t(reinterpret_cast<uintptr_t>(ctx->d_T.p)),
+                                       uint64_t(reinterpret_cast<uintptr_t>(ctx->h_T.p)),
+                                       uint64_t(reinterpret_cast<uintptr_t>(ctx->d_hist.p)),
+                                       uint64_t(reinterpret_cast<uintptr_t>(ctx->d_costs.p)),
+                                       uint64_t(reinterpret_cast<uintptr_t>(ctx->d_sums.p)),
+                                       uint64_t(reinterpret_cast<uintptr_t>(ctx->d_ztab.p)),
+                                       uint64_t(reinterpret_cast<uintptr_t>(ctx->h_costs.dev))};
+        const std::string key(reinterpret_cast<const char*>(key_fields), sizeof(key_fields));
+        auto it = ctx->graphs.find(key);
+        if (it == ctx->graphs.end()) {
+            if (ctx->graphs.size() >= kMaxGraphs) {
+                TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
+                clear_graphs(ctx);
+            }
+            TIPLHIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            hipGraph_t g = nullptr;
+            try {
+                enqueue();
+            } catch (...) {
+                (void)hipStreamEndCapture(ctx->stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            TIPLHIP_CHECK(hipStreamEndCapture(ctx->stream, &g));
+            hipGraphExec_t ex = nullptr;
+            const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            TIPLHIP_CHECK(e);
+            it = ctx->graphs.emplace(key, ex).first;
+        }
+        TIPLHIP_CHECK(hipGraphLaunch(it->second, ctx->stream));
     } else {
-        launch_mi_hist(ctx, st, bins);
-        launch_mi_entropy(ctx, st, bins, hc);
+        enqueue();
     }
     TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
     std::memcpy(costs, ctx->h_costs.p, sizeof(double) * K);

@@ -59,6 +59,12 @@ void pool_free(tiplhip_ctx* ctx, void* p)
     if (p) (void)hipFreeAsync(p, ctx->stream);
 }
 
+void clear_graphs(tiplhip_ctx* ctx)
+{
+    for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+    ctx->graphs.clear();
+}
+
 // ---------------------------------------------------------------------------
 // Host geometry, float arithmetic in the reference's order
 // ---------------------------------------------------------------------------
@@ -570,6 +576,7 @@ int tiplhip_ctx_destroy(tiplhip_ctx* ctx)
             (void)hipStreamSynchronize(ctx->stream);
             for (hipEvent_t e : {ctx->t0, ctx->t1, ctx->k0, ctx->k1, ctx->e_read})
                 if (e) (void)hipEventDestroy(e);
+            clear_graphs(ctx);
             ctx->d_T.release();
             ctx->d_hist.release();
             ctx->d_sums.release();
@@ -592,6 +599,20 @@ int tiplhip_ctx_synchronize(tiplhip_ctx* ctx)
         TIPLHIP_REQUIRE(ctx != nullptr, "null context");
         DeviceGuard dg(ctx->device);
         TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int tiplhip_ctx_set_graphs(tiplhip_ctx* ctx, int enable)
+{
+    return guarded([&] {
+        TIPLHIP_REQUIRE(ctx != nullptr, "null context");
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        DeviceGuard dg(ctx->device);
+        ctx->use_graphs = enable != 0;
+        if (!ctx->use_graphs) {
+            TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
+            clear_graphs(ctx);
+        }
     });
 }
 

@@ -181,6 +181,8 @@ struct tiplhip_ctx {
     hipEvent_t t0 = nullptr, t1 = nullptr, k0 = nullptr, k1 = nullptr;
     hipEvent_t e_read = nullptr;     // marks a device-to-host copy the host waits for
     bool kernel_timing = false;
+    bool use_graphs = false;         // see tiplhip_ctx_set_graphs
+    std::map<std::string, hipGraphExec_t> graphs;  // by launch shape
     tiplhip_ctx* aux = nullptr;      // see aux_context
     double kernel_ms = 0.0;
     int64_t kernel_launches = 0;
@@ -211,6 +213,8 @@ namespace tiplhip {
 // finished there before it is freed (every blocking entry point is).
 void* pool_alloc(tiplhip_ctx* ctx, size_t bytes);
 void pool_free(tiplhip_ctx* ctx, void* p);
+// Destroys the context's cached cost-launch graphs.
+void clear_graphs(tiplhip_ctx* ctx);
 
 // Allocates a volume object with device storage for `dim` voxels.
 tiplhip_volume* volume_alloc(tiplhip_ctx* ctx, const int* dim);
