@@ -495,8 +495,8 @@ __global__ __launch_bounds__(MiTraits<BINS>::kThreads) void mi_hist_kernel(
 // float(BINS*BINS) and negated (reg/linear.hpp:201).  to_h comes from the
 // histogram kernel.  Each candidate's bins are split into slices of 4096
 // (16 per thread), one workgroup per slice, so even a single candidate
-// fills many CUs; the slice sums are combined by mi_entropy_final_kernel
-// in a fixed tree.  Each wave first packs its non-zero bins (a ballot and a
+// fills many CUs; the slice sums of 256-bin histograms are combined by
+// mi_entropy_final_kernel in a fixed tree (a 64-bin histogram is one slice).  Each wave first packs its non-zero bins (a ballot and a
 // lane rank per bin slot) so the FP64 log terms run on dense lanes only.
 // The result is deterministic (the reference's sum is sequential in
 // row-major order; the two agree to rounding).
@@ -507,9 +507,9 @@ template <int BINS>
 constexpr int entropy_slices() { return BINS * BINS / kEntropySlice; }
 
 template <int BINS>
-__global__ __launch_bounds__(kEntropyThreads) void mi_entropy_part_kernel(
-    const Segment* __restrict__ segs, uint32_t nseg, int* hist,
-    const uint32_t* __restrict__ to_hist, double* __restrict__ part)
+__global__ __launch_bounds__(kEntropyThreads) void mi_entropy_kernel(
+    const Segment* __restrict__ segs, uint32_t nseg, int* hist, uint32_t* to_hist,
+    double* __restrict__ part, double* __restrict__ costs, double* __restrict__ host_costs)
 {
     constexpr int G = entropy_slices<BINS>();
     constexpr int kVec = kEntropySlice / 4 / kEntropyThreads;
@@ -564,12 +564,25 @@ __global__ __launch_bounds__(kEntropyThreads) void mi_entropy_part_kernel(
     acc = wave_sum_f64(acc);
     if ((threadIdx.x & 63) == 0) wpart[threadIdx.x >> 6] = acc;
     __syncthreads();
+    // One slice per candidate (64 bins): the cost itself, and the candidate's
+    // to_hist row cleared (this workgroup was its only reader).  Several
+    // slices (256 bins): the slice sum, combined by mi_entropy_final_kernel.
     if (threadIdx.x == 0) {
         double t[kWaves];
         for (int i = 0; i < kWaves; ++i) t[i] = wpart[i];
         for (int w = kWaves / 2; w > 0; w /= 2)
             for (int i = 0; i < w; ++i) t[i] = t[2 * i] + t[2 * i + 1];
-        part[blockIdx.x] = t[0];
+        if constexpr (G == 1) {
+            const double c = -(t[0] / double(float(BINS * BINS)));
+            costs[cand] = c;
+            if (host_costs) host_costs[cand] = c;
+        } else {
+            part[blockIdx.x] = t[0];
+        }
+    }
+    if constexpr (G == 1) {
+        uint4* row4 = reinterpret_cast<uint4*>(to_hist + size_t(cand) * BINS);
+        for (int i = int(threadIdx.x); i < BINS / 4; i += kEntropyThreads) row4[i] = make_uint4(0u, 0u, 0u, 0u);
     }
 }
 
@@ -947,13 +960,15 @@ void launch_mi_entropy_t(tiplhip_ctx* ctx, const Staged& st, double* host_costs)
     ctx->d_costs.reserve(sizeof(double) * size_t(st.K) * (1 + G));
     double* costs = ctx->d_costs.as<double>();
     double* part = costs + st.K;
-    hipLaunchKernelGGL(mi_entropy_part_kernel<BINS>, dim3(st.K * uint32_t(G)), dim3(kEntropyThreads),
-                       0, ctx->stream, st.segs, st.nseg, ctx->d_hist.as<int>(),
-                       to_hist_of(ctx, st, BINS), part);
+    hipLaunchKernelGGL(mi_entropy_kernel<BINS>, dim3(st.K * uint32_t(G)), dim3(kEntropyThreads), 0,
+                       ctx->stream, st.segs, st.nseg, ctx->d_hist.as<int>(), to_hist_of(ctx, st, BINS),
+                       part, costs, host_costs);
     TIPLHIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(mi_entropy_final_kernel<BINS>, dim3((st.K + 255) / 256), dim3(256), 0,
-                       ctx->stream, part, st.K, costs, host_costs, to_hist_of(ctx, st, BINS));
-    TIPLHIP_CHECK(hipGetLastError());
+    if constexpr (G > 1) {
+        hipLaunchKernelGGL(mi_entropy_final_kernel<BINS>, dim3((st.K + 255) / 256), dim3(256), 0,
+                           ctx->stream, part, st.K, costs, host_costs, to_hist_of(ctx, st, BINS));
+        TIPLHIP_CHECK(hipGetLastError());
+    }
     ctx->hist_dirty = 0;
 }
 
