@@ -350,7 +350,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (each pair or "
-              f"candidate set fits one MI355X; sharding is described in DESIGN.md)", file=sys.stderr)
+              f"candidate set fits one MI355X; registrations shard across GPUs with no exchange, "
+              f"one server per GPU)", file=sys.stderr)
         return 2
 
     reqs = []
