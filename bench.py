@@ -163,6 +163,52 @@ def batch_pair(i):
     return fixed, moving
 
 
+def batch_reg_pair(i):
+    """Pair i of the batched-registration workload: a cube with side uniform
+    in [64, 192], the tissue phantom and its remapped copy under a random
+    affine and a smooth nonlinear warp, float32 (seed 3200 + i)."""
+    from tipl_amd import phantoms
+    rng = np.random.default_rng(3200 + i)
+    side = int(rng.integers(64, 193))
+    fixed, moving, _ = phantoms.serving_pair(3200 + i, (side, side, side), True, np.float32)
+    return fixed, moving
+
+
+def make_batch_reg_pairs(n):
+    import multiprocessing as mp
+    with mp.get_context("fork").Pool(min(8, n)) as pool:
+        return pool.map(batch_reg_pair, range(n))
+
+
+def batch_registration_bench(pairs, slots=16):
+    """The batch of uneven affine+CDM pairs registered one after another on
+    the plain path (tiplhip_register) against all of them submitted to the
+    server at once (their candidate sets merged into shared multi-pair
+    launches, the CDM runs overlapped on the device); every served result is
+    compared with the plain one bit for bit."""
+    import tipl_amd
+    ctx = tipl_amd.Context(0)
+    tipl_amd.register(ctx, pairs[0][0], pairs[0][1], nonlinear=True)  # warm-up
+    t0 = time.perf_counter()
+    plain = [tipl_amd.register(ctx, f, m, nonlinear=True, want_field=False) for f, m in pairs]
+    plain_dt = time.perf_counter() - t0
+    ctx.close()
+    with tipl_amd.Server(0, slots=slots, hbm_budget=192 << 30) as srv:
+        t0 = time.perf_counter()
+        tickets = [srv.submit(f, m, nonlinear=True, want_field=False) for f, m in pairs]
+        served = [srv.wait(t) for t in tickets]
+        served_dt = time.perf_counter() - t0
+    same = all(np.array_equal(a["T"].view(np.uint32), b["T"].view(np.uint32)) and
+               a["evaluations"] == b["evaluations"] and a["cdm_iterations"] == b["cdm_iterations"]
+               for a, b in zip(served, plain))
+    return {"pairs": len(pairs), "sides": [int(f.shape[0]) for f, _ in pairs], "slots": slots,
+            "plain_pairs_per_s": round(len(pairs) / plain_dt, 3),
+            "batched_pairs_per_s": round(len(pairs) / served_dt, 3),
+            "batched_over_plain": round(plain_dt / served_dt, 2),
+            "cdm_iterations": int(sum(r["cdm_iterations"] for r in plain)),
+            "results_equal_plain": bool(same)}
+
+
 def make_batch_pairs(n):
     import multiprocessing as mp
     with mp.get_context("fork").Pool(min(8, n)) as pool:
@@ -343,7 +389,7 @@ def main():
                     help="requests in the serving leg (0 skips it)")
     ap.add_argument("--serve-slots", type=int, default=16)
     ap.add_argument("--batch-pairs", type=int, default=32,
-                    help="pairs in the multi-pair launch leg (0 skips it)")
+                    help="pairs in the multi-pair launch and batched-registration legs (0 skips them)")
     ap.add_argument("--serve-rate", type=float, default=1.0,
                     help="Poisson arrival rate of the latency run, as a multiple of the plain "
                          "path's rate")
@@ -358,10 +404,11 @@ def main():
     if args.serve_requests > 0:
         log(f"generating {args.serve_requests} serving requests")
         reqs = make_serving_requests(args.serve_requests)
-    bpairs = []
+    bpairs, rpairs = [], []
     if args.batch_pairs > 0:
-        log(f"generating {args.batch_pairs} uneven pairs")
+        log(f"generating {args.batch_pairs} uneven pairs (affine, and affine+CDM)")
         bpairs = make_batch_pairs(args.batch_pairs)
+        rpairs = make_batch_reg_pairs(args.batch_pairs)
 
     import tipl_amd
     from tipl_amd import phantoms
@@ -455,6 +502,11 @@ def main():
         log("serving leg")
         serve = serving_bench(reqs, args.serve_slots, args.serve_rate)
         log(f"serving {serve}")
+    breg = None
+    if rpairs:
+        log("batched affine+CDM registrations of uneven pairs")
+        breg = batch_registration_bench(rpairs)
+        log(f"batched registrations {breg}")
 
     cpu = cpu_corr = cpu_cdm = None
     if not args.no_cpu:
@@ -495,6 +547,7 @@ def main():
         "mm_reg_pipeline": mmreg,
         "multi_pair": mp_res,
         "serving": serve,
+        "batch_registration": breg,
         **{k: round(v, 4) for k, v in res.items()},
     }
     print(json.dumps(out), flush=True)
