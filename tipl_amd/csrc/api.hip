@@ -546,6 +546,7 @@ int tiplhip_ctx_create(int device, tiplhip_ctx** out)
         // costs written by kernels straight into host memory: uncached on the
         // device side, so a finished kernel's writes are already visible
         c->h_costs.flags = hipHostMallocMapped | hipHostMallocCoherent;
+        c->h_cdm.flags = hipHostMallocMapped | hipHostMallocCoherent;
         try {
             DeviceGuard dg(device);
             hipDeviceProp_t prop;
@@ -587,6 +588,7 @@ int tiplhip_ctx_destroy(tiplhip_ctx* ctx)
             ctx->h_T.release();
             ctx->h_out.release();
             ctx->h_costs.release();
+            ctx->h_cdm.release();
             (void)hipStreamDestroy(ctx->stream);
         }
         delete ctx;

@@ -23,6 +23,7 @@
 #include "internal.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <vector>
 
@@ -159,6 +160,11 @@ __global__ __launch_bounds__(256) void gradient_kernel(const uint8_t* __restrict
 // the product of the clipped extents, as get_window_at_width counts it.
 // ---------------------------------------------------------------------------
 constexpr int kZC = 16;
+
+// A quiet-NaN float pattern the cost reduction never writes, and how long
+// the host spins for an iteration's cost before it synchronises the stream.
+constexpr uint32_t kCostPending = 0x7FC0C057u;
+constexpr int kCostSpinUs = 2000;
 constexpr int kPX = kTX + 4, kPY = kTY + 4;
 
 struct Sums5 {
@@ -370,7 +376,8 @@ dim3 box_grid(const FieldGeo& g)
 // Sums the block partials in a fixed order and stores float(sum / n).
 __global__ __launch_bounds__(1024) void cost_reduce_kernel(const double* __restrict__ partials,
                                                            uint32_t count, double n,
-                                                           float* __restrict__ out)
+                                                           float* __restrict__ out,
+                                                           float* __restrict__ host_out)
 {
     double acc = 0.0;
     for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) acc += partials[i];
@@ -382,6 +389,7 @@ __global__ __launch_bounds__(1024) void cost_reduce_kernel(const double* __restr
         double s = 0.0;
         for (uint32_t w = 0; w < (blockDim.x >> 6); ++w) s += part[w];
         out[0] = float(s / n);
+        if (host_out) host_out[0] = float(s / n);
     }
 }
 
@@ -876,7 +884,8 @@ void launch_compose(tiplhip_ctx* ctx, const uint8_t* Is, const float* d, const F
 }
 
 void launch_gradient(tiplhip_ctx* ctx, const uint8_t* Js, const uint8_t* It, const FieldGeo& g,
-                     char type, int accumulate, float* nd, double* partials, float* cost_out)
+                     char type, int accumulate, float* nd, double* partials, float* cost_out,
+                     float* host_cost = nullptr)
 {
     uint32_t parts;
     if (type == 'd') {
@@ -891,7 +900,7 @@ void launch_gradient(tiplhip_ctx* ctx, const uint8_t* Js, const uint8_t* It, con
     }
     TIPLHIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(cost_reduce_kernel, dim3(1), dim3(1024), 0, ctx->stream, partials, parts,
-                       double(g.n), cost_out);
+                       double(g.n), cost_out, host_cost);
     TIPLHIP_CHECK(hipGetLastError());
 }
 
@@ -1058,32 +1067,49 @@ bool cdm_level(CdmRun& run, const std::vector<const uint8_t*>& It,
     float best_cost = 3.402823466e+38f;
     float cur_smoothing = 0.8f;
     std::vector<float> mod_cost(size_t(run.nmod));
+    TIPLHIP_REQUIRE(run.nmod >= 1 && run.nmod <= 16, "1 to 16 modalities");
+    ctx->h_cdm.reserve(sizeof(float) * 16);
     for (int index = 0; index < run.param.max_iterations; ++index) {
         if (is_terminated(run)) {
             run.stopped = true;
             break;
         }
         ++run.iterations;
+        // Each modality's cost is also written into coherent host memory,
+        // whose slots start as a NaN pattern the reduction never writes; the
+        // host spins on them (a stream synchronisation after kCostSpinUs).
+        // The Jacobi sweeps do not depend on the host's decision (the
+        // reference solves the same new_d whatever cdm_improved says), so
+        // they are queued before the host waits and run during the round trip.
+        uint32_t* slots = ctx->h_cdm.as<uint32_t>();
+        std::fill(slots, slots + run.nmod, kCostPending);
+        float* h_cost = static_cast<float*>(ctx->h_cdm.dev);
         for (int i = 0; i < run.nmod; ++i) {
             launch_compose(ctx, Is[size_t(i)], cur.as<float>(), g, Js.as<uint8_t>());
             const char type = run.param.gradient_type[std::min(i, 7)] == 'd' ? 'd' : 'r';
             launch_gradient(ctx, Js.as<uint8_t>(), It[size_t(i)], g, type, i > 0 ? 1 : 0,
-                            nd.as<float>(), partials.as<double>(), d_cost + i);
+                            nd.as<float>(), partials.as<double>(), d_cost + i, h_cost + i);
         }
-        // The cost goes to pinned memory behind an event; the Jacobi sweeps do
-        // not depend on the host's decision (the reference solves the same
-        // new_d whatever cdm_improved says), so they are queued before the
-        // host waits for the cost and run during the round trip.
-        ctx->h_out.reserve(sizeof(float) * 16);
-        float* h_cost = ctx->h_out.as<float>();
-        TIPLHIP_CHECK(hipMemcpyAsync(h_cost, d_cost, sizeof(float) * size_t(run.nmod),
-                                     hipMemcpyDeviceToHost, ctx->stream));
-        TIPLHIP_CHECK(hipEventRecord(ctx->e_read, ctx->stream));
         const bool need_theta = theta == 0.0f;
         float* solved = launch_poisson(ctx, nd.as<float>(), pa.as<float>(), pb.as<float>(), g, 12,
                                        need_theta ? d_maxlen : nullptr, maxparts.as<float>());
-        TIPLHIP_CHECK(hipEventSynchronize(ctx->e_read));
-        std::copy(h_cost, h_cost + run.nmod, mod_cost.begin());
+        {
+            const volatile uint32_t* seen = slots;
+            int ready = 0;
+            const auto t0 = std::chrono::steady_clock::now();
+            while (ready < run.nmod) {
+                if (seen[ready] != kCostPending) {
+                    ++ready;
+                    continue;
+                }
+                if (std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(kCostSpinUs)) {
+                    TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
+                    break;
+                }
+                __builtin_ia32_pause();
+            }
+            for (int i = 0; i < run.nmod; ++i) std::memcpy(&mod_cost[size_t(i)], &slots[i], sizeof(float));
+        }
         float sum_cost = 0.0f;
         for (int i = 0; i < run.nmod; ++i) {
             sum_cost += mod_cost[size_t(i)];

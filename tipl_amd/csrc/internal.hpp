@@ -172,6 +172,7 @@ struct tiplhip_ctx {
     tiplhip::PinnedBuffer h_T;
     tiplhip::PinnedBuffer h_out;
     tiplhip::PinnedBuffer h_costs;   // coherent: cost kernels write the K costs here directly
+    tiplhip::PinnedBuffer h_cdm;     // coherent: a CDM iteration's per-modality costs
     // d_hist once queued work ends: [0, hist_dirty) holds counts no entropy
     // pass consumed, [hist_dirty, hist_zero) is zero
     size_t hist_dirty = 0, hist_zero = 0;
