@@ -400,6 +400,9 @@ class Reference(_Lib, _Warp):
         self._subj = self._fn("tref_subject_pre", None, _f32p, _i32p, _u8p)
         self._otsu_median = self._fn("tref_otsu_median", C.c_float, _f32p, C.c_size_t)
         self._median = self._fn("tref_median", C.c_float, _f32p, C.c_size_t)
+        self._otsu = self._fn("tref_otsu_threshold", C.c_float, _f32p, C.c_size_t)
+        self._norm16 = self._fn("tref_normalize_i16", C.c_int, _i16p, C.c_size_t, _i32p, _u8p,
+                                C.c_float)
         self._cdm_iters = self._fn("tref_cdm_iterations", C.c_double, _u8p, _u8p, _i32p, C.c_int,
                                    C.c_void_p)
         self._bind_warp("tref_")
@@ -521,8 +524,12 @@ class Reference(_Lib, _Warp):
         return out
 
     def normalize(self, v, upper=255.99):
-        v = np.ascontiguousarray(v, np.float32)
         out = np.zeros(v.shape, np.uint8)
+        if v.dtype == np.int16:
+            v = np.ascontiguousarray(v)
+            self._norm16(v.ravel(), v.size, _dims(v), out.reshape(-1), upper)
+            return out
+        v = np.ascontiguousarray(v, np.float32)
         self._norm(v.ravel(), v.size, _dims(v), out.reshape(-1), upper)
         return out
 
@@ -588,6 +595,10 @@ class Reference(_Lib, _Warp):
     def median(self, v):
         v = np.ascontiguousarray(v, np.float32).reshape(-1)
         return float(self._median(v, v.size))
+
+    def otsu_threshold(self, v):
+        v = np.ascontiguousarray(v, np.float32).reshape(-1)
+        return float(self._otsu(v, v.size))
 
     def cdm_fixed(self, It, Is, levels=3, level_iterations=48, smoothing=0.8, bandwidth=8):
         It = np.ascontiguousarray(It, np.uint8)

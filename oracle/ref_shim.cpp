@@ -349,6 +349,25 @@ float tref_otsu_median(const float* in, size_t n)
     return tipl::segmentation::otsu_median(I);
 }
 
+// segmentation::otsu_threshold (numerical/otsu.hpp:66-80) of a float image.
+float tref_otsu_threshold(const float* in, size_t n)
+{
+    tipl::image<3> I(tipl::shape<3>(unsigned(n), 1u, 1u));
+    std::memcpy(&I[0], in, sizeof(float) * n);
+    return tipl::segmentation::otsu_threshold(I);
+}
+
+// normalize_upper_lower2 (numerical/numerical.hpp:909-927) of an int16 image.
+int tref_normalize_i16(const int16_t* in, size_t n, const int* dim, uint8_t* out, float upper)
+{
+    tipl::const_pointer_image<3, int16_t> I(in, shp(dim));
+    tipl::image<3, uint8_t> O;
+    tipl::normalize_upper_lower2(I, O, upper);
+    if (O.empty()) return 0;
+    std::memcpy(out, O.data(), n);
+    return 1;
+}
+
 float tref_median(const float* in, size_t n)
 {
     std::vector<float> v(in, in + n);

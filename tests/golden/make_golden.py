@@ -18,7 +18,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.dirname(HERE))
 
-from conftest import random_params, smooth_volume  # noqa: E402
+from conftest import prep_images, random_params, smooth_volume  # noqa: E402
 from oracle.bindings import Reference  # noqa: E402
 
 
@@ -81,6 +81,18 @@ def main():
     Is2 = shift(It2.astype(np.float32), (1.0, -1.5, 2.0), order=1).astype(np.uint8)
     ok, f2 = ref.cdm(It2, Is2)
     out.update(full_It=It2, full_Is=Is2, full_ok=np.int32(ok), full_field=f2)
+
+    # preprocessing: otsu threshold, otsu median, chunked median, subject_image_pre
+    imgs = prep_images()
+    out["prep_otsu"] = np.array([ref.otsu_threshold(v) for v in imgs], np.float32)
+    out["prep_otsu_median"] = np.array([ref.otsu_median(v) for v in imgs], np.float32)
+    out["prep_median"] = np.array([ref.median(v) for v in imgs], np.float32)
+    out["prep_subject"] = np.concatenate([ref.subject_pre(v).ravel() for v in imgs])
+    big = np.random.default_rng(77).gamma(2.0, 100.0, 182 * 218 * 182).astype(np.float32)
+    out["prep_median_full"] = np.float32(ref.median(big))
+    q16 = np.random.default_rng(112).integers(-32768, 32767, (9, 10, 11), endpoint=True)
+    q16 = q16.astype(np.int16)
+    out.update(q16_in=q16, q16_out=ref.normalize(q16, 255.99))
 
     path = os.path.join(HERE, "reference_vectors.npz")
     np.savez_compressed(path, **out)

@@ -6,6 +6,8 @@ import os
 import numpy as np
 import pytest
 
+from conftest import prep_images
+
 G = np.load(os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.npz"))
 
 
@@ -61,3 +63,17 @@ def test_whole_cdm_vector(oracle):
     assert ok == bool(G["full_ok"])
     rms = float(np.sqrt(((f - G["full_field"]) ** 2).sum(-1).mean()))
     assert rms <= 1e-6, rms
+
+
+def test_prep_vectors(oracle):
+    imgs = prep_images()
+    assert np.array_equal(np.array([oracle.otsu_threshold(v) for v in imgs], np.float32),
+                          G["prep_otsu"])
+    assert np.array_equal(np.array([oracle.otsu_median(v) for v in imgs], np.float32),
+                          G["prep_otsu_median"])
+    assert np.array_equal(np.array([oracle.median(v) for v in imgs], np.float32), G["prep_median"])
+    assert np.array_equal(np.concatenate([oracle.subject_pre(v).ravel() for v in imgs]),
+                          G["prep_subject"])
+    big = np.random.default_rng(77).gamma(2.0, 100.0, 182 * 218 * 182).astype(np.float32)
+    assert np.float32(oracle.median(big)) == G["prep_median_full"]
+    assert np.array_equal(oracle.normalize(G["q16_in"], 255.99), G["q16_out"])

@@ -91,6 +91,44 @@ def test_mi_row_sums_are_from_histogram(oracle):
         assert np.array_equal(h.sum(1), oracle.value_hist(A, bins))
 
 
+def test_mi64_block_sums_of_mi256(oracle):
+    """The 64-bin joint histogram (both values >> 2) is the 4x4 block sum of
+    the 256-bin one, which is pinned against the reference itself; a column
+    or row binning slip (t >> 3, swapped axes) breaks the identity."""
+    A, B = smooth_volume((13, 11, 12), 16), smooth_volume((10, 14, 9), 17)
+    rng = np.random.default_rng(18)
+    Ts = [I12, np.array([0.9, 0.1, 0.05, -0.1, 1.1, 0.0, 0.02, 0.0, 0.95, 1.3, -0.7, 2.1],
+                        np.float32),
+          np.array([1, 0, 0, 0, 1, 0, 0, 0, 1, 40, 0, 0], np.float32)]   # all outside
+    for _ in range(4):
+        T = I12.copy()
+        T[:9] += rng.uniform(-0.15, 0.15, 9).astype(np.float32)
+        T[9:] = rng.uniform(-2, 2, 3).astype(np.float32)
+        Ts.append(T)
+    for T in Ts:
+        h256 = oracle.mi_hist(A, B, T, 256).astype(np.int64)
+        h64 = oracle.mi_hist(A, B, T, 64)
+        assert np.array_equal(h64, h256.reshape(64, 4, 64, 4).sum(axis=(1, 3)))
+    assert np.array_equal(oracle.value_hist(A, 64), oracle.value_hist(A, 256).reshape(64, 4).sum(1))
+
+
+def test_mi64_relabelling_closed_form(oracle):
+    """Identity transform at 64 bins: a diagonal histogram of v >> 2 and
+    MI = sum h log h / 64^2; a one-to-one relabelling of the 64 labels of the
+    moving image leaves the cost unchanged."""
+    v = smooth_volume((9, 10, 11), 19)
+    h = oracle.mi_hist(v, v, I12, 64)
+    counts = np.bincount((v >> 2).ravel(), minlength=64)
+    assert np.array_equal(h, np.diag(counts))
+    nz = counts[counts > 0].astype(np.float64)
+    want = float((nz * np.log(nz)).sum() / 4096.0)
+    got = oracle.mi_cost(v, v, I12, 64)
+    assert abs(got - want) <= 1e-12 * abs(want)
+    perm = np.concatenate([[0], 1 + np.random.default_rng(20).permutation(63)])
+    moved = (perm[v >> 2] * 4 + (v & 3)).astype(np.uint8)
+    assert abs(oracle.mi_cost(v, moved, I12, 64) - got) <= 1e-12 * abs(got)
+
+
 def test_quantise_limits(oracle):
     v = np.linspace(-3, 5, 1000, dtype=np.float32).reshape(10, 10, 10)
     q = oracle.normalize(v, 255.99)

@@ -6,7 +6,7 @@ tests/golden/, see test_oracle_golden.py)."""
 import numpy as np
 import pytest
 
-from conftest import random_params, smooth_volume
+from conftest import prep_images, random_params, smooth_volume
 
 
 def _T(oracle, p, fshape, tshape, fvs=(1, 1, 1), tvs=(1, 1, 1)):
@@ -237,3 +237,37 @@ def test_mm_reg_pipeline(oracle, ref, reverse, masked):
     keys = ["r", "J_lin", "It_fov", "t2f", "to2from", "JJ"] + (["f2t", "from2to"] if reverse else [])
     for k in keys:
         assert np.array_equal(_bits(a[k]), _bits(b[k])), k
+
+
+def test_otsu_and_median(oracle, ref):
+    """segmentation::otsu_threshold / otsu_median (numerical/otsu.hpp:66-108)
+    and tipl::median's chunked rule (numerical/statistics.hpp:341-363)."""
+    for img in prep_images():
+        assert np.float32(oracle.otsu_threshold(img)) == np.float32(ref.otsu_threshold(img))
+        assert np.float32(oracle.otsu_median(img)) == np.float32(ref.otsu_median(img))
+        assert np.float32(oracle.median(img)) == np.float32(ref.median(img))
+
+
+def test_median_full_size(oracle, ref):
+    """The chunked median at 182x218x182 (7.2M values, 256 uneven chunks)."""
+    rng = np.random.default_rng(77)
+    v = (rng.gamma(2.0, 100.0, 182 * 218 * 182)).astype(np.float32)
+    assert np.float32(oracle.median(v)) == np.float32(ref.median(v))
+
+
+def test_subject_pre(oracle, ref):
+    """reg::subject_image_pre (reg/mm_reg.hpp:14-20): gaussian, then
+    normalize_otsu_median(255.99), then the uint8 conversion."""
+    for img in prep_images():
+        assert np.array_equal(oracle.subject_pre(img), ref.subject_pre(img)), img.shape
+
+
+def test_quantise_int16(oracle, ref):
+    """normalize_upper_lower2 on int16 input (numerical/numerical.hpp:909-927)."""
+    rng = np.random.default_rng(8)
+    for shape, lo, hi in [((9, 10, 11), -32768, 32767), ((5, 6, 7), 0, 4000), ((3, 4, 5), -7, 3)]:
+        v = rng.integers(lo, hi, shape, endpoint=True).astype(np.int16)
+        for upper in (255.99, 255.999):
+            assert np.array_equal(oracle.normalize(v, upper), ref.normalize(v, upper))
+    c = np.full((4, 4, 4), 12, np.int16)
+    assert np.array_equal(oracle.normalize(c), ref.normalize(c))
