@@ -78,10 +78,24 @@ inline cost_evaluator direct_evaluator(hip::context& ctx)
     };
 }
 
-// The device state of one registration: volumes per modality, voxel sizes.
+// The device state of one registration: volumes per modality, voxel sizes
+// and the evaluator that prices candidate sets on them.  A level provides
+// what the runner needs: the first modality's shapes, the number of
+// modalities, the costs of K candidates of modality i, and the half-
+// resolution level.  (Any type with these members can drive the runner,
+// e.g. host images priced by another evaluator.)
 struct linear_level {
     std::vector<std::shared_ptr<hip::volume>> from, to;
     vector<3> from_vs, to_vs;
+    cost_evaluator eval;
+    const shape<3>& from_shape() const { return from[0]->shape(); }
+    const shape<3>& to_shape() const { return to[0]->shape(); }
+    size_t modalities() const { return std::min(from.size(), to.size()); }
+    void costs(size_t i, const float* params, int K, int cost, int bins, double* out) const
+    {
+        eval(from[i]->get(), from_vs.begin(), to[i]->get(), to_vs.begin(), params, K, cost, bins,
+             out);
+    }
     linear_level down() const
     {
         linear_level d;
@@ -91,12 +105,14 @@ struct linear_level {
         d.to_vs = to_vs;
         d.from_vs *= 2.0f;
         d.to_vs *= 2.0f;
+        d.eval = eval;
         return d;
     }
 };
 
 // linear_reg_runner (reg/linear.hpp:290-561).
-class runner {
+template <class Level>
+class basic_runner {
 public:
     affine_param<float, 3> arg_upper, arg_lower;
     affine_param<float, 3>& arg_min;
@@ -107,8 +123,7 @@ public:
     cost_type cost = mutual_info;
     int bins = 256;
 
-    runner(cost_evaluator eval, linear_level lv, affine_param<float, 3>& arg)
-        : arg_min(arg), eval_(std::move(eval)), lv_(std::move(lv)) {}
+    basic_runner(Level lv, affine_param<float, 3>& arg) : arg_min(arg), lv_(std::move(lv)) {}
 
     std::pair<affine_param<float, 3>, affine_param<float, 3>> current_bound(reg_type type) const
     {
@@ -136,8 +151,8 @@ public:
         } else {
             arg_upper = arg_lower = arg_min;
         }
-        const shape<3>& fs = lv_.from[0]->shape();
-        const shape<3>& ts = lv_.to[0]->shape();
+        const shape<3>& fs = lv_.from_shape();
+        const shape<3>& ts = lv_.to_shape();
         for (int i = 0; i < 3; ++i) {
             if (type & translocation) {
                 const float range = std::max<float>(fs[i] * lv_.from_vs[i], ts[i] * lv_.to_vs[i]) * 0.5f;
@@ -168,11 +183,10 @@ public:
         std::vector<float> P(12 * xs.size());
         for (size_t k = 0; k < xs.size(); ++k) std::copy_n(xs[k].data(), 12, &P[12 * k]);
         std::vector<double> c(xs.size());
-        const size_t nmod = std::min(lv_.from.size(), lv_.to.size());
+        const size_t nmod = lv_.modalities();
         const int kind = cost == mutual_info ? TIPLHIP_COST_MI : TIPLHIP_COST_CORR;
         for (size_t i = 0; i < nmod; ++i) {
-            eval_(lv_.from[i]->get(), lv_.from_vs.begin(), lv_.to[i]->get(), lv_.to_vs.begin(),
-                  P.data(), int(xs.size()), kind, bins, c.data());
+            lv_.costs(i, P.data(), int(xs.size()), kind, bins, c.data());
             for (size_t k = 0; k < xs.size(); ++k) total[k] += c[k];
         }
         for (double& v : total) v /= double(float(nmod) * 2.0f);
@@ -216,10 +230,10 @@ public:
     template <class Term>
     float run_optimize_mr(Term&& is_terminated)
     {
-        const size_t sz = lv_.from[0]->shape().size();
+        const size_t sz = lv_.from_shape().size();
         const bool is_large = double(sz) > std::pow(96.0, 3);
         if (is_large) {
-            runner low(eval_, lv_.down(), arg_min);
+            basic_runner low(lv_.down(), arg_min);
             low.arg_upper = arg_upper;
             low.arg_lower = arg_lower;
             low.search_count = search_count;
@@ -238,9 +252,10 @@ public:
     size_t evaluations = 0;
 
 private:
-    cost_evaluator eval_;
-    linear_level lv_;
+    Level lv_;
 };
+
+using runner = basic_runner<linear_level>;
 
 // has_mask (numerical/basic_op.hpp:811-830) with bounding_box (:697-731).
 template <class I>
@@ -272,52 +287,98 @@ bool has_mask(const I& img)
     return false;
 }
 
-// Symmetric 3x3 eigen decomposition (Jacobi rotations); eigenvalues
-// descending, eigenvectors as rows of V.
-inline void eigen_sym3(const double* A_in, double* V, double* d)
+// eigen_decomposition_sym for a symmetric 3x3 matrix
+// (numerical/matrix.hpp:1829-1935): the closed-form trigonometric solution
+// of the characteristic cubic (O. K. Smith, "Eigenvalues of a symmetric
+// 3x3 matrix", CACM 4(4):168, 1961), eigenvectors from the cross product of
+// two rows of A - d I, and the reference's shortcuts for matrices with
+// zero off-diagonal entries.  Eigenvalues in the reference's order (not
+// sorted), eigenvector i in V[3i..3i+2] with the reference's signs, every
+// operation in the reference's order, so the moment-based starting affine
+// below matches the reference's bit for bit.
+inline void eigen_sym3(const double* A, double* V, double* d)
 {
-    double A[9], U[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-    std::copy(A_in, A_in + 9, A);
-    for (int sweep = 0; sweep < 50; ++sweep) {
-        const double off = A[1] * A[1] + A[2] * A[2] + A[5] * A[5];
-        if (off < 1e-30) break;
-        for (int p = 0; p < 2; ++p)
-            for (int q = p + 1; q < 3; ++q) {
-                const double apq = A[p * 3 + q];
-                if (std::abs(apq) < 1e-300) continue;
-                const double theta = (A[q * 3 + q] - A[p * 3 + p]) / (2 * apq);
-                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::abs(theta) + std::sqrt(theta * theta + 1));
-                const double c = 1 / std::sqrt(t * t + 1), s = t * c;
-                for (int k = 0; k < 3; ++k) {  // A = J^T A J
-                    const double akp = A[k * 3 + p], akq = A[k * 3 + q];
-                    A[k * 3 + p] = c * akp - s * akq;
-                    A[k * 3 + q] = s * akp + c * akq;
-                }
-                for (int k = 0; k < 3; ++k) {
-                    const double apk = A[p * 3 + k], aqk = A[q * 3 + k];
-                    A[p * 3 + k] = c * apk - s * aqk;
-                    A[q * 3 + k] = s * apk + c * aqk;
-                }
-                for (int k = 0; k < 3; ++k) {
-                    const double ukp = U[k * 3 + p], ukq = U[k * 3 + q];
-                    U[k * 3 + p] = c * ukp - s * ukq;
-                    U[k * 3 + q] = s * ukp + c * ukq;
-                }
+    auto identity = [&] {
+        std::fill(V, V + 9, 0.0);
+        V[0] = V[4] = V[8] = 1.0;
+    };
+    // one off-diagonal pair (p,q) non-zero, k the decoupled axis
+    auto plane = [&](int k, double app, double aqq, double apq, int vk, int v_pq0, int v_pq1,
+                     int v_a, int v_b) {
+        const double apq2 = apq * apq;
+        d[0] = A[k];
+        d[1] = d[2] = (app + aqq) * 0.5;
+        const double half = (app - aqq) * 0.5;
+        const double l = std::sqrt(half * half + apq2);
+        d[1] += l;
+        d[2] -= l;
+        std::fill(V, V + 9, 0.0);
+        V[vk] = 1.0;
+        const double a_d1 = app - d[1];
+        const double length = std::sqrt(apq2 + a_d1 * a_d1);
+        V[v_pq0] = V[v_pq1] = -apq / length;
+        V[v_a] = a_d1 / length;
+        V[v_b] = -V[v_a];
+    };
+    if (A[1] == 0.0) {
+        if (A[2] == 0.0) {
+            if (A[5] == 0.0) {
+                d[0] = A[0];
+                d[1] = A[4];
+                d[2] = A[8];
+                identity();
+                return;
             }
+            plane(0, A[4], A[8], A[5], 0, 4, 8, 5, 7);
+            return;
+        }
+        if (A[5] == 0.0) {
+            plane(4, A[0], A[8], A[2], 1, 3, 8, 5, 6);
+            return;
+        }
+    } else if (A[2] == 0.0 && A[5] == 0.0) {
+        plane(8, A[0], A[4], A[1], 2, 3, 7, 4, 6);
+        return;
     }
-    int order[3] = {0, 1, 2};
-    std::sort(order, order + 3, [&](int a, int b) { return A[a * 4] > A[b * 4]; });
-    for (int r = 0; r < 3; ++r) {
-        d[r] = A[order[r] * 4];
-        for (int k = 0; k < 3; ++k) V[r * 3 + k] = U[k * 3 + order[r]];
+    const double A2A1 = A[2] * A[1], A1A5 = A[1] * A[5], A2A5 = A[2] * A[5];
+    const double A0A4 = A[0] * A[4], A1A1 = A[1] * A[1], A2A2 = A[2] * A[2], A5A5 = A[5] * A[5];
+    const double A0_A4 = A[0] + A[4];
+    {
+        const double I1 = A0_A4 + A[8];
+        const double I2 = A0A4 + A0_A4 * A[8] - A1A1 - A2A2 - A5A5;
+        const double I3 = A0A4 * A[8] + 2.0 * A2A1 * A[5] - (A[8] * A1A1 + A[4] * A2A2 + A[0] * A5A5);
+        const double I1_3 = I1 / 3.0;
+        const double I1_3sq = I1_3 * I1_3;
+        const double v = I1_3sq - I2 / 3.0;
+        const double s = I1_3sq * I1_3 - I1 * I2 / 6.0 + I3 / 2.0;
+        if (v == 0.0) {
+            std::fill(d, d + 3, 0.0);
+            identity();
+            return;
+        }
+        const double sqrt_v = std::sqrt(v);
+        const double angle = std::acos(std::max<double>(-1.0, std::min<double>(1.0, s / v / sqrt_v))) / 3.0;
+        d[0] = I1_3 + 2.0 * sqrt_v * std::cos(angle);
+        d[1] = I1_3 - 2.0 * sqrt_v * std::cos(3.14159265358979323846 / 3.0 + angle);
+        d[2] = I1 - d[0] - d[1];
+    }
+    for (int i = 0; i < 3; ++i) {
+        double* e = V + 3 * i;
+        const double Ai = A[0] - d[i], Bi = A[4] - d[i], Ci = A[8] - d[i];
+        const double q1 = A2A1 - Ai * A[5], q2 = A1A5 - Bi * A[2], q3 = A2A5 - Ci * A[1];
+        e[0] = q2 * q3;
+        e[1] = q3 * q1;
+        e[2] = q2 * q1;
+        const double length = std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+        e[0] /= length;
+        e[1] /= length;
+        e[2] /= length;
     }
 }
 
 // estimate_affine_param (numerical/transformation.hpp:933-1021): initial
-// affine from the centroid and second moments of the non-zero voxels.  The
-// eigenvector signs and the eigen solver differ from the reference's, so
-// the initial guess agrees with it to within a principal-axis flip at most
-// (the result is only a starting point for the optimiser).
+// affine from the centroid and second moments of the non-zero voxels, in the
+// reference's double arithmetic and order.
 template <class I>
 void estimate_affine_param(const I& src, const vector<3>& svs, const I& tgt, const vector<3>& tvs,
                            affine_param<float, 3>& arg)
@@ -386,25 +447,22 @@ void estimate_affine_param(const I& src, const vector<3>& svs, const I& tgt, con
     }
 }
 
-// The body of tipl::reg::linear (reg/linear.hpp:622-700) on volumes already on
-// the device.  from / to are the host uint8 images (has_mask and the moment
-// estimate read them); dev_from / dev_to are the same images on the device.
-template <typename out_type = void>
-float linear_impl(const std::vector<const_pointer_image<3, unsigned char>>& from, vector<3> from_vs,
-                  const std::vector<const_pointer_image<3, unsigned char>>& to, vector<3> to_vs,
-                  const std::vector<std::shared_ptr<hip::volume>>& dev_from,
-                  const std::vector<std::shared_ptr<hip::volume>>& dev_to,
-                  affine_param<float, 3>& arg, linear_reg_param param, bool& terminated,
-                  const cost_evaluator& eval, size_t* evaluations = nullptr)
+// The body of tipl::reg::linear (reg/linear.hpp:622-700) over any level
+// type (see linear_level).  from / to are the host uint8 images (has_mask and
+// the moment estimate read them); `base` prices candidates on the same
+// images, its voxel sizes are set here.
+template <typename out_type = void, class Level>
+float linear_search(const std::vector<const_pointer_image<3, unsigned char>>& from, vector<3> from_vs,
+                    const std::vector<const_pointer_image<3, unsigned char>>& to, vector<3> to_vs,
+                    const Level& base, affine_param<float, 3>& arg, linear_reg_param param,
+                    bool& terminated, size_t* evaluations = nullptr)
 {
     auto term = [&] { return bool(terminated); };
     auto run = [&](const vector<3>& f_vs, const vector<3>& t_vs, affine_param<float, 3>& a, bool mr) {
-        linear_level lv;
-        lv.from = dev_from;
-        lv.to = dev_to;
+        Level lv = base;
         lv.from_vs = f_vs;
         lv.to_vs = t_vs;
-        runner r(eval, std::move(lv), a);
+        basic_runner<Level> r(std::move(lv), a);
         r.set_bound(param.reg_type, param.bound, param.absolute_bound);
         r.search_count = param.search_count;
         r.cost = param.cost_type;
@@ -440,6 +498,23 @@ float linear_impl(const std::vector<const_pointer_image<3, unsigned char>>& from
         arg = transformation_matrix<float, 3>(surrogate, from[0].shape(), from_vs, to[0].shape(), new_to_vs)
                   .to_affine_param(from[0].shape(), from_vs, to[0].shape(), to_vs);
     return result;
+}
+
+// linear_search on volumes already on the device, priced by `eval`.
+template <typename out_type = void>
+float linear_impl(const std::vector<const_pointer_image<3, unsigned char>>& from, vector<3> from_vs,
+                  const std::vector<const_pointer_image<3, unsigned char>>& to, vector<3> to_vs,
+                  const std::vector<std::shared_ptr<hip::volume>>& dev_from,
+                  const std::vector<std::shared_ptr<hip::volume>>& dev_to,
+                  affine_param<float, 3>& arg, linear_reg_param param, bool& terminated,
+                  const cost_evaluator& eval, size_t* evaluations = nullptr)
+{
+    linear_level base;
+    base.from = dev_from;
+    base.to = dev_to;
+    base.eval = eval;
+    return linear_search<out_type>(from, from_vs, to, to_vs, base, arg, param, terminated,
+                                   evaluations);
 }
 
 }  // namespace detail

@@ -55,6 +55,108 @@ void from_field(const field_t& F, float* f)
     std::memcpy(f, &F[0][0], sizeof(float) * 3 * F.size());
 }
 
+// Cost callbacks supplied by the tests: a candidate set of K 12-float
+// transforms (sr[9], shift[3]) on one image pair, and a plain function of
+// parameter vectors (the quadratic-bowl checks of the optimisers).
+typedef void (*tref_cost_cb)(const uint8_t* from, const int* fdim, const uint8_t* to,
+                             const int* tdim, const float* T, int K, int cost, int bins,
+                             double* costs, void* user);
+typedef void (*tref_param_cb)(const float* x, int n, int K, double* costs, void* user);
+
+// reg::linear with line search and L-BFGS run one after the other instead of
+// racing on two threads (reg/linear.hpp:446-499): line_search to completion,
+// then lbfgs from the better of the two points.  Every piece is the
+// reference's own -- linear_reg_runner's bounds and stage list, line_search,
+// lbfgs, transformation_matrix, downsample_with_padding, has_mask,
+// estimate_affine_param -- only the schedule is fixed, so the run is
+// deterministic; costs come from the callback.
+struct sequential_linear {
+    using runner_t = tipl::reg::linear_reg_runner<3, uint8_t, void>;
+    tref_cost_cb cb;
+    void* user;
+    int cost;
+    unsigned count = 0;
+
+    double fun(runner_t& r, const float* x)
+    {
+        tipl::affine_param<float, 3> p;
+        std::copy_n(x, 12, p.begin());
+        ++count;
+        double c = 0.0f;
+        for (size_t i = 0; i < r.from.size(); ++i) {
+            tipl::transformation_matrix<float, 3> trans(p, r.from[i].shape(), r.from_vs,
+                                                       r.to[i].shape(), r.to_vs);
+            const auto& fs = r.from[i].shape();
+            const auto& ts = r.to[i].shape();
+            const int fd[3] = {int(fs[0]), int(fs[1]), int(fs[2])};
+            const int td[3] = {int(ts[0]), int(ts[1]), int(ts[2])};
+            double ci = 0.0;
+            cb(r.from[i].data(), fd, r.to[i].data(), td, &trans[0], 1, cost, 256, &ci, user);
+            c += ci;
+        }
+        c /= r.from.size() * 2.0f;
+        return c;
+    }
+
+    float run_optimize(runner_t& r, unsigned cur_search_count)
+    {
+        auto f = [&](const auto& x) { return fun(r, &*x.begin()); };
+        auto never = [] { return false; };
+        double optimal_value = f(r.arg_min);
+        if (cur_search_count) {
+            for (size_t iter = 0; iter < r.reg_list.size(); ++iter) {
+                auto cur_bound = r.get_current_bound(r.reg_list[iter]);
+                auto arg_min2 = r.arg_min;
+                double optimal_value2 = optimal_value;
+                tipl::optimization::line_search(arg_min2.begin(), arg_min2.end(),
+                                                cur_bound.first.begin(), cur_bound.second.begin(),
+                                                {0, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1}, f,
+                                                optimal_value2, int(cur_search_count), never);
+                if (optimal_value2 < optimal_value) {
+                    optimal_value = optimal_value2;
+                    r.arg_min = arg_min2;
+                }
+                tipl::optimization::lbfgs(r.arg_min.begin(), r.arg_min.end(), cur_bound.first.begin(),
+                                          cur_bound.second.begin(), f, optimal_value, never,
+                                          r.precision, int(r.max_iterations));
+            }
+        } else {
+            auto cur_bound = r.get_current_bound(r.reg_list.back());
+            tipl::optimization::lbfgs(r.arg_min.begin(), r.arg_min.end(), cur_bound.first.begin(),
+                                      cur_bound.second.begin(), f, optimal_value, never,
+                                      r.precision * 0.2f, int(r.max_iterations));
+        }
+        return optimal_value;
+    }
+
+    // run_optimize_mr (reg/linear.hpp:527-550); the private down_sampling is
+    // the same downsample_with_padding calls on the runner's images.
+    float run_optimize_mr(runner_t& r)
+    {
+        const size_t sz = r.from[0].size();
+        const bool is_large = sz > std::pow(96, 3);
+        if (is_large) {
+            runner_t low(r);
+            std::vector<tipl::image<3, uint8_t>> buf(low.from.size() + low.to.size());
+            for (size_t i = 0; i < buf.size(); ++i)
+                tipl::downsample_with_padding(i < low.from.size() ? low.from[i]
+                                                                  : low.to[i - low.from.size()],
+                                              buf[i]);
+            for (size_t i = 0; i < low.from.size(); ++i)
+                low.from[i] = tipl::const_pointer_image<3, uint8_t>(&buf[i][0], buf[i].shape());
+            for (size_t i = 0; i < low.to.size(); ++i) {
+                auto& b = buf[low.from.size() + i];
+                low.to[i] = tipl::const_pointer_image<3, uint8_t>(&b[0], b.shape());
+            }
+            low.from_vs *= 2.0f;
+            low.to_vs *= 2.0f;
+            const float v = run_optimize_mr(low);
+            if (sz > std::pow(256, 3)) return v;
+        }
+        return run_optimize(r, is_large ? 0 : r.search_count);
+    }
+};
+
 }  // namespace
 
 extern "C" {
@@ -390,6 +492,108 @@ float tref_linear(const uint8_t* from, const int* fdim, const float* fvs, const 
                                       terminated);
     std::copy_n(a.begin(), 12, arg);
     return r;
+}
+
+// reg::linear (reg/linear.hpp:622-700) with the sequential schedule above;
+// arg (12 floats) is in/out, *count receives the number of cost evaluations.
+float tref_linear_sequential(const uint8_t* from, const int* fdim, const float* fvs,
+                             const uint8_t* to, const int* tdim, const float* tvs, int cost_type,
+                             int reg_type, float* arg, tref_cost_cb cb, void* user,
+                             unsigned* count)
+{
+    std::vector<tipl::const_pointer_image<3, uint8_t>> f{tipl::const_pointer_image<3, uint8_t>(from, shp(fdim))};
+    std::vector<tipl::const_pointer_image<3, uint8_t>> t{tipl::const_pointer_image<3, uint8_t>(to, shp(tdim))};
+    const tipl::vector<3> from_vs(fvs), to_vs(tvs);
+    tipl::affine_param<float, 3> a(arg);
+    tipl::reg::linear_reg_param param;
+    param.cost_type = cost_type ? tipl::reg::mutual_info : tipl::reg::corr;
+    param.reg_type = tipl::reg::reg_type(reg_type);
+    sequential_linear seq{cb, user, cost_type};
+    tipl::vector<3> new_to_vs = to_vs;
+    tipl::affine_param<float, 3> surrogate = a;
+    bool adjust_vs = false;
+    if (param.reg_type == tipl::reg::affine && tipl::has_mask(f[0]) && tipl::has_mask(t[0])) {
+        tipl::estimate_affine_param(f[0], from_vs, t[0], to_vs, surrogate);
+        a = surrogate;
+        for (int i = 0; i < 3; ++i) {
+            new_to_vs[i] = to_vs[i] / std::max<float>(surrogate.scaling[i], 0.01f);
+            surrogate.scaling[i] = 1.0f;
+        }
+        adjust_vs = true;
+    }
+    param.absolute_bound = true;
+    sequential_linear::runner_t reg(f, t, adjust_vs ? surrogate : a);
+    reg.from_vs = from_vs;
+    reg.to_vs = new_to_vs;
+    reg.set(param);
+    // one thread: the optimisers' par_for loops evaluate in index order
+    const int saved = tipl::max_thread_count;
+    tipl::max_thread_count = 1;
+    const float result = seq.run_optimize_mr(reg);
+    tipl::max_thread_count = saved;
+    if (adjust_vs)
+        a = tipl::transformation_matrix<float, 3>(surrogate, f[0], from_vs, t[0], new_to_vs)
+                .to_affine_param(f[0], from_vs, t[0], to_vs);
+    std::copy_n(a.begin(), 12, arg);
+    if (count) *count = seq.count;
+    return result;
+}
+
+// has_mask (numerical/basic_op.hpp:811-830).
+int tref_has_mask(const uint8_t* I, const int* dim)
+{
+    return tipl::has_mask(tipl::const_pointer_image<3, uint8_t>(I, shp(dim))) ? 1 : 0;
+}
+
+// estimate_affine_param (numerical/transformation.hpp:933-1021).
+void tref_estimate_affine_param(const uint8_t* src, const int* sdim, const float* svs,
+                                const uint8_t* tgt, const int* tdim, const float* tvs, float* arg)
+{
+    tipl::const_pointer_image<3, uint8_t> S(src, shp(sdim)), T(tgt, shp(tdim));
+    tipl::affine_param<float, 3> a;
+    tipl::estimate_affine_param(S, tipl::vector<3>(svs), T, tipl::vector<3>(tvs), a);
+    std::copy_n(a.begin(), 12, arg);
+}
+
+// optimization::line_search / lbfgs (numerical/optimization.hpp:137-512) on
+// n parameters with one thread (their par_for loops then run in index
+// order); x is in/out, the optimal value is returned.
+double tref_line_search(float* x, int n, const float* upper, const float* lower,
+                        const int* strategy, int grid, double optimal, tref_param_cb cb, void* user)
+{
+    const int saved = tipl::max_thread_count;
+    tipl::max_thread_count = 1;
+    std::vector<float> X(x, x + n), U(upper, upper + n), L(lower, lower + n);
+    std::vector<int> S(strategy, strategy + n);
+    auto fun = [&](const std::vector<float>& p) {
+        double c = 0.0;
+        cb(p.data(), n, 1, &c, user);
+        return c;
+    };
+    tipl::optimization::line_search(X.begin(), X.end(), U.begin(), L.begin(), S, fun, optimal, grid,
+                                    [] { return false; });
+    tipl::max_thread_count = saved;
+    std::copy(X.begin(), X.end(), x);
+    return optimal;
+}
+
+double tref_lbfgs(float* x, int n, const float* upper, const float* lower, double fun_x,
+                  double precision, int max_iteration, tref_param_cb cb, void* user)
+{
+    const int saved = tipl::max_thread_count;
+    tipl::max_thread_count = 1;
+    std::vector<float> X(x, x + n), U(upper, upper + n), L(lower, lower + n);
+    auto fun = [&](const std::vector<float>& p) {
+        double c = 0.0;
+        cb(p.data(), n, 1, &c, user);
+        return c;
+    };
+    auto never = [] { return false; };
+    tipl::optimization::lbfgs(X.begin(), X.end(), U.begin(), L.begin(), fun, fun_x, never, precision,
+                              max_iteration);
+    tipl::max_thread_count = saved;
+    std::copy(X.begin(), X.end(), x);
+    return fun_x;
 }
 
 // Post-registration warping (numerical/dif.hpp) and mm_reg's r values.
