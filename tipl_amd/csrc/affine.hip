@@ -18,6 +18,7 @@
 // Integer results (sums, histograms) are accumulated with integer atomics
 // only, so they are exact and independent of scheduling and batching.
 #include "device.hpp"
+#include "host_wait.hpp"
 #include "internal.hpp"
 
 #include <algorithm>
@@ -1095,6 +1096,10 @@ void cost_jobs(tiplhip_ctx* ctx, std::vector<PairJob> jobs, const float* T, int 
     // (which also reports a failed launch).
     uint64_t* slots = static_cast<uint64_t*>(ctx->h_costs.p);
     std::fill(slots, slots + K, kCostPending);
+    // from here on a kernel may be queued that stores into the slots: if a
+    // later step throws, drain the stream before the caller moves on
+    auto drain = [ctx] { (void)hipStreamSynchronize(ctx->stream); };
+    DrainOnUnwind<decltype(drain)> unwind(drain);
     if (ctx->use_graphs && !ctx->kernel_timing) {
         // the same launch shape replays one graph; only the staged bytes
         // (transforms, crop flags, segment table) are new
@@ -1140,20 +1145,9 @@ void cost_jobs(tiplhip_ctx* ctx, std::vector<PairJob> jobs, const float* T, int 
     } else {
         enqueue();
     }
-    const volatile uint64_t* seen = slots;
-    size_t ready = 0;
-    const auto t0 = std::chrono::steady_clock::now();
-    while (ready < K) {
-        if (seen[ready] != kCostPending) {
-            ++ready;
-            continue;
-        }
-        if (std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(kCostSpinUs)) {
-            TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
-            break;
-        }
-        __builtin_ia32_pause();
-    }
+    wait_for_slots(static_cast<const volatile uint64_t*>(slots), K, kCostPending,
+                   std::chrono::microseconds(kCostSpinUs),
+                   [ctx] { TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream)); });
     std::memcpy(costs, ctx->h_costs.p, sizeof(double) * K);
 }
 

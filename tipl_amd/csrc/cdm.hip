@@ -20,6 +20,7 @@
 // read back per iteration; everything else stays on the device.
 #include "device.hpp"
 #include "field.hpp"
+#include "host_wait.hpp"
 #include "internal.hpp"
 
 #include <algorithm>
@@ -1060,6 +1061,11 @@ bool cdm_level(CdmRun& run, const std::vector<const uint8_t*>& It,
     DevMem scal(ctx, sizeof(float) * 16 + sizeof(unsigned int));
     float* d_cost = scal.as<float>();
     unsigned int* d_maxlen = reinterpret_cast<unsigned int*>(scal.as<float>() + 16);
+    // the cost reductions store into ctx->h_cdm: if a step throws while one
+    // is queued, drain the stream before the slots (and the buffers above)
+    // can be reused
+    auto drain = [ctx] { (void)hipStreamSynchronize(ctx->stream); };
+    DrainOnUnwind<decltype(drain)> unwind(drain);
     copy_d2d(ctx, cur.p, best_d, fb);
 
     float theta = 0.0f;
@@ -1093,23 +1099,10 @@ bool cdm_level(CdmRun& run, const std::vector<const uint8_t*>& It,
         const bool need_theta = theta == 0.0f;
         float* solved = launch_poisson(ctx, nd.as<float>(), pa.as<float>(), pb.as<float>(), g, 12,
                                        need_theta ? d_maxlen : nullptr, maxparts.as<float>());
-        {
-            const volatile uint32_t* seen = slots;
-            int ready = 0;
-            const auto t0 = std::chrono::steady_clock::now();
-            while (ready < run.nmod) {
-                if (seen[ready] != kCostPending) {
-                    ++ready;
-                    continue;
-                }
-                if (std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(kCostSpinUs)) {
-                    TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
-                    break;
-                }
-                __builtin_ia32_pause();
-            }
-            for (int i = 0; i < run.nmod; ++i) std::memcpy(&mod_cost[size_t(i)], &slots[i], sizeof(float));
-        }
+        wait_for_slots(static_cast<const volatile uint32_t*>(slots), size_t(run.nmod), kCostPending,
+                       std::chrono::microseconds(kCostSpinUs),
+                       [ctx] { TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream)); });
+        for (int i = 0; i < run.nmod; ++i) std::memcpy(&mod_cost[size_t(i)], &slots[i], sizeof(float));
         float sum_cost = 0.0f;
         for (int i = 0; i < run.nmod; ++i) {
             sum_cost += mod_cost[size_t(i)];

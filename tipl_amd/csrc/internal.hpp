@@ -13,6 +13,7 @@
 #include <string>
 
 #include "../../include/tiplhip.h"
+#include "errors.hpp"
 
 // Every FP32 operation on the bit-exact paths must be a separately rounded
 // IEEE operation in source order (the reference's uint8 truncation after
@@ -23,14 +24,6 @@
 
 namespace tiplhip {
 
-struct Error : std::runtime_error {
-    int code;
-    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-void set_last_error(const std::string& msg);
-void clear_last_error();
-
 #define TIPLHIP_CHECK(expr)                                                                      \
     do {                                                                                         \
         hipError_t e_ = (expr);                                                                  \
@@ -39,34 +32,6 @@ void clear_last_error();
                                                              : TIPLHIP_ERR_DEVICE,               \
                                    std::string(#expr) + ": " + hipGetErrorString(e_));           \
     } while (0)
-
-#define TIPLHIP_REQUIRE(cond, msg)                                                               \
-    do {                                                                                         \
-        if (!(cond)) throw ::tiplhip::Error(TIPLHIP_ERR_INVALID, (msg));                         \
-    } while (0)
-
-// Runs f, converting exceptions into status codes and the thread's last error.
-template <class F>
-int guarded(F&& f)
-{
-    try {
-        clear_last_error();
-        f();
-        return TIPLHIP_OK;
-    } catch (const Error& e) {
-        set_last_error(e.what());
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        set_last_error("host allocation failed");
-        return TIPLHIP_ERR_MEMORY;
-    } catch (const std::exception& e) {
-        set_last_error(e.what());
-        return TIPLHIP_ERR_INVALID;
-    } catch (...) {
-        set_last_error("unknown error");
-        return TIPLHIP_ERR_INVALID;
-    }
-}
 
 // Device allocation that only grows.
 struct DeviceBuffer {
