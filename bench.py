@@ -51,63 +51,76 @@ def time_steps(ctx, fn, steps, warmup):
     return time.perf_counter() - t0
 
 
-def cpu_baseline(F, M, Ts, seconds_target=15.0):
-    """The reference's CPU MI cost (oracle/_ref, 256 bins) on a bounded
-    sample of the same candidate set, candidates spread over host threads
-    as the reference optimisers do (each evaluation on one thread); the
-    64-candidate set is repeated until about `seconds_target` of CPU work."""
-    from oracle.bindings import load_oracle, load_reference
-    threads = max(1, min(16, os.cpu_count() or 1))
-    ref = load_reference()
-    if ref is not None:
-        ref.set_threads(threads)
-        kind = "reference"
-        t0 = time.perf_counter()
-        ref.cost_batch("mi", F, M, Ts[:threads])
-        one = time.perf_counter() - t0
-        reps = max(1, int(round(seconds_target * threads / max(one * len(Ts), 1e-3))))
-        T = np.tile(np.asarray(Ts, np.float32).reshape(-1, 12), (reps, 1))
-        t0 = time.perf_counter()
-        ref.cost_batch("mi", F, M, T)
-        dt = time.perf_counter() - t0
-        K = len(T)
-    else:
-        orc = load_oracle()
-        kind, threads = "port", 1
-        t0 = time.perf_counter()
-        K = 0
-        while time.perf_counter() - t0 < seconds_target:
-            orc.mi_cost(F, M, Ts[K % len(Ts)])
-            K += 1
-        dt = time.perf_counter() - t0
-    evals = K / dt
-    return {"value": round(F.size * evals / 1e9, 4), "unit": "GVox/s", "cores": threads,
-            "kind": kind, "evals_per_s": round(evals, 3),
-            "sample": f"{K} MI (256-bin) evaluations of the 182x218x182 pair "
-                      f"({len(Ts)} candidates repeated), {threads} threads, {dt:.1f} s"}
+# The GPU box allots 16 host cores to each GPU; the CPU legs use at most that
+# many threads (fewer if the process may run on fewer cores).
+CPU_SHARE = 16
 
 
-def cpu_corr_baseline(F, M, Ts, seconds_target=12.0):
-    """The reference's CPU correlation cost (resample + int64 sums) on a bounded sample."""
+def cpu_info():
+    """The host's CPU model, core counts and the thread count the CPU legs use."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = os.cpu_count() or 1
+    return {"model": model, "nproc": os.cpu_count(), "affinity": affinity,
+            "threads": max(1, min(CPU_SHARE, affinity))}
+
+
+def ref_cost_rate(kind, F, M, Ts, threads, seconds_target):
+    """The reference's CPU cost (oracle/_ref: "mi" 256 bins or "corr") on a
+    bounded sample of the candidate set, candidates spread over `threads` host
+    threads as the reference's optimisers do (each evaluation on one thread);
+    the set is repeated until about `seconds_target` of work.  None without
+    the reference build."""
     from oracle.bindings import load_reference
     ref = load_reference()
     if ref is None:
         return None
-    threads = max(1, min(16, os.cpu_count() or 1))
     ref.set_threads(threads)
     t0 = time.perf_counter()
-    ref.cost_batch("corr", F, M, Ts[:threads])
-    one = time.perf_counter() - t0
-    reps = max(1, int(round(seconds_target * threads / max(one * len(Ts), 1e-3))))
-    T = np.tile(np.asarray(Ts, np.float32).reshape(-1, 12), (reps, 1))
+    ref.cost_batch(kind, F, M, Ts[:threads])
+    one = (time.perf_counter() - t0) / max(1, min(threads, len(Ts)))  # one evaluation per thread
+    n = max(threads, int(round(seconds_target * threads / max(one, 1e-4))))
+    Ta = np.asarray(Ts, np.float32).reshape(-1, 12)
+    T = np.tile(Ta, (-(-n // len(Ta)), 1))[:n]
     t0 = time.perf_counter()
-    ref.cost_batch("corr", F, M, T)
+    ref.cost_batch(kind, F, M, T)
     dt = time.perf_counter() - t0
     evals = len(T) / dt
+    name = "MI (256-bin)" if kind == "mi" else "correlation"
     return {"value": round(F.size * evals / 1e9, 4), "unit": "GVox/s", "cores": threads,
             "kind": "reference", "evals_per_s": round(evals, 3),
-            "sample": f"{len(T)} correlation evaluations ({len(Ts)} candidates repeated), "
-                      f"{threads} threads, {dt:.1f} s"}
+            "sample": f"{len(T)} {name} evaluations of the {'x'.join(map(str, F.shape[::-1]))} "
+                      f"pair (the {len(Ta)} candidates in turn), {threads} threads, {dt:.1f} s"}
+
+
+def cpu_baseline(F, M, Ts, threads, seconds_target=15.0):
+    """The reference's CPU MI cost, or the oracle's (one thread) without the
+    reference build."""
+    r = ref_cost_rate("mi", F, M, Ts, threads, seconds_target)
+    if r is not None:
+        return r
+    from oracle.bindings import load_oracle
+    orc = load_oracle()
+    t0 = time.perf_counter()
+    K = 0
+    while time.perf_counter() - t0 < seconds_target:
+        orc.mi_cost(F, M, Ts[K % len(Ts)])
+        K += 1
+    dt = time.perf_counter() - t0
+    evals = K / dt
+    return {"value": round(F.size * evals / 1e9, 4), "unit": "GVox/s", "cores": 1, "kind": "port",
+            "evals_per_s": round(evals, 3),
+            "sample": f"{K} MI (256-bin) oracle evaluations, 1 thread, {dt:.1f} s"}
 
 
 def profiled_traffic(kernel):
@@ -121,18 +134,19 @@ def profiled_traffic(kernel):
         return None
 
 
-def cpu_cdm_baseline(It, Is, seconds_target=12.0):
+def cpu_cdm_baseline(It, Is, threads, seconds_target=12.0):
     """The reference's cdm() loop body at one 182x218x182 level (its own step
     functions and par_for threads), timed over a bounded number of iterations."""
     from oracle.bindings import load_reference
-    threads = max(1, min(16, os.cpu_count() or 1))
     ref = load_reference()
     if ref is None:
         return None
     ref.set_threads(threads)
     one = ref.cdm_iterations(It, Is, 1)
     iters = max(1, min(20, int(seconds_target / max(one, 1e-3))))
-    dt = ref.cdm_iterations(It, Is, iters)
+    # one iteration already about the sample's length: it is the sample
+    dt = one if iters <= 2 else ref.cdm_iterations(It, Is, iters)
+    iters = 1 if iters <= 2 else iters
     return {"value": round(iters / dt, 4), "unit": "iter/s", "cores": threads, "kind": "reference",
             "sample": f"{iters} iterations of the reference cdm() loop body at 182x218x182, "
                       f"{threads} threads, {dt:.1f} s"}
@@ -310,44 +324,72 @@ def serving_bench(reqs, slots, rate_factor, seed=4000):
     }
 
 
-def cdm_config2(ctx):
-    """CDM at 160x192x160: the tissue phantom warped by 30 RBF displacements
-    (max 6 voxels, Jacobian > 0); fixed schedule of 3 levels x 48 iterations
-    with the field smoothed by a sigma-2 binomial gaussian each iteration.
-    Reports iterations/s over the whole run and the recovered field's error:
-    moving(x) = fixed(x + d(x)), and the CDM field f should satisfy
-    f(x) + d(x + f(x)) = 0."""
+def field_errors(fixed, d, f, step=2):
+    """Errors of a CDM field f against the generating displacement d
+    (moving(x) = fixed(x + d(x)); the CDM field should satisfy
+    f(x) + d(x + f(x)) = 0), on every `step`-th voxel:
+      * rms over the phantom (fixed > 150), and the truth's rms there;
+      * at edge voxels (the top 10% of |grad fixed|, sigma 1): the rms of the
+        residual along the image gradient -- the component an image
+        registration can observe -- and the truth's rms along it.
+    Inside the piecewise-constant phantom the tangential and interior parts
+    of d leave the images unchanged, so no image cost can recover them."""
     from scipy import ndimage
-    from tipl_amd import phantoms
-    fixed, moving, d = phantoms.warped_pair(2000, (160, 192, 160))
-    vt, vs = ctx.volume(fixed, 255.99), ctx.volume(moving, 255.99)
-    p = ctx.cdm_param(levels=3, level_iterations=48, field_bandwidth=8)
-    ctx.cdm(vt, vs, p)  # warm-up
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    rc, f, n = ctx.cdm(vt, vs, p)
-    dt = time.perf_counter() - t0
-    # error on every 2nd voxel inside the phantom
-    sl = tuple(slice(0, s, 2) for s in fixed.shape)
-    zz, yy, xx = np.meshgrid(*[np.arange(0, s, 2, dtype=np.float64) for s in fixed.shape], indexing="ij")
+    sl = tuple(slice(0, s, step) for s in fixed.shape)
+    zz, yy, xx = np.meshgrid(*[np.arange(0, s, step, dtype=np.float64) for s in fixed.shape],
+                             indexing="ij")
     fs = f[sl].astype(np.float64)
     coords = [zz + fs[..., 2], yy + fs[..., 1], xx + fs[..., 0]]
     dd = np.stack([ndimage.map_coordinates(d[..., k], coords, order=1, mode="nearest")
                    for k in range(3)], -1)
-    It8, Is8 = vt.read().astype(np.float64), vs.read()
-    Js8 = ctx.compose(Is8, f).astype(np.float64)
-    corr_before = float(np.corrcoef(It8.reshape(-1), Is8.astype(np.float64).reshape(-1))[0, 1])
-    corr_after = float(np.corrcoef(It8.reshape(-1), Js8.reshape(-1))[0, 1])
+    res = fs + dd
     inside = fixed[sl] > 150.0
-    err = np.sqrt(((fs + dd) ** 2).sum(-1))[inside]
-    mag = np.sqrt((d[sl] ** 2).sum(-1))[inside]
-    return {"shape": [160, 192, 160], "levels": 3, "iterations_per_level": 48,
-            "field_sigma_voxels": 2.0, "status": int(rc), "iterations": int(n),
-            "seconds": round(dt, 4), "iter_per_s": round(n / dt, 2),
-            "field_rms_error_voxels": round(float(np.sqrt((err ** 2).mean())), 4),
-            "true_displacement_rms_voxels": round(float(np.sqrt((mag ** 2).mean())), 4),
-            "image_correlation_before": round(corr_before, 4),
-            "image_correlation_after": round(corr_after, 4)}
+    fx = fixed.astype(np.float64)
+    g = np.stack([ndimage.gaussian_filter(fx, 1.0, order=o)[sl]
+                  for o in ((0, 0, 1), (0, 1, 0), (1, 0, 0))], -1)   # (gx, gy, gz)
+    gm = np.linalg.norm(g, axis=-1)
+    edge = gm > np.percentile(gm, 90)
+    gn = g / (gm[..., None] + 1e-12)
+    rms = lambda v: round(float(np.sqrt((v ** 2).mean())), 4)
+    return {"field_rms_error_voxels": rms(np.linalg.norm(res, axis=-1)[inside]),
+            "true_displacement_rms_voxels": rms(np.linalg.norm(d[sl], axis=-1)[inside]),
+            "edge_normal_rms_error_voxels": rms((res * gn).sum(-1)[edge]),
+            "edge_normal_true_rms_voxels": rms((d[sl] * gn).sum(-1)[edge])}
+
+
+def cdm_config2(ctx):
+    """CDM at 160x192x160: the tissue phantom warped by 30 RBF displacements
+    (max 6 voxels, Jacobian > 0), registered two ways:
+      (i)  the reference's cdm(): cdm_param defaults, its pyramid, cost-driven
+           iteration control and cdm_smooth annealing (reg/cdm.hpp:36-44, 373-474);
+      (ii) the fixed schedule: 3 levels x 48 iterations, the field smoothed by
+           a sigma-2 binomial gaussian each iteration.
+    Reports iterations/s over each whole run and the recovered field's errors."""
+    from tipl_amd import phantoms
+    fixed, moving, d = phantoms.warped_pair(2000, (160, 192, 160))
+    vt, vs = ctx.volume(fixed, 255.99), ctx.volume(moving, 255.99)
+    It8, Is8 = vt.read().astype(np.float64), vs.read()
+    corr_before = float(np.corrcoef(It8.reshape(-1), Is8.astype(np.float64).reshape(-1))[0, 1])
+    out = {"shape": [160, 192, 160], "image_correlation_before": round(corr_before, 4)}
+    modes = {"reference_cdm": ctx.cdm_param(),
+             "fixed_schedule": ctx.cdm_param(levels=3, level_iterations=48, field_bandwidth=8)}
+    for name, p in modes.items():
+        ctx.cdm(vt, vs, p)  # warm-up
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        rc, f, n = ctx.cdm(vt, vs, p)
+        dt = time.perf_counter() - t0
+        Js8 = ctx.compose(Is8, f).astype(np.float64)
+        r = {"status": int(rc), "iterations": int(n), "seconds": round(dt, 4),
+             "iter_per_s": round(n / dt, 2),
+             "image_correlation_after": round(float(np.corrcoef(It8.reshape(-1), Js8.reshape(-1))[0, 1]), 4)}
+        r.update(field_errors(fixed, d, f))
+        if name == "fixed_schedule":
+            r.update({"levels": 3, "iterations_per_level": 48, "field_sigma_voxels": 2.0})
+        out[name] = r
+    vt.free()
+    vs.free()
+    return out
 
 
 def mm_reg_pipeline(ctx):
@@ -490,7 +532,7 @@ def main():
         mp_res = multi_pair_bench(ctx, bpairs)
         log(f"multi-pair {mp_res}")
 
-    log("CDM fixed schedule at 160x192x160")
+    log("CDM at 160x192x160: the reference cdm() and the fixed schedule")
     cdm2 = cdm_config2(ctx)
     log(f"cdm config 2 {cdm2}")
     log("mm_reg nonlinear half at 160x192x160")
@@ -509,16 +551,37 @@ def main():
         log(f"batched registrations {breg}")
 
     cpu = cpu_corr = cpu_cdm = None
+    cpu_1core = {}
+    host = cpu_info()
     if not args.no_cpu:
-        log("CPU reference baselines")
+        log(f"CPU reference baselines on {host}")
         F8 = vf.read()
         M8 = vm.read()
-        cpu = cpu_baseline(F8, M8, Ts)
+        nt = host["threads"]
+        cpu = cpu_baseline(F8, M8, Ts, nt)
         log(f"cpu mi {cpu}")
-        cpu_corr = cpu_corr_baseline(F8, M8, Ts)
+        cpu_corr = ref_cost_rate("corr", F8, M8, Ts, nt, 12.0)
         log(f"cpu corr {cpu_corr}")
-        cpu_cdm = cpu_cdm_baseline(vt.read(), vs.read())
+        cpu_cdm = cpu_cdm_baseline(vt.read(), vs.read(), nt)
         log(f"cpu cdm {cpu_cdm}")
+        # the same legs on one core (shorter samples)
+        cpu_1core = {"mi256": ref_cost_rate("mi", F8, M8, Ts, 1, 5.0),
+                     "corr": ref_cost_rate("corr", F8, M8, Ts, 1, 5.0),
+                     "cdm": cpu_cdm_baseline(vt.read(), vs.read(), 1, 6.0)}
+        log(f"cpu one core {cpu_1core}")
+    # GPU over CPU on the same computation (the CPU legs are the reference's
+    # 256-bin MI, correlation and cdm() loop body)
+    speedup = {}
+    if cpu:
+        speedup["mi256_k64_vs_cpu_all_cores"] = round(res["mi256_k64_gvox_s"] / cpu["value"], 1)
+    if cpu_corr:
+        speedup["corr_k64_vs_cpu_all_cores"] = round(res["corr_k64_gvox_s"] / cpu_corr["value"], 1)
+    if cpu_cdm:
+        speedup["cdm_vs_cpu_all_cores"] = round(cdm_ips / cpu_cdm["value"], 1)
+    for k, r in cpu_1core.items():
+        if r:
+            gpu = {"mi256": res["mi256_k64_gvox_s"], "corr": res["corr_k64_gvox_s"], "cdm": cdm_ips}[k]
+            speedup[f"{k}_vs_cpu_one_core"] = round(gpu / r["value"], 1)
 
     out = {
         "metric": METRIC,
@@ -542,8 +605,11 @@ def main():
         "cdm_shape": list(shape),
         "cdm_cpu_baseline": cpu_cdm,
         "corr_cpu_baseline": cpu_corr,
+        "cpu_one_core": cpu_1core or None,
+        "cpu_host": host,
+        "speedup_same_computation": speedup or None,
         "hbm_copy_gbps": round(copy_gbps, 1),
-        "cdm_fixed_schedule": cdm2,
+        "cdm_config2": cdm2,
         "mm_reg_pipeline": mmreg,
         "multi_pair": mp_res,
         "serving": serve,

@@ -106,6 +106,15 @@ int tiplhip_timer_stop(tiplhip_ctx* ctx, float* ms);
  * counters.  No counterpart in the reference (measurement support). */
 int tiplhip_stream_copy(tiplhip_ctx* ctx, size_t bytes, int reps, double* gbps);
 
+/* Kernels of known HBM traffic over a `bytes` buffer (rounded down to 48),
+ * `reps` launches: kind 0 the copy above (read + write), kind 1 a read of
+ * every byte with 16-byte loads, kind 2 a read of every byte as packed
+ * 12-byte float3 voxels (three 4-byte loads a lane, the field kernels'
+ * pattern).  gbps = bytes moved / time.  With memory counters they give the
+ * factor between a counter and the bytes a kernel really moved, per access
+ * pattern.  No counterpart in the reference (measurement support). */
+int tiplhip_traffic_probe(tiplhip_ctx* ctx, int kind, size_t bytes, int reps, double* gbps);
+
 /* ---- volumes ------------------------------------------------------------- */
 
 /* Upload a volume and keep it resident on the device as uint8.

@@ -26,7 +26,7 @@ EXPORTS = (
     "tiplhip_last_error", "tiplhip_device_count", "tiplhip_ctx_create", "tiplhip_ctx_destroy",
     "tiplhip_ctx_synchronize", "tiplhip_ctx_set_kernel_timing", "tiplhip_ctx_kernel_time",
     "tiplhip_ctx_set_graphs",
-    "tiplhip_timer_start", "tiplhip_timer_stop", "tiplhip_stream_copy", "tiplhip_volume_create",
+    "tiplhip_timer_start", "tiplhip_timer_stop", "tiplhip_stream_copy", "tiplhip_traffic_probe", "tiplhip_volume_create",
     "tiplhip_volume_create_device", "tiplhip_volume_destroy", "tiplhip_volume_dims",
     "tiplhip_volume_read", "tiplhip_volume_stats", "tiplhip_volume_downsample",
     "tiplhip_resample", "tiplhip_corr_sums", "tiplhip_mi_hist", "tiplhip_cost_batch",
@@ -126,6 +126,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "tiplhip_timer_start": (ip, [vp]),
         "tiplhip_timer_stop": (ip, [vp, P(C.c_float)]),
         "tiplhip_stream_copy": (ip, [vp, C.c_size_t, ip, P(C.c_double)]),
+        "tiplhip_traffic_probe": (ip, [vp, ip, C.c_size_t, ip, P(C.c_double)]),
         "tiplhip_volume_create": (ip, [vp, vp, ip, P(ip), C.c_float, P(vp)]),
         "tiplhip_volume_create_device": (ip, [vp, vp, ip, P(ip), C.c_float, P(vp)]),
         "tiplhip_volume_destroy": (ip, [vp]),
@@ -313,6 +314,12 @@ class Context:
         return g.value
 
     # -- volumes -----------------------------------------------------------
+    def traffic_probe(self, kind: int, nbytes: int, reps: int = 20) -> float:
+        """GB/s of a kernel of known traffic: 0 copy, 1 16-byte reads, 2 float3 reads."""
+        g = C.c_double(0)
+        _check(load_library().tiplhip_traffic_probe(self.handle, kind, nbytes, reps, C.byref(g)))
+        return g.value
+
     def volume(self, data: np.ndarray, upper_limit: float = 255.99) -> Volume:
         if data.dtype == np.uint8:
             dt = U8

@@ -300,20 +300,72 @@ __global__ __launch_bounds__(256) void pack8_kernel(const uint8_t* __restrict__ 
     }
 }
 
-__global__ __launch_bounds__(256) void stream_copy_kernel(const uint4* __restrict__ in,
-                                                         uint4* __restrict__ out, size_t n)
+// Kernels of known HBM traffic (bandwidth ceiling and memory-counter
+// calibration).  Each block owns a contiguous chunk; every thread keeps
+// kProbeUnroll 16-byte accesses in flight; loads and stores are
+// non-temporal (streamed data is not kept in the caches).
+constexpr int kProbeThreads = 256;
+constexpr int kProbeUnroll = 8;
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// kind 0: copy, 16 bytes a lane.
+__global__ __launch_bounds__(kProbeThreads) void stream_copy_kernel(const u32x4* __restrict__ in,
+                                                                    u32x4* __restrict__ out, size_t n,
+                                                                    size_t per_block)
 {
-    // four 16-byte loads in flight per thread, then four stores
-    const size_t stride = size_t(gridDim.x) * blockDim.x;
-    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    for (; i + 3 * stride < n; i += 4 * stride) {
-        const uint4 a = in[i], b = in[i + stride], c = in[i + 2 * stride], d = in[i + 3 * stride];
-        out[i] = a;
-        out[i + stride] = b;
-        out[i + 2 * stride] = c;
-        out[i + 3 * stride] = d;
+    const size_t b0 = size_t(blockIdx.x) * per_block, b1 = min(b0 + per_block, n);
+    constexpr size_t step = size_t(kProbeThreads) * kProbeUnroll;
+    size_t i = b0 + threadIdx.x;
+    for (; i + (kProbeUnroll - 1) * kProbeThreads < b1; i += step) {
+        u32x4 v[kProbeUnroll];
+#pragma unroll
+        for (int u = 0; u < kProbeUnroll; ++u) v[u] = __builtin_nontemporal_load(in + i + u * kProbeThreads);
+#pragma unroll
+        for (int u = 0; u < kProbeUnroll; ++u) __builtin_nontemporal_store(v[u], out + i + u * kProbeThreads);
     }
-    for (; i < n; i += stride) out[i] = in[i];
+    for (; i < b1; i += kProbeThreads) __builtin_nontemporal_store(__builtin_nontemporal_load(in + i), out + i);
+}
+
+// kind 1: read-only, 16 bytes a lane; kind 2: read-only as the field
+// kernels read packed float3 voxels (three 4-byte loads a lane at a 12-byte
+// stride).  The xor of everything read goes to `sink` (one word a block),
+// so no load is dropped.
+template <int KIND>
+__global__ __launch_bounds__(kProbeThreads) void stream_read_kernel(const u32x4* __restrict__ in, size_t n,
+                                                                    size_t per_block, uint32_t* __restrict__ sink)
+{
+    const size_t b0 = size_t(blockIdx.x) * per_block, b1 = min(b0 + per_block, n);
+    uint32_t acc = 0;
+    if constexpr (KIND == 1) {
+        constexpr size_t step = size_t(kProbeThreads) * kProbeUnroll;
+        size_t i = b0 + threadIdx.x;
+        for (; i + (kProbeUnroll - 1) * kProbeThreads < b1; i += step) {
+#pragma unroll
+            for (int u = 0; u < kProbeUnroll; ++u) {
+                const u32x4 v = __builtin_nontemporal_load(in + i + u * kProbeThreads);
+                acc ^= v.x ^ v.y ^ v.z ^ v.w;
+            }
+        }
+        for (; i < b1; i += kProbeThreads) {
+            const u32x4 v = in[i];
+            acc ^= v.x ^ v.y ^ v.z ^ v.w;
+        }
+    } else {
+        // the chunk as 12-byte voxels: voxel j at words 3j, 3j+1, 3j+2
+        const uint32_t* w = reinterpret_cast<const uint32_t*>(in);
+        const size_t v0 = b0 * 4 / 3, v1 = b1 * 4 / 3;  // 16-byte units -> 12-byte voxels
+        size_t j = v0 + threadIdx.x;
+        for (; j + (kProbeUnroll - 1) * kProbeThreads < v1; j += size_t(kProbeThreads) * kProbeUnroll) {
+#pragma unroll
+            for (int u = 0; u < kProbeUnroll; ++u) {
+                const size_t q = 3 * (j + size_t(u) * kProbeThreads);
+                acc ^= w[q] ^ w[q + 1] ^ w[q + 2];
+            }
+        }
+        for (; j < v1; j += kProbeThreads) acc ^= w[3 * j] ^ w[3 * j + 1] ^ w[3 * j + 2];
+    }
+    for (int off = 32; off > 0; off >>= 1) acc ^= __shfl_xor(acc, off, 64);
+    if ((threadIdx.x & 63u) == 0) atomicXor(&sink[blockIdx.x], acc);
 }
 
 int grid_for(size_t n, int cap = 16384)
@@ -659,39 +711,67 @@ int tiplhip_timer_stop(tiplhip_ctx* ctx, float* ms)
     });
 }
 
-int tiplhip_stream_copy(tiplhip_ctx* ctx, size_t bytes, int reps, double* gbps)
+int tiplhip_traffic_probe(tiplhip_ctx* ctx, int kind, size_t bytes, int reps, double* gbps)
 {
     return guarded([&] {
         TIPLHIP_REQUIRE(ctx && gbps, "null argument");
-        TIPLHIP_REQUIRE(bytes >= 16 && reps >= 1, "need bytes >= 16 and reps >= 1");
+        TIPLHIP_REQUIRE(kind >= 0 && kind <= 2, "kind must be 0 (copy), 1 or 2 (reads)");
+        TIPLHIP_REQUIRE(bytes >= 192 && reps >= 1, "need bytes >= 192 and reps >= 1");
         std::lock_guard<std::recursive_mutex> lk(ctx->mu);
         DeviceGuard dg(ctx->device);
-        const size_t n = bytes / 16;
+        // whole 48-byte groups (four 12-byte voxels = three 16-byte units)
+        const size_t n = bytes / 48 * 3;
+        // about 8 blocks per CU, each a contiguous chunk (a multiple of 3 units)
+        const size_t blocks = std::max<size_t>(1, std::min<size_t>(size_t(ctx->cu_count) * 8,
+                                                                   n / (3 * kProbeThreads)));
+        const size_t per_block = (n / blocks + 2) / 3 * 3;
+        const size_t nb = (n + per_block - 1) / per_block;
         void *a = nullptr, *b = nullptr;
         TIPLHIP_CHECK(hipMalloc(&a, n * 16));
-        hipError_t e = hipMalloc(&b, n * 16);
+        hipError_t e = hipMalloc(&b, kind == 0 ? n * 16 : nb * sizeof(uint32_t));
         if (e != hipSuccess) {
             (void)hipFree(a);
             TIPLHIP_CHECK(e);
         }
-        const int blocks = int(std::min<size_t>((n + 255) / 256, size_t(ctx->cu_count) * 16));
         auto launch = [&] {
-            hipLaunchKernelGGL(stream_copy_kernel, dim3(blocks), dim3(256), 0, ctx->stream,
-                               static_cast<const uint4*>(a), static_cast<uint4*>(b), n);
+            if (kind == 0)
+                hipLaunchKernelGGL(stream_copy_kernel, dim3(uint32_t(nb)), dim3(kProbeThreads), 0, ctx->stream,
+                                   static_cast<const u32x4*>(a), static_cast<u32x4*>(b), n, per_block);
+            else if (kind == 1)
+                hipLaunchKernelGGL(stream_read_kernel<1>, dim3(uint32_t(nb)), dim3(kProbeThreads), 0,
+                                   ctx->stream, static_cast<const u32x4*>(a), n, per_block,
+                                   static_cast<uint32_t*>(b));
+            else
+                hipLaunchKernelGGL(stream_read_kernel<2>, dim3(uint32_t(nb)), dim3(kProbeThreads), 0,
+                                   ctx->stream, static_cast<const u32x4*>(a), n, per_block,
+                                   static_cast<uint32_t*>(b));
         };
-        (void)hipMemsetAsync(a, 1, n * 16, ctx->stream);
+        e = hipMemsetAsync(a, 1, n * 16, ctx->stream);
+        if (e == hipSuccess && kind != 0) e = hipMemsetAsync(b, 0, nb * sizeof(uint32_t), ctx->stream);
+        if (e != hipSuccess) {
+            (void)hipFree(a);
+            (void)hipFree(b);
+            TIPLHIP_CHECK(e);
+        }
         launch();  // warm up
-        (void)hipEventRecord(ctx->t0, ctx->stream);
-        for (int r = 0; r < reps; ++r) launch();
-        (void)hipEventRecord(ctx->t1, ctx->stream);
-        e = hipEventSynchronize(ctx->t1);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipEventRecord(ctx->t0, ctx->stream);
+        for (int r = 0; r < reps && e == hipSuccess; ++r) launch();
+        if (e == hipSuccess) e = hipEventRecord(ctx->t1, ctx->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(ctx->t1);
         float ms = 0.0f;
         if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->t0, ctx->t1);
+        (void)hipStreamSynchronize(ctx->stream);
         (void)hipFree(a);
         (void)hipFree(b);
         TIPLHIP_CHECK(e);
-        *gbps = 2.0 * double(n * 16) * reps / (double(ms) * 1e-3) / 1e9;
+        *gbps = double(kind == 0 ? 2 : 1) * double(n * 16) * reps / (double(ms) * 1e-3) / 1e9;
     });
+}
+
+int tiplhip_stream_copy(tiplhip_ctx* ctx, size_t bytes, int reps, double* gbps)
+{
+    return tiplhip_traffic_probe(ctx, 0, bytes, reps, gbps);
 }
 
 int tiplhip_volume_create(tiplhip_ctx* ctx, const void* host, int dtype, const int dim[3],

@@ -145,10 +145,7 @@ __global__ __launch_bounds__(256) void gradient_kernel(const uint8_t* __restrict
     const uint32_t tid = threadIdx.y * blockDim.x + threadIdx.x;
     if ((tid & 63u) == 0) part[tid >> 6] = c;
     __syncthreads();
-    if (tid == 0) {
-        const uint32_t b = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-        partials[b] = (part[0] + part[1]) + (part[2] + part[3]);
-    }
+    if (tid == 0) partials[logical_block().lin] = (part[0] + part[1]) + (part[2] + part[3]);
 }
 
 // ---------------------------------------------------------------------------
@@ -233,8 +230,9 @@ __global__ __launch_bounds__(256) void gradient_r_box_kernel(const uint8_t* __re
     __shared__ Sums5 xs[kPY][kTX];
     const int tx = int(threadIdx.x), ty = int(threadIdx.y);
     const int tid = ty * kTX + tx;
-    const int x0 = int(blockIdx.x) * kTX, y0 = int(blockIdx.y) * kTY;
-    const int z0 = int(blockIdx.z) * zc, z1 = min(z0 + zc, g.d);
+    const Block blk = logical_block();
+    const int x0 = blk.x * kTX, y0 = blk.y * kTY;
+    const int z0 = blk.z * zc, z1 = min(z0 + zc, g.d);
     Sums5 ring[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) ring[k] = Sums5{0u, 0u, 0u, 0u, 0u};
@@ -352,10 +350,7 @@ __global__ __launch_bounds__(256) void gradient_r_box_kernel(const uint8_t* __re
     double c = wave_sum_f64(cost_sum);
     if ((tid & 63) == 0) part[tid >> 6] = c;
     __syncthreads();
-    if (tid == 0) {
-        const uint32_t b = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-        partials[b] = (part[0] + part[1]) + (part[2] + part[3]);
-    }
+    if (tid == 0) partials[logical_block().lin] = (part[0] + part[1]) + (part[2] + part[3]);
 }
 
 // Planes per block of the box-sum gradient: kZC for large volumes; on
@@ -466,8 +461,7 @@ __global__ __launch_bounds__(256) void poisson_kernel(const float* __restrict__ 
         if ((tid & 63u) == 0) part[tid >> 6] = m;
         __syncthreads();
         if (tid == 0) {
-            const uint32_t blk = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-            maxparts[blk] = fmaxf(fmaxf(part[0], part[1]), fmaxf(part[2], part[3]));
+            maxparts[logical_block().lin] = fmaxf(fmaxf(part[0], part[1]), fmaxf(part[2], part[3]));
         }
     }
 }
@@ -497,8 +491,9 @@ __global__ __launch_bounds__(kTX * kPTY) void poisson_zm_kernel(const float* __r
 {
     __shared__ float3 plane[kPTY][kTX];
     const int tx = int(threadIdx.x), ty = int(threadIdx.y);
-    const int x = int(blockIdx.x) * kTX + tx, y = int(blockIdx.y) * kPTY + ty;
-    const int z0 = int(blockIdx.z) * kPZ, z1 = min(z0 + kPZ, g.d);
+    const Block blk = logical_block();
+    const int x = blk.x * kTX + tx, y = blk.y * kPTY + ty;
+    const int z0 = blk.z * kPZ, z1 = min(z0 + kPZ, g.d);
     const bool ok = x < g.w && y < g.h;
     const uint32_t n = g.n, w = uint32_t(g.w), wh = g.wh;
     auto A = [&](uint32_t q) -> float3 {
@@ -559,7 +554,7 @@ __global__ __launch_bounds__(kTX * kPTY) void poisson_zm_kernel(const float* __r
         if (tid == 0) {
             float r = part[0];
             for (int i = 1; i < kPTY; ++i) r = fmaxf(r, part[i]);
-            maxparts[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = r;
+            maxparts[blk.lin] = r;
         }
     }
 }
@@ -626,8 +621,9 @@ __global__ __launch_bounds__(kQX * kQY) void poisson2_kernel(const float* __rest
     __shared__ float3 S[3][kS2Y][kS2X];
     __shared__ float3 T[3][kS1Y][kS1X];
     const int tx = int(threadIdx.x), ty = int(threadIdx.y), tid = ty * kQX + tx;
-    const int x0 = int(blockIdx.x) * kQX, y0 = int(blockIdx.y) * kQY;
-    const int z0 = int(blockIdx.z) * zq, z1 = min(z0 + zq, g.d);
+    const Block blk = logical_block();
+    const int x0 = blk.x * kQX, y0 = blk.y * kQY;
+    const int z0 = blk.z * zq, z1 = min(z0 + zq, g.d);
     const long long n = g.n, w = g.w, wh = g.wh;
     const long long base = (long long)y0 * w + x0;
     const float3 zero = make_float3(0.0f, 0.0f, 0.0f);
@@ -746,7 +742,7 @@ __global__ __launch_bounds__(kQX * kQY) void poisson2_kernel(const float* __rest
         if (tid == 0) {
             float r = part[0];
             for (int i = 1; i < kQY; ++i) r = fmaxf(r, part[i]);
-            maxparts[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = r;
+            maxparts[blk.lin] = r;
         }
     }
 }
@@ -756,19 +752,17 @@ __global__ __launch_bounds__(kQX * kQY) void poisson2_kernel(const float* __rest
 // new(x) = m(x + new(x)) - x with m(y) = cur(y) + y by trilinear
 // interpolation; cur(x) where new(x) == 0 or the sample is outside.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void accumulate_kernel(const float* __restrict__ cur,
-                                                         float* __restrict__ nd, float k,
-                                                         FieldGeo g)
+// The accumulated displacement at voxel p = (x, y, z).
+__device__ inline float3 accumulated(const float* __restrict__ cur, const float* __restrict__ nd,
+                                     float k, const FieldGeo& g, uint32_t p, int x, int y, int z)
 {
-    const Vox v = this_voxel(g);
-    if (!v.ok) return;
-    float3 d = ld3(nd, v.p);
+    float3 d = ld3(nd, p);
     d.x = d.x * k;
     d.y = d.y * k;
     d.z = d.z * k;
     if (d.x != 0.0f || d.y != 0.0f || d.z != 0.0f) {
         Tri t;
-        if (locate(g, d.x + float(v.x), d.y + float(v.y), d.z + float(v.z), t)) {
+        if (locate(g, d.x + float(x), d.y + float(y), d.z + float(z), t)) {
             const int ix = t.ix, iy = t.iy, iz = t.iz;
             float3 acc = make_float3(0.0f, 0.0f, 0.0f);
 #pragma unroll
@@ -787,12 +781,88 @@ __global__ __launch_bounds__(256) void accumulate_kernel(const float* __restrict
                     acc.z = acc.z + mz * t.w[c];
                 }
             }
-            st3(nd, v.p,
-                make_float3(acc.x - float(v.x), acc.y - float(v.y), acc.z - float(v.z)));
-            return;
+            return make_float3(acc.x - float(x), acc.y - float(y), acc.z - float(z));
         }
     }
-    st3(nd, v.p, ld3(cur, v.p));
+    return ld3(cur, p);
+}
+
+__global__ __launch_bounds__(256) void accumulate_kernel(const float* __restrict__ cur,
+                                                         float* __restrict__ nd, float k,
+                                                         FieldGeo g)
+{
+    const Vox v = this_voxel(g);
+    if (!v.ok) return;
+    st3(nd, v.p, accumulated(cur, nd, k, g, v.p, v.x, v.y, v.z));
+}
+
+// ---------------------------------------------------------------------------
+// accumulate_displacement then cdm_smooth in one pass, marching z:
+// out = smooth(acc), acc = the accumulated field of accumulate_kernel.
+// Tiles are laid out in flat-index space as in poisson2_kernel, so smooth's
+// neighbours p-+1 and p-+w are offsets (i-+1, j), (i, j-+1) of the tile grown
+// by 1 (row and plane wrap included) and p-+wh the same offset in planes
+// m-+1.  A position outside [0, n) holds +0.0, which adds nothing to the
+// neighbour sum (that sum starts at +0.0 and is never -0.0), as the
+// reference drops it.  Per plane the block computes acc once on the tile and
+// its 148-position ring and keeps three planes in LDS.  HBM traffic per
+// voxel: new_d read, cur gathered, out written (36 B, against 36 + 24 B for
+// the two passes).  out must not alias cur or nd.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kQX * kQY) void accumulate_smooth_kernel(
+    const float* __restrict__ cur, const float* __restrict__ nd, float k, float w_6, float w_1,
+    FieldGeo g, int zq, float* __restrict__ out)
+{
+    __shared__ float3 A[3][kS1Y][kS1X];
+    const int tx = int(threadIdx.x), ty = int(threadIdx.y), tid = ty * kQX + tx;
+    const Block blk = logical_block();
+    const int x0 = blk.x * kQX, y0 = blk.y * kQY;
+    const int z0 = blk.z * zq, z1 = min(z0 + zq, g.d);
+    const long long n = g.n, w = g.w, wh = g.wh;
+    const long long base = (long long)y0 * w + x0;
+    int ri = 0, rj = 0;
+    const bool has_ring = tid < kRing;
+    if (has_ring) ring_offset(tid, ri, rj);
+    const long long ro = (long long)rj * w + ri;
+    const long long io = (long long)ty * w + tx;
+    auto acc_at = [&](long long q) -> float3 {
+        if (q < 0 || q >= n) return make_float3(0.0f, 0.0f, 0.0f);
+        const uint32_t p = uint32_t(q);
+        const uint32_t z = p / g.wh, r = p - z * g.wh;
+        const uint32_t y = r / uint32_t(g.w), x = r - y * uint32_t(g.w);
+        return accumulated(cur, nd, k, g, p, int(x), int(y), int(z));
+    };
+    auto fill = [&](int m) {
+        float3(*dst)[kS1X] = A[(m + 3) % 3];
+        const long long pm = (long long)m * wh + base;
+        dst[ty + 1][tx + 1] = acc_at(pm + io);
+        if (has_ring) dst[rj + 1][ri + 1] = acc_at(pm + ro);
+    };
+    fill(z0 - 1);
+    fill(z0);
+    for (int m = z0; m < z1; ++m) {
+        fill(m + 1);
+        __syncthreads();
+        if (x0 + tx < g.w && y0 + ty < g.h) {
+            float3(*am)[kS1X] = A[(m + 2) % 3];
+            float3(*ac)[kS1X] = A[(m + 3) % 3];
+            float3(*ap)[kS1X] = A[(m + 4) % 3];
+            const int ci = tx + 1, cj = ty + 1;
+            const float3 t[6] = {ac[cj][ci + 1], ac[cj][ci - 1], ac[cj + 1][ci],
+                                 ac[cj - 1][ci], ap[cj][ci],     am[cj][ci]};
+            float3 v = make_float3(0.0f, 0.0f, 0.0f);
+#pragma unroll
+            for (int e = 0; e < 6; ++e) {
+                v.x = v.x + t[e].x;
+                v.y = v.y + t[e].y;
+                v.z = v.z + t[e].z;
+            }
+            const float3 c = ac[cj][ci];
+            st3(out, uint32_t((long long)m * wh + base + io),
+                make_float3(c.x * w_1 + v.x * w_6, c.y * w_1 + v.y * w_6, c.z * w_1 + v.z * w_6));
+        }
+        __syncthreads();
+    }
 }
 
 // cdm_smooth: dd = d*(1-s) + (s/6)*(neighbours +1, -1, +w, -w, +wh, -wh by
@@ -971,6 +1041,18 @@ void launch_accumulate(tiplhip_ctx* ctx, const float* cur, float* nd, float k, c
     TIPLHIP_CHECK(hipGetLastError());
 }
 
+// accumulate then cdm_smooth with `smoothing` (non-zero) in one pass into out.
+void launch_accumulate_smooth(tiplhip_ctx* ctx, const float* cur, const float* nd, float k,
+                              float smoothing, const FieldGeo& g, float* out)
+{
+    const float w_6 = smoothing / float(2.0f * 3.0f);
+    const float w_1 = 1.0f - smoothing;
+    const int zq = poisson2_zq(g);
+    hipLaunchKernelGGL(accumulate_smooth_kernel, poisson2_grid(g, zq), dim3(kQX, kQY), 0, ctx->stream,
+                       cur, nd, k, w_6, w_1, g, zq, out);
+    TIPLHIP_CHECK(hipGetLastError());
+}
+
 void launch_smooth(tiplhip_ctx* ctx, const float* d, float* dd, float smoothing, const FieldGeo& g)
 {
     const float w_6 = smoothing / float(2.0f * 3.0f);
@@ -1055,7 +1137,13 @@ bool cdm_level(CdmRun& run, const std::vector<const uint8_t*>& It,
     }
     if (run.stopped) return true;
 
-    DevMem cur(ctx, fb), nd(ctx, fb), pa(ctx, fb), pb(ctx, fb), Js(ctx, n);
+    // three fields rotate: the running field and the two Jacobi buffers;
+    // accumulate+smooth writes the new running field into the Jacobi buffer
+    // that does not hold the solved new_d
+    DevMem f0(ctx, fb), f1(ctx, fb), f2(ctx, fb), nd(ctx, fb), Js(ctx, n);
+    float* cur = f0.as<float>();
+    float* pa = f1.as<float>();
+    float* pb = f2.as<float>();
     const uint32_t nblocks = block_count(g);
     DevMem partials(ctx, sizeof(double) * nblocks), maxparts(ctx, sizeof(float) * nblocks);
     DevMem scal(ctx, sizeof(float) * 16 + sizeof(unsigned int));
@@ -1066,7 +1154,7 @@ bool cdm_level(CdmRun& run, const std::vector<const uint8_t*>& It,
     // can be reused
     auto drain = [ctx] { (void)hipStreamSynchronize(ctx->stream); };
     DrainOnUnwind<decltype(drain)> unwind(drain);
-    copy_d2d(ctx, cur.p, best_d, fb);
+    copy_d2d(ctx, cur, best_d, fb);
 
     float theta = 0.0f;
     std::vector<float> cost;
@@ -1091,13 +1179,13 @@ bool cdm_level(CdmRun& run, const std::vector<const uint8_t*>& It,
         std::fill(slots, slots + run.nmod, kCostPending);
         float* h_cost = static_cast<float*>(ctx->h_cdm.dev);
         for (int i = 0; i < run.nmod; ++i) {
-            launch_compose(ctx, Is[size_t(i)], cur.as<float>(), g, Js.as<uint8_t>());
+            launch_compose(ctx, Is[size_t(i)], cur, g, Js.as<uint8_t>());
             const char type = run.param.gradient_type[std::min(i, 7)] == 'd' ? 'd' : 'r';
             launch_gradient(ctx, Js.as<uint8_t>(), It[size_t(i)], g, type, i > 0 ? 1 : 0,
                             nd.as<float>(), partials.as<double>(), d_cost + i, h_cost + i);
         }
         const bool need_theta = theta == 0.0f;
-        float* solved = launch_poisson(ctx, nd.as<float>(), pa.as<float>(), pb.as<float>(), g, 12,
+        float* solved = launch_poisson(ctx, nd.as<float>(), pa, pb, g, 12,
                                        need_theta ? d_maxlen : nullptr, maxparts.as<float>());
         wait_for_slots(static_cast<const volatile uint32_t*>(slots), size_t(run.nmod), kCostPending,
                        std::chrono::microseconds(kCostSpinUs),
@@ -1111,11 +1199,11 @@ bool cdm_level(CdmRun& run, const std::vector<const uint8_t*>& It,
         cost.push_back(sum_cost / float(run.nmod));
         if (cost.back() < best_cost) {
             best_cost = cost.back();
-            if (index) copy_d2d(ctx, best_d, cur.p, fb);
+            if (index) copy_d2d(ctx, best_d, cur, fb);
         }
         if (!cdm_improved(cost) && !run.no_early_exit) {
             if (cur_smoothing <= run.param.smoothing) break;
-            copy_d2d(ctx, cur.p, best_d, fb);
+            copy_d2d(ctx, cur, best_d, fb);
             cur_smoothing *= 0.5f;
             cost.clear();
         }
@@ -1126,11 +1214,18 @@ bool cdm_level(CdmRun& run, const std::vector<const uint8_t*>& It,
         }
         if (theta == 0.0f) break;
         const float k = run.param.speed / theta;
-        launch_accumulate(ctx, cur.as<float>(), solved, k, g);
-        if (cur_smoothing != 0.0f) launch_smooth(ctx, solved, cur.as<float>(), cur_smoothing, g);
+        // cdm_smooth with weight 0 leaves the running field as it was (the
+        // accumulated field is then unused), as in the reference
+        if (cur_smoothing != 0.0f) {
+            float* spare = solved == pa ? pb : pa;
+            launch_accumulate_smooth(ctx, cur, solved, k, cur_smoothing, g, spare);
+            pa = cur;
+            pb = solved;
+            cur = spare;
+        }
     }
     // timing mode reports the running field after the fixed iteration count
-    if (run.no_early_exit) copy_d2d(ctx, best_d, cur.p, fb);
+    if (run.no_early_exit) copy_d2d(ctx, best_d, cur, fb);
     TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
     return true;
 }

@@ -45,12 +45,36 @@ struct Vox {
     bool ok;
 };
 
+// The block's logical position in a 3-D grid and its x-fastest linear
+// index.  Hardware hands workgroup b (in x-fastest order) to XCD b % 8;
+// through xcd_remap each XCD works through a contiguous range of logical
+// blocks instead, so the halo and neighbourhood reads of adjacent tiles and
+// planes meet in one XCD's L2.  Per-block partial results are stored at the
+// logical index, so reductions over them keep their order.
+struct Block {
+    int x, y, z;
+    uint32_t lin;
+};
+__device__ inline Block logical_block()
+{
+    const uint32_t gx = gridDim.x, gy = gridDim.y;
+    const uint32_t hw = (blockIdx.z * gy + blockIdx.y) * gx + blockIdx.x;
+    const uint32_t L = xcd_remap(hw, gx * gy * gridDim.z);
+    Block b;
+    b.x = int(L % gx);
+    b.y = int((L / gx) % gy);
+    b.z = int(L / (gx * gy));
+    b.lin = L;
+    return b;
+}
+
 __device__ inline Vox this_voxel(const FieldGeo& g)
 {
+    const Block b = logical_block();
     Vox v;
-    v.x = int(blockIdx.x) * kTX + int(threadIdx.x);
-    v.y = int(blockIdx.y) * kTY + int(threadIdx.y);
-    v.z = int(blockIdx.z);
+    v.x = b.x * kTX + int(threadIdx.x);
+    v.y = b.y * kTY + int(threadIdx.y);
+    v.z = b.z;
     v.ok = v.x < g.w && v.y < g.h;
     v.p = uint32_t(v.z) * g.wh + uint32_t(v.y) * uint32_t(g.w) + uint32_t(v.x);
     return v;

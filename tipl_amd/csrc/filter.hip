@@ -48,7 +48,11 @@ __global__ __launch_bounds__(kGThreads) void gaussian_round_kernel(const float* 
     __shared__ float A[NA];
     __shared__ float X[AR][XC];
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * kGX + tx;
-    const int x0 = int(blockIdx.x) * kGX, y0 = int(blockIdx.y) * kGY, z0 = int(blockIdx.z) * zq;
+    // each XCD takes a contiguous range of blocks (x fastest), so the halo
+    // rows and planes adjacent blocks share are read through one L2
+    const uint32_t gx = gridDim.x, gy = gridDim.y;
+    const uint32_t L = xcd_remap((blockIdx.z * gy + blockIdx.y) * gx + blockIdx.x, gx * gy * gridDim.z);
+    const int x0 = int(L % gx) * kGX, y0 = int((L / gx) % gy) * kGY, z0 = int(L / (gx * gy)) * zq;
     const int zend = min(z0 + zq, g.d);
     const uint32_t plane = g.wh * uint32_t(CH);
 
