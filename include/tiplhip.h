@@ -184,6 +184,23 @@ int tiplhip_mi_hist(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_
 int tiplhip_cost_batch(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_volume* to,
                        const float* T, int K, int cost, int bins, double* costs);
 
+/* tiplhip_cost_batch that allocates nothing (no counterpart in the reference;
+ * for callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Every
+ * device buffer beyond the inputs and outputs -- staged transforms, segment
+ * table, joint histograms, correlation sums, costs -- lives in a 20 MiB
+ * static device array that each GPU the library runs on holds until the
+ * process ends.  Limits: K <= 256; with 256 bins, K <= 79 (the histograms
+ * must fit the array).  The sampled volume (`to`, or `from` when correlation
+ * swaps the roles of a larger `from`) must already hold its packed copy,
+ * which its first use in tiplhip_cost_batch builds.  Costs equal
+ * tiplhip_cost_batch's bit for bit.  Calls on one GPU share the array:
+ * calls that overlap on one GPU race (the context's own lock serialises
+ * calls on one context). */
+int tiplhip_cost_batch_noalloc(tiplhip_ctx* ctx, const tiplhip_volume* from,
+                               const tiplhip_volume* to, const float* T, int K, int cost, int bins,
+                               double* costs);
+
 /* Autotuner of the cost kernels' launch shape (no reference counterpart):
  * times the histogram (MI) or sums (correlation) kernel for these K
  * transforms over a grid of work-item depths (8, 16, 32 planes) and

@@ -29,7 +29,7 @@ EXPORTS = (
     "tiplhip_timer_start", "tiplhip_timer_stop", "tiplhip_stream_copy", "tiplhip_traffic_probe", "tiplhip_volume_create",
     "tiplhip_volume_create_device", "tiplhip_volume_destroy", "tiplhip_volume_dims",
     "tiplhip_volume_read", "tiplhip_volume_stats", "tiplhip_volume_downsample",
-    "tiplhip_resample", "tiplhip_corr_sums", "tiplhip_mi_hist", "tiplhip_cost_batch",
+    "tiplhip_resample", "tiplhip_corr_sums", "tiplhip_mi_hist", "tiplhip_cost_batch", "tiplhip_cost_batch_noalloc",
     "tiplhip_cost_batch_params", "tiplhip_affine_matrix", "tiplhip_matrix_inverse",
     "tiplhip_cdm_param_default", "tiplhip_cdm", "tiplhip_cdm_level_iterations",
     "tiplhip_compose_displacement", "tiplhip_cdm_gradient", "tiplhip_cdm_poisson",
@@ -138,6 +138,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "tiplhip_corr_sums": (ip, [vp, vp, vp, vp, ip, vp]),
         "tiplhip_mi_hist": (ip, [vp, vp, vp, vp, ip, ip, vp]),
         "tiplhip_cost_batch": (ip, [vp, vp, vp, vp, ip, ip, ip, vp]),
+        "tiplhip_cost_batch_noalloc": (ip, [vp, vp, vp, vp, ip, ip, ip, vp]),
         "tiplhip_cost_batch_params": (ip, [vp, vp, vp, vp, vp, vp, ip, ip, ip, vp]),
         "tiplhip_affine_matrix": (ip, [vp, P(ip), vp, P(ip), vp, vp]),
         "tiplhip_matrix_inverse": (ip, [vp]),
@@ -371,6 +372,18 @@ class Context:
         _check(load_library().tiplhip_cost_batch(self.handle, frm.handle, to.handle, _ptr(Ts),
                                                  Ts.shape[0], COST_MI if cost == "mi" else COST_CORR,
                                                  bins, _ptr(out)))
+        return out
+
+    def cost_batch_noalloc(self, frm: Volume, to: Volume, Ts, cost: str = "mi",
+                           bins: int = 256) -> np.ndarray:
+        """cost_batch through the entry point that allocates no device memory
+        (K <= 256; 256 bins: K <= 79; the sampled volume packed by an earlier
+        cost_batch)."""
+        Ts = np.ascontiguousarray(Ts, np.float32).reshape(-1, 12)
+        out = np.zeros(Ts.shape[0], np.float64)
+        _check(load_library().tiplhip_cost_batch_noalloc(
+            self.handle, frm.handle, to.handle, _ptr(Ts), Ts.shape[0],
+            COST_MI if cost == "mi" else COST_CORR, bins, _ptr(out)))
         return out
 
     def autotune(self, frm: Volume, to: Volume, Ts, cost: str = "mi", bins: int = 256, reps: int = 3):

@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 namespace tiplhip {
@@ -1151,6 +1152,158 @@ void cost_jobs(tiplhip_ctx* ctx, std::vector<PairJob> jobs, const float* T, int 
     std::memcpy(costs, ctx->h_costs.p, sizeof(double) * K);
 }
 
+// ---------------------------------------------------------------------------
+// Cost evaluation that allocates nothing (tiplhip_cost_batch_noalloc).
+//
+// Every device buffer the launch needs beyond its inputs and outputs lives
+// in `scratch`, a file-scope array each device holds for the life of the
+// process.  Its address on a device is looked up once, with that device
+// current, by hipGetSymbolAddress; host code uses only that address (the
+// array's name in host code is a host-side stand-in, not device memory).
+// Layout (byte offsets): transforms, crop flags, segment table, costs and
+// entropy partials, correlation sums, then the K joint histograms and
+// to_hist rows.  Calls on one device share the array: calls that overlap on
+// one GPU race.
+// ---------------------------------------------------------------------------
+constexpr int kNoAllocMaxK = 256;
+constexpr size_t kNoAllocScratchBytes = size_t(20) << 20;  // 20 MiB per device
+constexpr size_t kNaT = 0;                                  // K x 12 floats
+constexpr size_t kNaCrop = kNaT + sizeof(float) * 12 * kNoAllocMaxK;
+constexpr size_t kNaSeg = kNaCrop + sizeof(int4) * kNoAllocMaxK;
+constexpr size_t kNaCosts = kNaSeg + 1024;                  // K x (1 + 16) doubles
+constexpr size_t kNaSums = kNaCosts + sizeof(double) * 17 * kNoAllocMaxK;
+constexpr size_t kNaHist = (kNaSums + sizeof(unsigned long long) * 4 * kNoAllocMaxK + 255) & ~size_t(255);
+static_assert(sizeof(Segment) <= 1024, "segment slot");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};
+
+// The device address of `scratch` on ctx's device (current), looked up on
+// the first call for that device.
+unsigned char* scratch_of(tiplhip_ctx* ctx)
+{
+    TIPLHIP_REQUIRE(ctx->device >= 0 && ctx->device < kMaxDevices, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[ctx->device]) {
+        void* p = nullptr;
+        TIPLHIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        TIPLHIP_REQUIRE(p != nullptr, "no device address for the static scratch buffer");
+        g_scratch[ctx->device] = p;
+    }
+    return static_cast<unsigned char*>(g_scratch[ctx->device]);
+}
+
+void cost_noalloc(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_volume* to,
+                  const float* T, int K, int cost, int bins, double* costs)
+{
+    TIPLHIP_REQUIRE(T != nullptr && costs != nullptr, "null transform or cost buffer");
+    TIPLHIP_REQUIRE(cost == TIPLHIP_COST_CORR || cost == TIPLHIP_COST_MI, "unknown cost type");
+    const bool mi = cost == TIPLHIP_COST_MI;
+    if (mi) TIPLHIP_REQUIRE(bins == 256 || bins == 64, "bins must be 256 or 64");
+    TIPLHIP_REQUIRE(K >= 1 && K <= kNoAllocMaxK,
+                    "K must be in [1, 256] for the allocation-free cost evaluation");
+    check_volume(ctx, from, "from");
+    check_volume(ctx, to, "to");
+    const size_t hist_need = mi ? sizeof(int) * (size_t(bins) * bins + size_t(bins)) * size_t(K) : 0;
+    TIPLHIP_REQUIRE(kNaHist + hist_need <= kNoAllocScratchBytes,
+                    "the histograms of this many candidates exceed the 20 MiB static scratch "
+                    "buffer (256 bins: at most 79 candidates)");
+    DeviceGuard dg(ctx->device);
+    unsigned char* scr = scratch_of(ctx);
+    // host-side staging on the stack: transforms (with the correlation's
+    // role swap), crop flags, the segment
+    float Tk[12 * kNoAllocMaxK];
+    int4 crop[kNoAllocMaxK];
+    std::memcpy(Tk, T, sizeof(float) * 12 * size_t(K));
+    if (!mi && from->n > to->n) {
+        // reg::correlation swaps the roles when `from` is larger (reg/linear.hpp:28-33)
+        std::swap(from, to);
+        for (int k = 0; k < K; ++k) (void)matrix_inverse(&Tk[12 * k]);
+    }
+    for (int k = 0; k < K; ++k) {
+        int pos[3] = {0, 0, 0};
+        const bool c = !mi && is_integer_shift(&Tk[12 * k], pos);
+        crop[k] = make_int4(c ? 1 : 0, pos[0], pos[1], pos[2]);
+        for (int i = 9; i < 12; ++i)
+            if (Tk[12 * k + i] == 0.0f) Tk[12 * k + i] = 0.0f;  // +0.0 (packed sampler's bounds test)
+    }
+    TIPLHIP_REQUIRE(to->d_pack != nullptr,
+                    "the sampled volume has no packed copy yet: evaluate it once with "
+                    "tiplhip_cost_batch (which builds it) before allocation-free calls");
+    LaunchShape shape = default_shape(cost, bins);
+    auto it = ctx->tuned.find(tune_key(from, to, K, cost, bins));
+    if (it != ctx->tuned.end()) shape = it->second;
+    Segment sg;
+    sg.g = batch_geometry(from, to, K, shape.zc);
+    const double work = double(sg.g.items) * double(K);
+    const double target = shape.per_cu == 0
+                              ? 1.0
+                              : std::max(double(ctx->cu_count) * double(shape.per_cu),
+                                         std::ceil(work / double(kMaxItemsPerBlock)));
+    batch_blocks(sg.g, hist256(cost, bins) ? 65535u : (1u << 23),
+                 uint32_t(std::min(std::ceil(target), 2147483647.0)));
+    sg.from = from->d;
+    sg.to = to->d;
+    sg.to8 = to->d_pack;
+    sg.from_hist = from->d_hist + (bins == 64 ? 256 : 0);
+    sg.sx = (long long)from->sum;
+    sg.sxx = (long long)from->sum_sq;
+    sg.n = double(from->n);
+    sg.block_begin = 0;
+    sg.cand_begin = 0;
+    const uint32_t blocks = sg.g.total_blocks;
+    const Segment* d_seg = reinterpret_cast<const Segment*>(scr + kNaSeg);
+    const float* d_T = reinterpret_cast<const float*>(scr + kNaT);
+    const int4* d_crop = reinterpret_cast<const int4*>(scr + kNaCrop);
+    double* d_costs = reinterpret_cast<double*>(scr + kNaCosts);
+    hipStream_t st = ctx->stream;
+    // the host arrays above stay alive until the stream is synchronised below
+    auto drain = [st] { (void)hipStreamSynchronize(st); };
+    DrainOnUnwind<decltype(drain)> unwind(drain);
+    TIPLHIP_CHECK(hipMemcpyAsync(scr + kNaT, Tk, sizeof(float) * 12 * size_t(K), hipMemcpyHostToDevice, st));
+    TIPLHIP_CHECK(hipMemcpyAsync(scr + kNaSeg, &sg, sizeof(Segment), hipMemcpyHostToDevice, st));
+    if (mi) {
+        // the array keeps the previous call's contents: clear the histograms
+        // (a fill kernel, which a graph capture may also record)
+        zero_bytes16(ctx, scr + kNaHist, hist_need);
+        int* h = reinterpret_cast<int*>(scr + kNaHist);
+        uint32_t* th = reinterpret_cast<uint32_t*>(h + size_t(bins) * bins * size_t(K));
+        if (bins == 256) {
+            hipLaunchKernelGGL((mi_hist_kernel<256, false>), dim3(blocks), dim3(1024), 0, st, d_seg, 1u,
+                               blocks, d_T, nullptr, 0u, h, th);
+            TIPLHIP_CHECK(hipGetLastError());
+            constexpr int G = entropy_slices<256>();
+            hipLaunchKernelGGL(mi_entropy_kernel<256>, dim3(uint32_t(K) * uint32_t(G)), dim3(kEntropyThreads),
+                               0, st, d_seg, 1u, h, th, d_costs + K, d_costs, nullptr);
+            TIPLHIP_CHECK(hipGetLastError());
+            hipLaunchKernelGGL(mi_entropy_final_kernel<256>, dim3((uint32_t(K) + 255) / 256), dim3(256), 0,
+                               st, d_costs + K, uint32_t(K), d_costs, nullptr, th);
+        } else {
+            hipLaunchKernelGGL((mi_hist_kernel<64, false>), dim3(blocks), dim3(256), 0, st, d_seg, 1u,
+                               blocks, d_T, nullptr, 0u, h, th);
+            TIPLHIP_CHECK(hipGetLastError());
+            hipLaunchKernelGGL(mi_entropy_kernel<64>, dim3(uint32_t(K)), dim3(kEntropyThreads), 0, st, d_seg,
+                               1u, h, th, d_costs + K, d_costs, nullptr);
+        }
+        TIPLHIP_CHECK(hipGetLastError());
+    } else {
+        TIPLHIP_CHECK(hipMemcpyAsync(scr + kNaCrop, crop, sizeof(int4) * size_t(K), hipMemcpyHostToDevice, st));
+        unsigned long long* sums = reinterpret_cast<unsigned long long*>(scr + kNaSums);
+        zero_u64(ctx, sums, 4 * size_t(K));
+        hipLaunchKernelGGL(corr_sums_kernel<false>, dim3(blocks), dim3(256), 0, st, d_seg, 1u, blocks, d_T,
+                           d_crop, nullptr, 0u, sums);
+        TIPLHIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(corr_finalize_kernel, dim3((uint32_t(K) + 63) / 64), dim3(64), 0, st, d_seg, 1u,
+                           sums, K, d_costs, nullptr, nullptr);
+        TIPLHIP_CHECK(hipGetLastError());
+    }
+    TIPLHIP_CHECK(hipMemcpyAsync(costs, d_costs, sizeof(double) * size_t(K), hipMemcpyDeviceToHost, st));
+    TIPLHIP_CHECK(hipStreamSynchronize(st));
+}
+
 }  // namespace
 }  // namespace tiplhip
 
@@ -1322,6 +1475,17 @@ int tiplhip_cost_batch(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplh
         TIPLHIP_REQUIRE(ctx != nullptr, "null context");
         std::lock_guard<std::recursive_mutex> lk(ctx->mu);
         cost_jobs(ctx, {PairJob{from, to, K}}, T, cost, bins, costs);
+    });
+}
+
+int tiplhip_cost_batch_noalloc(tiplhip_ctx* ctx, const tiplhip_volume* from,
+                               const tiplhip_volume* to, const float* T, int K, int cost, int bins,
+                               double* costs)
+{
+    return guarded([&] {
+        TIPLHIP_REQUIRE(ctx != nullptr, "null context");
+        std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        cost_noalloc(ctx, from, to, T, K, cost, bins, costs);
     });
 }
 
