@@ -427,6 +427,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cdm-iters", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-config2", action="store_true",
+                    help="skip the 160x192x160 CDM and mm_reg legs (for short profiling runs)")
     ap.add_argument("--serve-requests", type=int, default=96,
                     help="requests in the serving leg (0 skips it)")
     ap.add_argument("--serve-slots", type=int, default=16)
@@ -532,12 +534,14 @@ def main():
         mp_res = multi_pair_bench(ctx, bpairs)
         log(f"multi-pair {mp_res}")
 
-    log("CDM at 160x192x160: the reference cdm() and the fixed schedule")
-    cdm2 = cdm_config2(ctx)
-    log(f"cdm config 2 {cdm2}")
-    log("mm_reg nonlinear half at 160x192x160")
-    mmreg = mm_reg_pipeline(ctx)
-    log(f"mm_reg pipeline {mmreg}")
+    cdm2 = mmreg = None
+    if not args.no_config2:
+        log("CDM at 160x192x160: the reference cdm() and the fixed schedule")
+        cdm2 = cdm_config2(ctx)
+        log(f"cdm config 2 {cdm2}")
+        log("mm_reg nonlinear half at 160x192x160")
+        mmreg = mm_reg_pipeline(ctx)
+        log(f"mm_reg pipeline {mmreg}")
 
     serve = None
     if reqs:

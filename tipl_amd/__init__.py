@@ -29,7 +29,7 @@ EXPORTS = (
     "tiplhip_timer_start", "tiplhip_timer_stop", "tiplhip_stream_copy", "tiplhip_traffic_probe", "tiplhip_volume_create",
     "tiplhip_volume_create_device", "tiplhip_volume_destroy", "tiplhip_volume_dims",
     "tiplhip_volume_read", "tiplhip_volume_stats", "tiplhip_volume_downsample",
-    "tiplhip_resample", "tiplhip_corr_sums", "tiplhip_mi_hist", "tiplhip_cost_batch", "tiplhip_cost_batch_noalloc",
+    "tiplhip_resample", "tiplhip_corr_sums", "tiplhip_mi_hist", "tiplhip_cost_batch", "tiplhip_cost_batch_noalloc", "tiplhip_set_launch_shape",
     "tiplhip_cost_batch_params", "tiplhip_affine_matrix", "tiplhip_matrix_inverse",
     "tiplhip_cdm_param_default", "tiplhip_cdm", "tiplhip_cdm_level_iterations",
     "tiplhip_compose_displacement", "tiplhip_cdm_gradient", "tiplhip_cdm_poisson",
@@ -139,6 +139,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "tiplhip_mi_hist": (ip, [vp, vp, vp, vp, ip, ip, vp]),
         "tiplhip_cost_batch": (ip, [vp, vp, vp, vp, ip, ip, ip, vp]),
         "tiplhip_cost_batch_noalloc": (ip, [vp, vp, vp, vp, ip, ip, ip, vp]),
+        "tiplhip_set_launch_shape": (ip, [vp, vp, vp, ip, ip, ip, ip, ip, ip]),
         "tiplhip_cost_batch_params": (ip, [vp, vp, vp, vp, vp, vp, ip, ip, ip, vp]),
         "tiplhip_affine_matrix": (ip, [vp, P(ip), vp, P(ip), vp, vp]),
         "tiplhip_matrix_inverse": (ip, [vp]),
@@ -385,6 +386,13 @@ class Context:
             self.handle, frm.handle, to.handle, _ptr(Ts), Ts.shape[0],
             COST_MI if cost == "mi" else COST_CORR, bins, _ptr(out)))
         return out
+
+    def set_launch_shape(self, frm: Volume, to: Volume, K: int, cost: str = "mi", bins: int = 256,
+                         zc: int = 16, per_cu: int = 2, parts: int = 1):
+        """Pins the launch shape of later single-pair launches with this key."""
+        _check(load_library().tiplhip_set_launch_shape(
+            self.handle, frm.handle, to.handle, K, COST_MI if cost == "mi" else COST_CORR, bins,
+            zc, per_cu, parts))
 
     def autotune(self, frm: Volume, to: Volume, Ts, cost: str = "mi", bins: int = 256, reps: int = 3):
         """Picks the cost kernel's launch shape for this pair, K and cost (kept on
