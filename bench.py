@@ -123,6 +123,32 @@ def cpu_baseline(F, M, Ts, threads, seconds_target=15.0):
             "sample": f"{K} MI (256-bin) oracle evaluations, 1 thread, {dt:.1f} s"}
 
 
+# Issue cost of one wave64 VALU instruction on this GPU, measured with a
+# timing probe of single instruction kinds (8 waves per SIMD, 8 independent
+# chains each): FP32 add/mul/sub and bitwise ops, and conversions, v_fract,
+# 24-bit multiplies, v_perm, v_cndmask.
+VALU_CYCLES_FAST, VALU_CYCLES_SLOW = 2.5, 4.3
+SIMDS, CLOCK_GHZ = 1024, 2.4
+
+
+def valu_issue(prof, kernel_ms):
+    """The binding resource of the affine kernels: VALU issue.  From the
+    committed counter pass (SQ_INSTS_VALU per launch and the kernel's mix of
+    fast and slow instructions): the time the SIMDs need just to issue the
+    launch's VALU instructions, against the measured kernel time."""
+    if not prof or "valu_insts" not in prof:
+        return None
+    per_simd = prof["valu_insts"] / SIMDS
+    cyc = prof["slow_frac"] * VALU_CYCLES_SLOW + (1.0 - prof["slow_frac"]) * VALU_CYCLES_FAST
+    issue_ms = per_simd * cyc / (CLOCK_GHZ * 1e9) * 1e3
+    return {"resource": "VALU issue", "valu_instructions_per_launch": prof["valu_insts"],
+            "simds": SIMDS, "clock_ghz": CLOCK_GHZ,
+            "cycles_per_instruction": round(cyc, 2),
+            "slow_instruction_fraction": prof["slow_frac"],
+            "issue_ms": round(issue_ms, 4), "frac": round(issue_ms / kernel_ms, 3),
+            "source": prof["source"]}
+
+
 def profiled_traffic(kernel):
     """HBM bytes per launch of `kernel` as the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE passes measured them (calibrated against the copy kernel), or
@@ -515,6 +541,19 @@ def main():
     cdm_dt = time.perf_counter() - t0
     cdm_ips = args.cdm_iters / cdm_dt
     log(f"CDM {cdm_ips:.1f} iter/s")
+    # the CDM iteration against HBM: 520 B of algorithmic traffic per voxel,
+    # the reference's step sequence at its fused minimum (compose 14, gradient
+    # 14, twelve Jacobi sweeps 12 x 36, accumulate 36, smooth 24); the
+    # two-sweep Jacobi kernel and the fused accumulate+smooth move less than
+    # that; whole iterations timed, host control included
+    cdm_alg = 520.0 * N
+    cdm_roofline = {"bound": "hbm", "achieved": round(cdm_alg * cdm_ips / 1e9, 1),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(cdm_alg * cdm_ips / 1e9 / HBM_PEAK_GBS, 4),
+                    "algorithmic_bytes": cdm_alg,
+                    "traffic": (profiled_traffic("cdm_iteration") or {}).get("bytes"),
+                    "what": "one cdm() loop iteration at 182x218x182 (compose, gradient, 12 "
+                            "Jacobi sweeps, accumulate + smooth), host control included"}
 
     # --- roofline of the dominant kernel (MI 64-bin joint histogram; a launch
     # this large takes the z-product-table instantiation) --------------------
@@ -526,7 +565,8 @@ def main():
                 "traffic": prof["bytes"] if prof else None,
                 "kernel": "mi_hist_kernel<64, true>", "kernel_ms": round(k_avg_ms, 4),
                 "algorithmic_bytes": alg_bytes,
-                "traffic_source": prof["source"] if prof else None}
+                "traffic_source": prof["source"] if prof else None,
+                "limiter": valu_issue(prof, k_avg_ms)}
 
     mp_res = None
     if bpairs:
@@ -606,6 +646,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "cdm_iter_per_s": round(cdm_ips, 3),
+        "cdm_roofline": cdm_roofline,
         "cdm_shape": list(shape),
         "cdm_cpu_baseline": cpu_cdm,
         "corr_cpu_baseline": cpu_corr,

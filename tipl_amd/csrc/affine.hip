@@ -135,6 +135,15 @@ struct ByCandidate {
     __device__ uint32_t operator()(const Segment& s) const { return s.cand_begin; }
 };
 
+// (mask & a) | (~mask & b) in one v_bfi_b32 (mask from a scalar register:
+// gfx9 VOP3 instructions take no literal operand).
+__device__ inline uint32_t bitfield_insert(uint32_t mask, uint32_t a, uint32_t b)
+{
+    uint32_t r;
+    asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "s"(mask), "v"(a), "v"(b));
+    return r;
+}
+
 // Wave-uniform decode of an item into this lane's (x, y) and the z run.
 struct Item {
     uint32_t x, y;      // this lane's voxel column
@@ -427,21 +436,25 @@ __global__ __launch_bounds__((MiTraits<BINS, PARTS>::kThreads)) void mi_hist_ker
         const ColumnSums cs = column_sums(T, float(m.x), float(m.y));
         uint32_t fidx = m.z0 * fwh + m.y * uint32_t(g.fw) + m.x;
         auto count = [&](uint32_t fv, uint32_t tv) {
-            // the row within this part's range
-            const uint32_t bin = (((fv >> Tr::kShift) & uint32_t(Tr::kRows - 1)) << Tr::kBand) |
-                                 (tv >> Tr::kShift);
             if constexpr (Tr::kAggregate) {
-                const uint32_t b0 = __builtin_amdgcn_readfirstlane(bin);
-                const bool same = bin == b0;
+                // 64 bins: the byte address of word (fv >> 2) * 64 + (tv >> 2)
+                // in one bitfield insert -- bits 8..13 from fv << 6, bits
+                // 2..7 from tv (both values are below 256)
+                const uint32_t a = bitfield_insert(0x3F03u, fv << 6, tv);
+                const uint32_t a0 = __builtin_amdgcn_readfirstlane(a);
+                const bool same = a == a0;
                 const unsigned long long mk = __ballot(same);
+                uint32_t* const w = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(lds) + a);
                 if (same) {
                     const uint32_t leader = uint32_t(__ffsll((long long)mk) - 1);
-                    if (lane == leader)
-                        atomicAdd(&lds[Tr::word(b0)], Tr::inc(b0) * uint32_t(__popcll(mk)));
+                    if (lane == leader) atomicAdd(w, uint32_t(__popcll(mk)));
                 } else {
-                    atomicAdd(&lds[Tr::word(bin)], Tr::inc(bin));
+                    atomicAdd(w, 1u);
                 }
             } else {
+                // the row within this part's range
+                const uint32_t bin = (((fv >> Tr::kShift) & uint32_t(Tr::kRows - 1)) << Tr::kBand) |
+                                     (tv >> Tr::kShift);
                 atomicAdd(&lds[Tr::word(bin)], Tr::inc(bin));
             }
         };

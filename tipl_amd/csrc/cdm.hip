@@ -796,6 +796,61 @@ __global__ __launch_bounds__(256) void accumulate_kernel(const float* __restrict
     st3(nd, v.p, accumulated(cur, nd, k, g, v.p, v.x, v.y, v.z));
 }
 
+// One packed float3 voxel through a raw buffer resource: a load past the
+// end returns 0 and never faults.
+__device__ inline float3 ld3_rsrc(__amdgpu_buffer_rsrc_t r, uint32_t p)
+{
+    const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, int(p * 12u), 0, 0);
+    return make_float3(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]));
+}
+
+// accumulated() without branches, so that several evaluations per thread
+// keep their loads in flight together: the eight corners are always loaded
+// (at the voxel itself when the sample is not taken, so corner 0 is then
+// cur(p); corners past the field's end read 0 and are unused) and the
+// result is selected.  Where the sample is taken every operation is
+// accumulated()'s, in its order.  Needs 12 n < 2^32.
+__device__ inline float3 accumulated_bf(__amdgpu_buffer_rsrc_t rcur, __amdgpu_buffer_rsrc_t rnd,
+                                        float k, const FieldGeo& g, uint32_t p, int x, int y, int z)
+{
+    float3 d = ld3_rsrc(rnd, p);
+    d.x = d.x * k;
+    d.y = d.y * k;
+    d.z = d.z * k;
+    const bool moved = d.x != 0.0f || d.y != 0.0f || d.z != 0.0f;
+    const float sx = d.x + float(x), sy = d.y + float(y), sz = d.z + float(z);
+    const bool in = (fminf(fminf(sx, sy), sz) >= 0.0f) & (sx < g.wm1) & (sy < g.hm1) & (sz < g.dm1);
+    const bool use = moved && in;
+    const uint32_t ix = use ? cvt_u32_sat(sx) : 0u, iy = use ? cvt_u32_sat(sy) : 0u,
+                   iz = use ? cvt_u32_sat(sz) : 0u;
+    const uint32_t i0 = use ? iz * g.wh + iy * uint32_t(g.w) + ix : p;
+    float3 m[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) m[c] = ld3_rsrc(rcur, corner(g, i0, c));
+    const float p0 = sx - float(ix), p1 = sy - float(iy), p2 = sz - float(iz);
+    const float n0 = 1.0f - p0, n1 = 1.0f - p1, n2 = 1.0f - p2;
+    const float n0n1 = n0 * n1, p0n1 = p0 * n1, n0p1 = n0 * p1, p0p1 = p0 * p1;
+    const float w[8] = {n0n1 * n2, p0n1 * n2, n0p1 * n2, p0p1 * n2,
+                        n0n1 * p2, p0n1 * p2, n0p1 * p2, p0p1 * p2};
+    float3 acc;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const float mx = m[c].x + float(int(ix) + (c & 1));
+        const float my = m[c].y + float(int(iy) + ((c >> 1) & 1));
+        const float mz = m[c].z + float(int(iz) + ((c >> 2) & 1));
+        if (c == 0) {
+            acc.x = mx * w[0];
+            acc.y = my * w[0];
+            acc.z = mz * w[0];
+        } else {
+            acc.x = acc.x + mx * w[c];
+            acc.y = acc.y + my * w[c];
+            acc.z = acc.z + mz * w[c];
+        }
+    }
+    return use ? make_float3(acc.x - float(x), acc.y - float(y), acc.z - float(z)) : m[0];
+}
+
 // ---------------------------------------------------------------------------
 // accumulate_displacement then cdm_smooth in one pass, marching z:
 // out = smooth(acc), acc = the accumulated field of accumulate_kernel.
@@ -805,62 +860,74 @@ __global__ __launch_bounds__(256) void accumulate_kernel(const float* __restrict
 // m-+1.  A position outside [0, n) holds +0.0, which adds nothing to the
 // neighbour sum (that sum starts at +0.0 and is never -0.0), as the
 // reference drops it.  Per plane the block computes acc once on the tile and
-// its 148-position ring and keeps three planes in LDS.  HBM traffic per
+// its 148-position ring (two branch-free evaluations per thread, their loads
+// in flight together) and keeps the planes in an LDS ring.  HBM traffic per
 // voxel: new_d read, cur gathered, out written (36 B, against 36 + 24 B for
-// the two passes).  out must not alias cur or nd.
+// the two passes).  Measured at 182x218x182, whole iterations: 1558
+// iterations/s against 1546 with the two passes, 1490 with branching
+// evaluations and 1538 with two planes a step (four evaluations, 125
+// VGPRs, half the occupancy).  out must not alias cur or nd.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kQX * kQY) void accumulate_smooth_kernel(
     const float* __restrict__ cur, const float* __restrict__ nd, float k, float w_6, float w_1,
     FieldGeo g, int zq, float* __restrict__ out)
 {
-    __shared__ float3 A[3][kS1Y][kS1X];
+    __shared__ float3 A[4][kS1Y][kS1X];
     const int tx = int(threadIdx.x), ty = int(threadIdx.y), tid = ty * kQX + tx;
     const Block blk = logical_block();
     const int x0 = blk.x * kQX, y0 = blk.y * kQY;
     const int z0 = blk.z * zq, z1 = min(z0 + zq, g.d);
     const long long n = g.n, w = g.w, wh = g.wh;
     const long long base = (long long)y0 * w + x0;
+    const __amdgpu_buffer_rsrc_t rcur = make_rsrc(cur, g.n * 12u);
+    const __amdgpu_buffer_rsrc_t rnd = make_rsrc(nd, g.n * 12u);
     int ri = 0, rj = 0;
     const bool has_ring = tid < kRing;
     if (has_ring) ring_offset(tid, ri, rj);
     const long long ro = (long long)rj * w + ri;
     const long long io = (long long)ty * w + tx;
+    const float3 zero = make_float3(0.0f, 0.0f, 0.0f);
     auto acc_at = [&](long long q) -> float3 {
-        if (q < 0 || q >= n) return make_float3(0.0f, 0.0f, 0.0f);
-        const uint32_t p = uint32_t(q);
+        const bool ok = q >= 0 && q < n;
+        const uint32_t p = ok ? uint32_t(q) : 0u;
         const uint32_t z = p / g.wh, r = p - z * g.wh;
         const uint32_t y = r / uint32_t(g.w), x = r - y * uint32_t(g.w);
-        return accumulated(cur, nd, k, g, p, int(x), int(y), int(z));
+        const float3 v = accumulated_bf(rcur, rnd, k, g, p, int(x), int(y), int(z));
+        return ok ? v : zero;
     };
     auto fill = [&](int m) {
-        float3(*dst)[kS1X] = A[(m + 3) % 3];
         const long long pm = (long long)m * wh + base;
-        dst[ty + 1][tx + 1] = acc_at(pm + io);
-        if (has_ring) dst[rj + 1][ri + 1] = acc_at(pm + ro);
+        const float3 a_in = acc_at(pm + io);
+        const float3 a_r = has_ring ? acc_at(pm + ro) : zero;
+        float3(*da)[kS1X] = A[(m + 4) & 3];
+        da[ty + 1][tx + 1] = a_in;
+        if (has_ring) da[rj + 1][ri + 1] = a_r;
+    };
+    auto smooth = [&](int m) {
+        if (m >= z1 || x0 + tx >= g.w || y0 + ty >= g.h) return;
+        float3(*am)[kS1X] = A[(m + 3) & 3];
+        float3(*ac)[kS1X] = A[(m + 4) & 3];
+        float3(*ap)[kS1X] = A[(m + 5) & 3];
+        const int ci = tx + 1, cj = ty + 1;
+        const float3 t[6] = {ac[cj][ci + 1], ac[cj][ci - 1], ac[cj + 1][ci],
+                             ac[cj - 1][ci], ap[cj][ci],     am[cj][ci]};
+        float3 v = zero;
+#pragma unroll
+        for (int e = 0; e < 6; ++e) {
+            v.x = v.x + t[e].x;
+            v.y = v.y + t[e].y;
+            v.z = v.z + t[e].z;
+        }
+        const float3 c = ac[cj][ci];
+        st3(out, uint32_t((long long)m * wh + base + io),
+            make_float3(c.x * w_1 + v.x * w_6, c.y * w_1 + v.y * w_6, c.z * w_1 + v.z * w_6));
     };
     fill(z0 - 1);
     fill(z0);
     for (int m = z0; m < z1; ++m) {
         fill(m + 1);
         __syncthreads();
-        if (x0 + tx < g.w && y0 + ty < g.h) {
-            float3(*am)[kS1X] = A[(m + 2) % 3];
-            float3(*ac)[kS1X] = A[(m + 3) % 3];
-            float3(*ap)[kS1X] = A[(m + 4) % 3];
-            const int ci = tx + 1, cj = ty + 1;
-            const float3 t[6] = {ac[cj][ci + 1], ac[cj][ci - 1], ac[cj + 1][ci],
-                                 ac[cj - 1][ci], ap[cj][ci],     am[cj][ci]};
-            float3 v = make_float3(0.0f, 0.0f, 0.0f);
-#pragma unroll
-            for (int e = 0; e < 6; ++e) {
-                v.x = v.x + t[e].x;
-                v.y = v.y + t[e].y;
-                v.z = v.z + t[e].z;
-            }
-            const float3 c = ac[cj][ci];
-            st3(out, uint32_t((long long)m * wh + base + io),
-                make_float3(c.x * w_1 + v.x * w_6, c.y * w_1 + v.y * w_6, c.z * w_1 + v.z * w_6));
-        }
+        smooth(m);
         __syncthreads();
     }
 }
