@@ -88,8 +88,9 @@ def ref_cost_rate(kind, F, M, Ts, threads, seconds_target):
     ref.set_threads(threads)
     t0 = time.perf_counter()
     ref.cost_batch(kind, F, M, Ts[:threads])
-    one = (time.perf_counter() - t0) / max(1, min(threads, len(Ts)))  # one evaluation per thread
-    n = max(threads, int(round(seconds_target * threads / max(one, 1e-4))))
+    # wall time per evaluation with every thread busy (one evaluation each)
+    one = (time.perf_counter() - t0) / max(1, min(threads, len(Ts)))
+    n = max(threads, int(round(seconds_target / max(one, 1e-4))))
     Ta = np.asarray(Ts, np.float32).reshape(-1, 12)
     T = np.tile(Ta, (-(-n // len(Ta)), 1))[:n]
     t0 = time.perf_counter()
@@ -609,10 +610,12 @@ def main():
         cpu_cdm = cpu_cdm_baseline(vt.read(), vs.read(), nt)
         log(f"cpu cdm {cpu_cdm}")
         # the same legs on one core (shorter samples)
-        cpu_1core = {"mi256": ref_cost_rate("mi", F8, M8, Ts, 1, 5.0),
-                     "corr": ref_cost_rate("corr", F8, M8, Ts, 1, 5.0),
-                     "cdm": cpu_cdm_baseline(vt.read(), vs.read(), 1, 6.0)}
-        log(f"cpu one core {cpu_1core}")
+        cpu_1core = {}
+        for k, leg in (("mi256", lambda: ref_cost_rate("mi", F8, M8, Ts, 1, 5.0)),
+                       ("corr", lambda: ref_cost_rate("corr", F8, M8, Ts, 1, 5.0)),
+                       ("cdm", lambda: cpu_cdm_baseline(vt.read(), vs.read(), 1, 6.0))):
+            cpu_1core[k] = leg()
+            log(f"cpu one core {k} {cpu_1core[k]}")
     # GPU over CPU on the same computation (the CPU legs are the reference's
     # 256-bin MI, correlation and cdm() loop body)
     speedup = {}

@@ -203,11 +203,9 @@ int tiplhip_cost_batch_noalloc(tiplhip_ctx* ctx, const tiplhip_volume* from,
 
 /* Autotuner of the cost kernels' launch shape (no reference counterpart):
  * times the histogram (MI) or sums (correlation) kernel for these K
- * transforms over a grid of work-item depths (8, 16, 32 planes),
+ * transforms over a grid of work-item depths (8, 16, 32 planes) and
  * workgroups per CU (per_cu 0: the fewest workgroups, each as full as the
- * histogram counters allow) and, for 256-bin MI, the number of `from`-row
- * ranges the histogram is split into (1, 2, 4: one workgroup each, 128 KB /
- * ranges of LDS), `reps` launches each, and keeps the fastest for
+ * histogram counters allow), `reps` launches each, and keeps the fastest for
  * every later single-pair launch on ctx with the same shapes, K, cost and
  * bins.  Results do not depend on the shape (integer accumulation).  Outputs
  * may be NULL. */
@@ -227,11 +225,10 @@ typedef struct tiplhip_pair_costs {
 /* Pins the launch shape of later single-pair cost launches on ctx with these
  * volumes' shapes, K, cost and bins, as the autotuner would (for example a
  * shape found offline): zc planes per work item, per_cu workgroups per CU
- * (0: the fewest), parts `from`-row ranges of a 256-bin MI histogram (1, 2
- * or 4; 1 otherwise).  Results do not depend on the shape.  No reference
+ * (0: the fewest).  Results do not depend on the shape.  No reference
  * counterpart. */
 int tiplhip_set_launch_shape(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_volume* to,
-                             int K, int cost, int bins, int zc, int per_cu, int parts);
+                             int K, int cost, int bins, int zc, int per_cu);
 
 /* tiplhip_cost_batch for n image pairs of any sizes in one launch (all
  * candidates of all pairs, at most 4096, share one histogram or sum kernel

@@ -169,30 +169,30 @@ def test_large_launch_z_table_is_bit_identical(ctx, oracle):
 
 
 @pytest.mark.parametrize("shape,K", [((40, 44, 36), 7), ((64, 64, 64), 130), ((23, 17, 31), 3)])
-def test_split_range_histograms_are_identical(ctx, oracle, shape, K):
-    """The 256-bin histogram split into 1, 2 or 4 ranges of `from` rows (one
-    workgroup per range) at several launch shapes: histograms, to_hist-based
-    costs and the oracle agree bit for bit; 130 candidates of 64^3 take the
+@pytest.mark.parametrize("bins", [256, 64])
+def test_launch_shapes_are_identical(ctx, oracle, shape, K, bins):
+    """Histograms and costs do not depend on the launch shape (planes per
+    work item, workgroups per CU, pinned with set_launch_shape): bit for bit
+    across shapes and against the oracle; 130 candidates of 64^3 take the
     z-table kernels."""
     rng = np.random.default_rng(41)
     A, B = smooth_volume(shape, 42), smooth_volume(shape[::-1], 43)
     va, vb = ctx.volume(A), ctx.volume(B)
     Ts = np.stack([_T(oracle, p, A.shape, B.shape) for p in random_params(rng, K)])
     ref_h = ref_c = None
-    for parts in (1, 2, 4):
-        for zc, per_cu in ((16, 2), (8, 0), (32, 4)):
-            ctx.set_launch_shape(va, vb, K, "mi", 256, zc=zc, per_cu=per_cu, parts=parts)
-            h = ctx.mi_hist(va, vb, Ts, bins=256)
-            c = ctx.cost_batch(va, vb, Ts, "mi", 256)
-            if ref_h is None:
-                ref_h, ref_c = h, c
-                for k in (0, K - 1):
-                    assert np.array_equal(h[k], oracle.mi_hist(A, B, Ts[k]))
-            assert np.array_equal(h, ref_h), (parts, zc, per_cu)
-            assert np.array_equal(c.view(np.uint64), ref_c.view(np.uint64)), (parts, zc, per_cu)
+    for zc, per_cu in ((16, 2), (8, 0), (32, 4), (4, 16)):
+        ctx.set_launch_shape(va, vb, K, "mi", bins, zc=zc, per_cu=per_cu)
+        h = ctx.mi_hist(va, vb, Ts, bins=bins)
+        c = ctx.cost_batch(va, vb, Ts, "mi", bins)
+        if ref_h is None:
+            ref_h, ref_c = h, c
+            for k in (0, K - 1):
+                assert np.array_equal(h[k], oracle.mi_hist(A, B, Ts[k], bins=bins))
+        assert np.array_equal(h, ref_h), (zc, per_cu)
+        assert np.array_equal(c.view(np.uint64), ref_c.view(np.uint64)), (zc, per_cu)
     import tipl_amd
     with pytest.raises(tipl_amd.TiplHipError):
-        ctx.set_launch_shape(va, vb, K, "mi", 64, parts=2)
+        ctx.set_launch_shape(va, vb, K, "mi", 100)
 
 
 def test_batch_equals_single(ctx, oracle):

@@ -139,7 +139,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "tiplhip_mi_hist": (ip, [vp, vp, vp, vp, ip, ip, vp]),
         "tiplhip_cost_batch": (ip, [vp, vp, vp, vp, ip, ip, ip, vp]),
         "tiplhip_cost_batch_noalloc": (ip, [vp, vp, vp, vp, ip, ip, ip, vp]),
-        "tiplhip_set_launch_shape": (ip, [vp, vp, vp, ip, ip, ip, ip, ip, ip]),
+        "tiplhip_set_launch_shape": (ip, [vp, vp, vp, ip, ip, ip, ip, ip]),
         "tiplhip_cost_batch_params": (ip, [vp, vp, vp, vp, vp, vp, ip, ip, ip, vp]),
         "tiplhip_affine_matrix": (ip, [vp, P(ip), vp, P(ip), vp, vp]),
         "tiplhip_matrix_inverse": (ip, [vp]),
@@ -388,11 +388,11 @@ class Context:
         return out
 
     def set_launch_shape(self, frm: Volume, to: Volume, K: int, cost: str = "mi", bins: int = 256,
-                         zc: int = 16, per_cu: int = 2, parts: int = 1):
+                         zc: int = 16, per_cu: int = 2):
         """Pins the launch shape of later single-pair launches with this key."""
         _check(load_library().tiplhip_set_launch_shape(
             self.handle, frm.handle, to.handle, K, COST_MI if cost == "mi" else COST_CORR, bins,
-            zc, per_cu, parts))
+            zc, per_cu))
 
     def autotune(self, frm: Volume, to: Volume, Ts, cost: str = "mi", bins: int = 256, reps: int = 3):
         """Picks the cost kernel's launch shape for this pair, K and cost (kept on

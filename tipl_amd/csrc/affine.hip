@@ -39,9 +39,7 @@ struct BatchGeo {
     uint32_t items;             // ncols * nzc
     uint32_t items_per_block;
     uint32_t K;
-    uint32_t parts;             // 256-bin MI: workgroups per chunk and candidate, one per
-                                // range of `from` rows (1 for other costs)
-    uint32_t total_blocks;      // K * parts * ceil(items / items_per_block)
+    uint32_t total_blocks;      // K * ceil(items / items_per_block)
     uint32_t from_bytes, to_bytes8;
     VolGeo to;
 };
@@ -75,7 +73,6 @@ BatchGeo batch_geometry(const tiplhip_volume* from, const tiplhip_volume* to, in
     b.nzc = (uint32_t(b.fd) + b.zc - 1) / b.zc;
     b.items = b.ncols * b.nzc;
     b.K = uint32_t(K);
-    b.parts = 1;
     b.items_per_block = 1;
     b.total_blocks = 0;
     b.from_bytes = uint32_t(from->n);
@@ -84,19 +81,19 @@ BatchGeo batch_geometry(const tiplhip_volume* from, const tiplhip_volume* to, in
     return b;
 }
 
-// Chooses items per workgroup so that the pair's K candidates (each split
-// into b.parts workgroups per chunk) take about `target_blocks` workgroups;
-// max_voxels_per_block bounds the voxels one workgroup adds (the uint16 LDS
-// counters of the 256-bin histogram hold at most 65535).
+// Chooses items per workgroup so that the pair's K candidates take about
+// `target_blocks` workgroups; max_voxels_per_block bounds the voxels one
+// workgroup adds (the uint16 LDS counters of the 256-bin histogram hold at
+// most 65535).
 void batch_blocks(BatchGeo& b, uint32_t max_voxels_per_block, uint32_t target_blocks)
 {
-    target_blocks = std::max<uint32_t>(1u, target_blocks / std::max<uint32_t>(1u, b.parts));
+    target_blocks = std::max<uint32_t>(1u, target_blocks);
     const uint32_t max_items = std::max<uint32_t>(1u, max_voxels_per_block / (64u * b.zc));
     uint64_t want = (uint64_t(b.items) * uint64_t(b.K) + target_blocks - 1) / target_blocks;
     uint32_t ipb = uint32_t(std::min<uint64_t>(want, max_items));
     ipb = std::max<uint32_t>(ipb, std::min<uint32_t>(4u, max_items));
     b.items_per_block = ipb;
-    b.total_blocks = ((b.items + ipb - 1) / ipb) * b.K * b.parts;
+    b.total_blocks = ((b.items + ipb - 1) / ipb) * b.K;
 }
 
 // One image pair of a launch.  Candidates are numbered globally: the
@@ -365,46 +362,31 @@ __global__ void corr_finalize_kernel(const Segment* __restrict__ segs, uint32_t 
 // the end the non-zero words are flushed into the candidate's global int32
 // histogram.
 // ---------------------------------------------------------------------------
-// 256 bins are split into PARTS ranges of `from` rows: the workgroups of
-// one chunk and candidate each count the samples whose `from` value lies in
-// their range (1/PARTS of the histogram in LDS: 128 KB / PARTS), so more
-// workgroups fit a CU.  A wave samples a plane only when one of its lanes
-// has a `from` value in the range -- `from` is spatially coherent, so most
-// waves sample in exactly one of the workgroups.
-template <int BINS, int PARTS>
+template <int BINS>
 struct MiTraits;
-template <int PARTS>
-struct MiTraits<256, PARTS> {
+template <>
+struct MiTraits<256> {
     static constexpr bool kAggregate = false;  // values spread over many bins
-    static constexpr int kRows = 256 / PARTS;  // `from` rows a workgroup counts
-    static constexpr int kWords = 128 * kRows;
+    static constexpr int kWords = 32768;
     static constexpr int kBand = 8;
-    static constexpr int kShift = 0;
-    static constexpr int kThreads = PARTS == 4 ? 512 : 1024;
-    static constexpr int kPartShift = PARTS == 1 ? 8 : PARTS == 2 ? 7 : 6;  // from >> this = part
+    static constexpr int kThreads = 1024;
     __device__ static uint32_t word(uint32_t bin) { return bin >> 1; }
     __device__ static uint32_t inc(uint32_t bin) { return (bin & 1u) ? 0x10000u : 1u; }
 };
 template <>
-struct MiTraits<64, 1> {
+struct MiTraits<64> {
     static constexpr bool kAggregate = true;  // 64 x 64 bins: runs of equal bins are common
-    static constexpr int kRows = 64;
     static constexpr int kWords = 4096;
-    static constexpr int kBand = 6;
-    static constexpr int kShift = 2;
     static constexpr int kThreads = 256;
-    static constexpr int kPartShift = 8;
-    __device__ static uint32_t word(uint32_t bin) { return bin; }
-    __device__ static uint32_t inc(uint32_t) { return 1u; }
 };
 
-template <int BINS, int PARTS, bool ZT>
-__global__ __launch_bounds__((MiTraits<BINS, PARTS>::kThreads)) void mi_hist_kernel(
+template <int BINS, bool ZT>
+__global__ __launch_bounds__(MiTraits<BINS>::kThreads) void mi_hist_kernel(
     const Segment* __restrict__ segs, uint32_t nseg, uint32_t total_blocks,
     const float* __restrict__ Ts, const float4* __restrict__ ztab, uint32_t zstride,
     int* __restrict__ hist, uint32_t* __restrict__ to_hist)
 {
-    using Tr = MiTraits<BINS, PARTS>;
+    using Tr = MiTraits<BINS>;
     __shared__ uint32_t lds[Tr::kWords];
     for (uint32_t i = threadIdx.x; i < uint32_t(Tr::kWords); i += blockDim.x) lds[i] = 0u;
     __syncthreads();
@@ -412,12 +394,9 @@ __global__ __launch_bounds__((MiTraits<BINS, PARTS>::kThreads)) void mi_hist_ker
     const uint32_t L = xcd_remap(blockIdx.x, total_blocks);
     const Segment& sg = segs[find_segment(segs, nseg, L, ByBlock{})];
     const BatchGeo g = sg.g;
-    // logical block = ((chunk * K) + candidate) * PARTS + part: the parts of
-    // one chunk and candidate are neighbours (one XCD, the same `to` cells)
     const uint32_t local = L - sg.block_begin;
-    const uint32_t part = local % uint32_t(PARTS);
-    const uint32_t cand = sg.cand_begin + (local / uint32_t(PARTS)) % g.K;
-    const uint32_t chunk = local / uint32_t(PARTS) / g.K;
+    const uint32_t cand = sg.cand_begin + local % g.K;
+    const uint32_t chunk = local / g.K;
     float T[12];
 #pragma unroll
     for (int i = 0; i < 12; ++i) T[i] = Ts[12 * cand + i];
@@ -452,50 +431,31 @@ __global__ __launch_bounds__((MiTraits<BINS, PARTS>::kThreads)) void mi_hist_ker
                     atomicAdd(w, 1u);
                 }
             } else {
-                // the row within this part's range
-                const uint32_t bin = (((fv >> Tr::kShift) & uint32_t(Tr::kRows - 1)) << Tr::kBand) |
-                                     (tv >> Tr::kShift);
+                const uint32_t bin = (fv << Tr::kBand) | tv;
                 atomicAdd(&lds[Tr::word(bin)], Tr::inc(bin));
             }
-        };
-        // one sample of plane z (whole wave), counted where `in`
-        auto one = [&](uint32_t z, uint32_t fv, bool in) {
-            const float4 zp = z_products<ZT>(ztab, zrow, z, T);
-            const uint32_t tv = sample_u8_packed(r8, g.to, (cs.hx + zp.x) + T[9],
-                                                 (cs.hy + zp.y) + T[10], (cs.hz + zp.z) + T[11]);
-            if (in) count(fv, tv);
         };
         uint32_t z = m.z0;
         // two planes per step: both samples' gathers are in flight together
         for (; z + 1 < m.z1; z += 2, fidx += 2 * fwh) {
             const uint32_t fva = load_u8(rf, fidx, 0), fvb = load_u8(rf, fidx + fwh, 0);
-            const bool ina = PARTS == 1 || (fva >> Tr::kPartShift) == part;
-            const bool inb = PARTS == 1 || (fvb >> Tr::kPartShift) == part;
-            const bool any_a = PARTS == 1 || __ballot(ina) != 0ull;  // wave-uniform
-            const bool any_b = PARTS == 1 || __ballot(inb) != 0ull;
-            if (any_a && any_b) {
-                const float4 za = z_products<ZT>(ztab, zrow, z, T);
-                const float4 zb = z_products<ZT>(ztab, zrow, z + 1, T);
-                const float pxa = (cs.hx + za.x) + T[9];
-                const float pya = (cs.hy + za.y) + T[10];
-                const float pza = (cs.hz + za.z) + T[11];
-                const float pxb = (cs.hx + zb.x) + T[9];
-                const float pyb = (cs.hy + zb.y) + T[10];
-                const float pzb = (cs.hz + zb.z) + T[11];
-                uint32_t tva, tvb;
-                sample2_u8_packed(r8, g.to, pxa, pya, pza, pxb, pyb, pzb, tva, tvb);
-                if (ina) count(fva, tva);
-                if (inb) count(fvb, tvb);
-            } else if (any_a) {
-                one(z, fva, ina);
-            } else if (any_b) {
-                one(z + 1, fvb, inb);
-            }
+            const float4 za = z_products<ZT>(ztab, zrow, z, T), zb = z_products<ZT>(ztab, zrow, z + 1, T);
+            const float pxa = (cs.hx + za.x) + T[9];
+            const float pya = (cs.hy + za.y) + T[10];
+            const float pza = (cs.hz + za.z) + T[11];
+            const float pxb = (cs.hx + zb.x) + T[9];
+            const float pyb = (cs.hy + zb.y) + T[10];
+            const float pzb = (cs.hz + zb.z) + T[11];
+            uint32_t tva, tvb;
+            sample2_u8_packed(r8, g.to, pxa, pya, pza, pxb, pyb, pzb, tva, tvb);
+            count(fva, tva);
+            count(fvb, tvb);
         }
         if (z < m.z1) {
             const uint32_t fv = load_u8(rf, fidx, 0);
-            const bool in = PARTS == 1 || (fv >> Tr::kPartShift) == part;
-            if (PARTS == 1 || __ballot(in) != 0ull) one(z, fv, in);
+            const float4 zp = z_products<ZT>(ztab, zrow, z, T);
+            count(fv, sample_u8_packed(r8, g.to, (cs.hx + zp.x) + T[9], (cs.hy + zp.y) + T[10],
+                                       (cs.hz + zp.z) + T[11]));
         }
     }
     __syncthreads();
@@ -504,12 +464,12 @@ __global__ __launch_bounds__((MiTraits<BINS, PARTS>::kThreads)) void mi_hist_ker
     // thread's words lie in one column (pair): it also sums that column over
     // its rows, and the partial column sums go to the candidate's to_hist
     // (the histogram of the sampled values, get_mutual_info's to_hist).
-    constexpr uint32_t kWpr = uint32_t(Tr::kWords) / uint32_t(Tr::kRows);  // words per row
+    constexpr uint32_t kWpr = uint32_t(Tr::kWords) / BINS;   // words per row
     constexpr uint32_t kBpw = BINS / kWpr;                   // bins per word
     constexpr uint32_t kQ = uint32_t(Tr::kThreads) / kWpr;   // threads per column word
     static_assert(uint32_t(Tr::kThreads) % kWpr == 0 && kQ * BINS <= uint32_t(Tr::kWords),
                   "flush layout");
-    int* h = hist + size_t(cand) * BINS * BINS + size_t(part) * Tr::kRows * BINS;
+    int* h = hist + size_t(cand) * BINS * BINS;
     uint32_t col_lo = 0, col_hi = 0;
     for (uint32_t i = threadIdx.x; i < uint32_t(Tr::kWords); i += blockDim.x) {
         const uint32_t v = lds[i];
@@ -779,7 +739,6 @@ struct Staged {
     const int4* crop = nullptr;
     double samples = 0.0;   // voxel-candidate samples of the launch
     uint32_t zstride = 0;   // the largest `from` depth: rows of the z-product table
-    uint32_t parts = 1;     // 256-bin MI: histogram ranges (LaunchShape::parts)
     size_t bytes = 0;       // staged bytes (transforms, crop flags, segment table)
     bool grew = false;      // the device staging buffer was reallocated
 };
@@ -809,14 +768,13 @@ constexpr double kMaxItemsPerBlock = 256.0;
 // candidates: the 256-bin histogram (128 KB of LDS, one 1024-thread
 // workgroup per CU) is fastest with about one workgroup per CU -- fewer,
 // fuller workgroups amortise the LDS clear and flush -- and the 64-bin one
-// with about eight.  Splitting the 256-bin histogram into two ranges of
-// `from` rows (two workgroups per CU) made the serving mix's large launches
-// slower (409 -> 657 us on average; each workgroup waits for its `from`
-// values before it may skip a plane), so one range is the default and the
-// split is left to the autotuner.
+// with about eight.  (Splitting the 256-bin histogram into 2 or 4 ranges
+// of `from` rows, one workgroup each with a half or a quarter of the LDS,
+// was slower at every size from 64^3 with 24 or more candidates: 1.4x and
+// 2.6x at 182x218x182, 64 candidates.)
 LaunchShape default_shape(int cost, int bins)
 {
-    return hist256(cost, bins) ? LaunchShape{16u, 1u, 1u} : LaunchShape{16u, 8u, 1u};
+    return LaunchShape{16u, hist256(cost, bins) ? 1u : 8u};
 }
 
 TuneKey tune_key(const tiplhip_volume* from, const tiplhip_volume* to, int K, int cost, int bins)
@@ -844,11 +802,9 @@ Staged stage_jobs(tiplhip_ctx* ctx, const std::vector<PairJob>& jobs, const floa
         auto it = ctx->tuned.find(tune_key(jobs[0].from, jobs[0].to, jobs[0].K, cost, bins));
         if (it != ctx->tuned.end()) shape = it->second;
     }
-    st.parts = hist256(cost, bins) ? shape.parts : 1u;
     double work = 0.0;
     for (size_t j = 0; j < jobs.size(); ++j) {
         segs[j].g = batch_geometry(jobs[j].from, jobs[j].to, jobs[j].K, shape.zc);
-        segs[j].g.parts = st.parts;
         work += double(segs[j].g.items) * double(jobs[j].K);
     }
     // at least per_cu workgroups per CU, and no workgroup with more than
@@ -1003,24 +959,18 @@ void launch_mi_hist(tiplhip_ctx* ctx, const Staged& st, int bins)
     int* h = ctx->d_hist.as<int>();
     uint32_t* th = to_hist_of(ctx, st, bins);
     timing_begin(ctx);
-    auto go = [&](auto kernel, int threads) {
-        hipLaunchKernelGGL(kernel, dim3(st.total_blocks), dim3(threads), 0, ctx->stream, st.segs,
-                           st.nseg, st.total_blocks, st.T, zt, zt ? st.zstride : 0u, h, th);
-    };
-    if (bins == 64) {
-        if (zt) go(mi_hist_kernel<64, 1, true>, MiTraits<64, 1>::kThreads);
-        else go(mi_hist_kernel<64, 1, false>, MiTraits<64, 1>::kThreads);
-    } else if (st.parts == 4) {
-        if (zt) go(mi_hist_kernel<256, 4, true>, MiTraits<256, 4>::kThreads);
-        else go(mi_hist_kernel<256, 4, false>, MiTraits<256, 4>::kThreads);
-    } else if (st.parts == 2) {
-        if (zt) go(mi_hist_kernel<256, 2, true>, MiTraits<256, 2>::kThreads);
-        else go(mi_hist_kernel<256, 2, false>, MiTraits<256, 2>::kThreads);
-    } else {
-        TIPLHIP_REQUIRE(st.parts == 1, "histogram ranges must be 1, 2 or 4");
-        if (zt) go(mi_hist_kernel<256, 1, true>, MiTraits<256, 1>::kThreads);
-        else go(mi_hist_kernel<256, 1, false>, MiTraits<256, 1>::kThreads);
-    }
+    if (bins == 256 && zt)
+        hipLaunchKernelGGL((mi_hist_kernel<256, true>), dim3(st.total_blocks), dim3(1024), 0,
+                           ctx->stream, st.segs, st.nseg, st.total_blocks, st.T, zt, st.zstride, h, th);
+    else if (bins == 256)
+        hipLaunchKernelGGL((mi_hist_kernel<256, false>), dim3(st.total_blocks), dim3(1024), 0,
+                           ctx->stream, st.segs, st.nseg, st.total_blocks, st.T, nullptr, 0u, h, th);
+    else if (zt)
+        hipLaunchKernelGGL((mi_hist_kernel<64, true>), dim3(st.total_blocks), dim3(256), 0,
+                           ctx->stream, st.segs, st.nseg, st.total_blocks, st.T, zt, st.zstride, h, th);
+    else
+        hipLaunchKernelGGL((mi_hist_kernel<64, false>), dim3(st.total_blocks), dim3(256), 0,
+                           ctx->stream, st.segs, st.nseg, st.total_blocks, st.T, nullptr, 0u, h, th);
     TIPLHIP_CHECK(hipGetLastError());
     timing_end(ctx);
 }
@@ -1175,7 +1125,7 @@ void cost_jobs(tiplhip_ctx* ctx, std::vector<PairJob> jobs, const float* T, int 
         const uint64_t key_fields[] = {uint64_t(cost), uint64_t(mi ? bins : 0), uint64_t(K),
                                        uint64_t(st.nseg), uint64_t(st.total_blocks), uint64_t(st.bytes),
                                        uint64_t(st.samples >= kZTableSamples), uint64_t(st.zstride),
-                                       uint64_t(st.crop != nullptr), uint64_t(st.parts),
+                                       uint64_t(st.crop != nullptr),
                                        uint64_t(reinterpret_cast<uintptr_t>(ctx->d_T.p)),
                                        uint64_t(reinterpret_cast<uintptr_t>(ctx->h_T.p)),
                                        uint64_t(reinterpret_cast<uintptr_t>(ctx->d_hist.p)),
@@ -1336,7 +1286,7 @@ void cost_noalloc(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_vo
         int* h = reinterpret_cast<int*>(scr + kNaHist);
         uint32_t* th = reinterpret_cast<uint32_t*>(h + size_t(bins) * bins * size_t(K));
         if (bins == 256) {
-            hipLaunchKernelGGL((mi_hist_kernel<256, 1, false>), dim3(blocks), dim3(1024), 0, st, d_seg, 1u,
+            hipLaunchKernelGGL((mi_hist_kernel<256, false>), dim3(blocks), dim3(1024), 0, st, d_seg, 1u,
                                blocks, d_T, nullptr, 0u, h, th);
             TIPLHIP_CHECK(hipGetLastError());
             constexpr int G = entropy_slices<256>();
@@ -1346,7 +1296,7 @@ void cost_noalloc(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_vo
             hipLaunchKernelGGL(mi_entropy_final_kernel<256>, dim3((uint32_t(K) + 255) / 256), dim3(256), 0,
                                st, d_costs + K, uint32_t(K), d_costs, nullptr, th);
         } else {
-            hipLaunchKernelGGL((mi_hist_kernel<64, 1, false>), dim3(blocks), dim3(256), 0, st, d_seg, 1u,
+            hipLaunchKernelGGL((mi_hist_kernel<64, false>), dim3(blocks), dim3(256), 0, st, d_seg, 1u,
                                blocks, d_T, nullptr, 0u, h, th);
             TIPLHIP_CHECK(hipGetLastError());
             hipLaunchKernelGGL(mi_entropy_kernel<64>, dim3(uint32_t(K)), dim3(kEntropyThreads), 0, st, d_seg,
@@ -1491,23 +1441,19 @@ int tiplhip_autotune(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip
             }
         }
         volume_ensure_packed(ctx, const_cast<tiplhip_volume*>(t));
-        const bool wide = hist256(cost, bins);  // large-LDS workgroups, split into row ranges
+        const bool wide = hist256(cost, bins);  // 1024-thread workgroups
         const uint32_t zcs[] = {8u, 16u, 32u};
         const uint32_t pcs_narrow[] = {2u, 4u, 8u, 16u, 0u}, pcs_wide[] = {1u, 2u, 4u, 8u, 0u};
         const uint32_t* pcs = wide ? pcs_wide : pcs_narrow;
-        const uint32_t parts_wide[] = {1u, 2u, 4u}, parts_narrow[] = {1u};
-        const uint32_t* parts = wide ? parts_wide : parts_narrow;
-        const int nparts = wide ? 3 : 1;
         LaunchShape best = default_shape(cost, bins);
         double best_t = 1e30;
         const bool keep_timing = ctx->kernel_timing;
         const double keep_ms = ctx->kernel_ms;
         const int64_t keep_n = ctx->kernel_launches;
         ctx->kernel_timing = true;
-        for (int q = 0; q < nparts; ++q)
         for (uint32_t z : zcs)
             for (int p = 0; p < 5; ++p) {
-                const LaunchShape shape{z, pcs[p], parts[q]};
+                const LaunchShape shape{z, pcs[p]};
                 const Staged st = stage_jobs(ctx, {PairJob{f, t, K}}, Tk.data(),
                                              crop.empty() ? nullptr : crop.data(), cost, bins, &shape);
                 ctx->kernel_ms = 0.0;
@@ -1537,7 +1483,7 @@ int tiplhip_autotune(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip
 }
 
 int tiplhip_set_launch_shape(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_volume* to,
-                             int K, int cost, int bins, int zc, int per_cu, int parts)
+                             int K, int cost, int bins, int zc, int per_cu)
 {
     return guarded([&] {
         TIPLHIP_REQUIRE(ctx != nullptr, "null context");
@@ -1549,13 +1495,10 @@ int tiplhip_set_launch_shape(tiplhip_ctx* ctx, const tiplhip_volume* from, const
         if (cost == TIPLHIP_COST_MI) TIPLHIP_REQUIRE(bins == 256 || bins == 64, "bins must be 256 or 64");
         TIPLHIP_REQUIRE(zc >= 1 && zc <= 1024, "planes per item must be in [1, 1024]");
         TIPLHIP_REQUIRE(per_cu >= 0 && per_cu <= 64, "workgroups per CU must be in [0, 64]");
-        TIPLHIP_REQUIRE(parts == 1 || ((parts == 2 || parts == 4) && hist256(cost, bins)),
-                        "histogram ranges must be 1, or 2 or 4 for 256-bin MI");
         // correlation evaluates with the roles swapped when `from` is larger
         const tiplhip_volume *f = from, *t = to;
         if (cost == TIPLHIP_COST_CORR && f->n > t->n) std::swap(f, t);
-        ctx->tuned[tune_key(f, t, K, cost, bins)] =
-            LaunchShape{uint32_t(zc), uint32_t(per_cu), uint32_t(parts)};
+        ctx->tuned[tune_key(f, t, K, cost, bins)] = LaunchShape{uint32_t(zc), uint32_t(per_cu)};
     });
 }
 
@@ -1604,7 +1547,7 @@ int tiplhip_cost_batch_pairs(tiplhip_ctx* ctx, const tiplhip_pair_costs* pairs, 
         // sets of many registrations): 256-bin histograms then use the fewest,
         // fullest workgroups, which spend the least on clearing and flushing
         // their 128 KB LDS histograms per sample.
-        const LaunchShape fullest{16u, 0u, 1u};
+        const LaunchShape fullest{16u, 0u};
         cost_jobs(ctx, jobs, T.data(), cost, bins, costs.data(), hist256(cost, bins) ? &fullest : nullptr);
         size_t base = 0;
         for (int i = 0; i < n; ++i) {

@@ -111,13 +111,10 @@ bool matrix_inverse(float* T);
 bool is_integer_shift(const float* T, int* from_pos);
 
 // Launch shape of the affine cost kernels for one image pair: planes per
-// work item, workgroups per CU and, for 256-bin MI histograms, the number of
-// `from`-row ranges the histogram is split into (one workgroup each; 1, 2
-// or 4) -- chosen by tiplhip_autotune per key.
+// work item and workgroups per CU (chosen by tiplhip_autotune per key).
 struct LaunchShape {
     uint32_t zc;
     uint32_t per_cu;
-    uint32_t parts = 1;
 };
 // from dims, to dims, K, cost, bins
 using TuneKey = std::array<int, 9>;
