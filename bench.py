@@ -225,8 +225,9 @@ def batch_registration_bench(pairs, slots=16):
     """The batch of uneven affine+CDM pairs registered one after another on
     the plain path (tiplhip_register) against all of them submitted to the
     server at once (their candidate sets merged into shared multi-pair
-    launches, the CDM runs overlapped on the device); every served result is
-    compared with the plain one bit for bit."""
+    launches, their CDM iterations into batched steps); the served run is
+    repeated with CDM batching off to measure what it adds.  Every served
+    result is compared with the plain one."""
     import tipl_amd
     ctx = tipl_amd.Context(0)
     tipl_amd.register(ctx, pairs[0][0], pairs[0][1], nonlinear=True)  # warm-up
@@ -234,20 +235,34 @@ def batch_registration_bench(pairs, slots=16):
     plain = [tipl_amd.register(ctx, f, m, nonlinear=True, want_field=False) for f, m in pairs]
     plain_dt = time.perf_counter() - t0
     ctx.close()
-    with tipl_amd.Server(0, slots=slots, hbm_budget=192 << 30) as srv:
-        t0 = time.perf_counter()
-        tickets = [srv.submit(f, m, nonlinear=True, want_field=False) for f, m in pairs]
-        served = [srv.wait(t) for t in tickets]
-        served_dt = time.perf_counter() - t0
-    same = all(np.array_equal(a["T"].view(np.uint32), b["T"].view(np.uint32)) and
-               a["evaluations"] == b["evaluations"] and a["cdm_iterations"] == b["cdm_iterations"]
-               for a, b in zip(served, plain))
+
+    def served_run(cdm_batch):
+        os.environ["TIPLHIP_SERVER_CDM_BATCH"] = "1" if cdm_batch else "0"
+        try:
+            with tipl_amd.Server(0, slots=slots, hbm_budget=192 << 30) as srv:
+                t0 = time.perf_counter()
+                tickets = [srv.submit(f, m, nonlinear=True, want_field=False) for f, m in pairs]
+                served = [srv.wait(t) for t in tickets]
+                dt = time.perf_counter() - t0
+                st = srv.stats()
+        finally:
+            os.environ.pop("TIPLHIP_SERVER_CDM_BATCH", None)
+        same = all(np.array_equal(a["T"].view(np.uint32), b["T"].view(np.uint32)) and
+                   a["evaluations"] == b["evaluations"] and a["cdm_iterations"] == b["cdm_iterations"]
+                   and a["r_nonlinear"] == b["r_nonlinear"] for a, b in zip(served, plain))
+        return dt, st, same
+
+    alone_dt, _, same_alone = served_run(False)
+    served_dt, st, same = served_run(True)
     return {"pairs": len(pairs), "sides": [int(f.shape[0]) for f, _ in pairs], "slots": slots,
             "plain_pairs_per_s": round(len(pairs) / plain_dt, 3),
             "batched_pairs_per_s": round(len(pairs) / served_dt, 3),
             "batched_over_plain": round(plain_dt / served_dt, 2),
             "cdm_iterations": int(sum(r["cdm_iterations"] for r in plain)),
-            "results_equal_plain": bool(same)}
+            "cdm_batched_steps": int(st["cdm_steps"]), "cdm_batched_fields": int(st["cdm_fields"]),
+            "served_cdm_unbatched_pairs_per_s": round(len(pairs) / alone_dt, 3),
+            "cdm_batching_gain": round(alone_dt / served_dt, 3),
+            "results_equal_plain": bool(same and same_alone)}
 
 
 def make_batch_pairs(n):
@@ -345,6 +360,7 @@ def serving_bench(reqs, slots, rate_factor, seed=4000):
         "requests": len(reqs), "slots": slots,
         "nonlinear_requests": int(sum(nl for _, _, nl in reqs)),
         "batch_launches": int(sat["batch_launches"]), "batched_pairs": int(sat["batched_pairs"]),
+        "cdm_batched_steps": int(sat["cdm_steps"]), "cdm_batched_fields": int(sat["cdm_fields"]),
         "hbm_peak_gib": round(sat["hbm_peak"] / 2 ** 30, 2),
         "results_equal_plain": bool(same1 and same2),
         "plain_mean_split_ms": split,

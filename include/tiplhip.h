@@ -509,6 +509,10 @@ typedef struct tiplhip_server_stats {
     double latency_p50_ms, latency_p95_ms, latency_p99_ms;  /* submit to done */
     double wall_s;            /* first submit to last completion */
     double regs_per_s;        /* completed / wall_s */
+    /* batched CDM iterations: steps launched since the server started and
+       the fields over them (counted on the device's batcher, which servers
+       on one device share; both 0 when batching is off) */
+    int64_t cdm_steps, cdm_fields;
 } tiplhip_server_stats;
 
 int tiplhip_server_create(int device, int slots, size_t hbm_budget, tiplhip_server** out);

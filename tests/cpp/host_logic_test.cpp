@@ -8,10 +8,16 @@
 //     isolation (a throwing registration, a pair the cost launch refuses),
 //     ticket misuse, shutdown with requests still queued;
 //   * the host's wait for results that kernels store into host memory
-//     (tipl_amd/csrc/host_wait.hpp): spin, timeout, fallback, drain on unwind.
+//     (tipl_amd/csrc/host_wait.hpp): spin, timeout, fallback, drain on unwind;
+//   * the gate of batched CDM steps (tipl_amd/csrc/step_gate.hpp): jobs in
+//     step share batches, every step is launched once and in order, a
+//     leaving job releases the steps waiting for it, a failed launch fails
+//     exactly the jobs of its batch.
 // Exits 0 when every check holds; prints the failed check otherwise.
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
+#include <mutex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -21,6 +27,7 @@
 
 #include "../../tipl_amd/csrc/host_wait.hpp"
 #include "../../tipl_amd/csrc/server_core.hpp"
+#include "../../tipl_amd/csrc/step_gate.hpp"
 
 // errors.hpp declares these for guarded(); the core does not call them.
 namespace tiplhip {
@@ -396,6 +403,164 @@ void test_wait_for_slots()
     }
 }
 
+// ---------------------------------------------------------------------------
+// StepGate
+// ---------------------------------------------------------------------------
+struct GateJob : tiplhip::GateSlot {
+    int id = 0;
+    int next = 0;        // the step the owner queues (set before run)
+    int launched_steps = 0;  // written by the launch (under the gate's lock)
+    bool poison = false; // the launch fails the batch holding this step
+};
+
+struct GateLog {
+    std::vector<int> sizes;  // batch sizes, in launch order (gate lock held)
+};
+
+// Start barrier for the gate's threads.
+struct Barrier {
+    std::mutex mu;
+    std::condition_variable cv;
+    int waiting = 0, n;
+    explicit Barrier(int count) : n(count) {}
+    void arrive()
+    {
+        std::unique_lock<std::mutex> lk(mu);
+        if (++waiting == n) {
+            cv.notify_all();
+            return;
+        }
+        cv.wait(lk, [&] { return waiting >= n; });
+    }
+};
+
+void test_step_gate_lockstep(int threads, int max_batch)
+{
+    GateLog log;
+    tiplhip::StepGate<GateJob> gate(max_batch, std::chrono::seconds(10), [&](GateJob* const* J, int n) {
+        CHECK(n >= 1 && n <= max_batch);
+        for (int i = 0; i < n; ++i) {
+            for (int k = 0; k < i; ++k) CHECK(J[k] != J[i]);
+            CHECK(J[i]->next == J[i]->launched_steps);  // in order, once each
+            ++J[i]->launched_steps;
+        }
+        log.sizes.push_back(n);
+    });
+    const int steps = 40;
+    std::vector<GateJob> jobs(static_cast<size_t>(threads));
+    Barrier start(threads);
+    std::vector<std::thread> ts;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int t = 0; t < threads; ++t) {
+        ts.emplace_back([&, t] {
+            GateJob& j = jobs[size_t(t)];
+            j.id = t;
+            gate.join();
+            start.arrive();
+            for (int i = 0; i < steps; ++i) {
+                j.next = i;
+                gate.run(j);
+                jitter(30);  // the device step and the host's decision
+            }
+            gate.leave([] {});
+        });
+    }
+    for (auto& th : ts) th.join();
+    CHECK(std::chrono::steady_clock::now() - t0 < std::chrono::seconds(5));  // no gather timeout
+    for (const GateJob& j : jobs) CHECK(j.launched_steps == steps);
+    CHECK(gate.steps() == int64_t(threads) * steps);
+    CHECK(gate.batches() == int64_t(log.sizes.size()));
+    if (threads <= max_batch) {
+        // every job joined before the first step: each batch holds all of them
+        CHECK(gate.batches() == steps);
+        for (int n : log.sizes) CHECK(n == threads);
+    } else {
+        for (int n : log.sizes) CHECK(n <= max_batch);
+        CHECK(gate.batches() < gate.steps());
+    }
+}
+
+void test_step_gate_leave_releases()
+{
+    // A and B wait for C, which leaves without queueing another step: its
+    // leave() launches their steps (the gather time is far longer than the test).
+    tiplhip::StepGate<GateJob> gate(8, std::chrono::seconds(30), [](GateJob* const* J, int n) {
+        for (int i = 0; i < n; ++i) ++J[i]->launched_steps;
+    });
+    GateJob a, b, c;
+    gate.join();
+    gate.join();
+    gate.join();
+    const auto t0 = std::chrono::steady_clock::now();
+    std::thread ta([&] { gate.run(a); });
+    std::thread tb([&] { gate.run(b); });
+    std::this_thread::sleep_for(std::chrono::milliseconds(20));
+    CHECK(!a.launched.load() && !b.launched.load());
+    gate.leave([] {});
+    ta.join();
+    tb.join();
+    CHECK(a.launched_steps == 1 && b.launched_steps == 1 && c.launched_steps == 0);
+    CHECK(std::chrono::steady_clock::now() - t0 < std::chrono::seconds(5));
+    gate.leave([] {});
+    gate.leave([] {});
+    // alone: a step goes out at once
+    gate.join();
+    gate.run(a);
+    CHECK(a.launched_steps == 2);
+    gate.leave([] {});
+    // gather timeout: two active, one queues; it is launched after ~gather
+    tiplhip::StepGate<GateJob> quick(8, std::chrono::milliseconds(2), [](GateJob* const* J, int n) {
+        for (int i = 0; i < n; ++i) ++J[i]->launched_steps;
+    });
+    quick.join();
+    quick.join();
+    const auto t1 = std::chrono::steady_clock::now();
+    quick.run(c);
+    CHECK(c.launched_steps == 1);
+    CHECK(std::chrono::steady_clock::now() - t1 >= std::chrono::milliseconds(2));
+    quick.leave([] {});
+    quick.leave([] {});
+}
+
+void test_step_gate_failure()
+{
+    // Four jobs in step; at step 3 job 2's launch throws: exactly the jobs of
+    // that batch see the error, and later steps launch normally.
+    const int threads = 4, steps = 8;
+    tiplhip::StepGate<GateJob> gate(8, std::chrono::seconds(10), [](GateJob* const* J, int n) {
+        for (int i = 0; i < n; ++i)
+            if (J[i]->poison) throw Error(TIPLHIP_ERR_DEVICE, "launch failed");
+        for (int i = 0; i < n; ++i) ++J[i]->launched_steps;
+    });
+    std::vector<GateJob> jobs(static_cast<size_t>(threads));
+    std::vector<int> errors(static_cast<size_t>(threads), 0);
+    Barrier start(threads);
+    std::vector<std::thread> ts;
+    for (int t = 0; t < threads; ++t) {
+        ts.emplace_back([&, t] {
+            GateJob& j = jobs[size_t(t)];
+            gate.join();
+            start.arrive();
+            for (int i = 0; i < steps; ++i) {
+                j.poison = t == 2 && i == 3;
+                try {
+                    gate.run(j);
+                } catch (const Error& e) {
+                    CHECK(e.code == TIPLHIP_ERR_DEVICE);
+                    ++errors[size_t(t)];
+                }
+            }
+            gate.leave([] {});
+        });
+    }
+    for (auto& th : ts) th.join();
+    // all four were in the failed batch (each batch holds every job)
+    for (int t = 0; t < threads; ++t) {
+        CHECK(errors[size_t(t)] == 1);
+        CHECK(jobs[size_t(t)].launched_steps == steps - 1);
+    }
+}
+
 void test_drain_on_unwind()
 {
     int drained = 0;
@@ -437,6 +602,10 @@ int main()
     test_failure_isolation();
     test_ticket_misuse_and_admission();
     test_shutdown_with_queue();
+    test_step_gate_lockstep(6, 8);
+    test_step_gate_lockstep(12, 8);
+    test_step_gate_leave_releases();
+    test_step_gate_failure();
     std::printf("host logic ok (%d stub launches)\n", StubBackend::launches.load());
     return 0;
 }

@@ -217,3 +217,53 @@ def test_bidirectional_served_equals_plain(ctx, mix):
         got = [srv.wait(t) for t in ts]
     for a, b in zip(got, plain):
         _same(a, b)
+
+
+@pytest.fixture(scope="module")
+def cdm_mix():
+    """Six nonlinear pairs of different shapes, for the batched CDM tests."""
+    from tipl_amd import phantoms
+    shapes = [(40, 48, 40), (64, 64, 64), (56, 44, 60), (72, 80, 64), (48, 48, 48), (96, 80, 72)]
+    return [phantoms.serving_pair(4300 + i, s, True, np.float32)[:2] for i, s in enumerate(shapes)]
+
+
+def test_cdm_batched_across_registrations(ctx, cdm_mix):
+    """Six nonlinear registrations, both CDM directions, served together on
+    six slots: the server's cdm() iterations go out as batched steps (more
+    fields than steps, several fields per launch) and every result -- fields
+    of both directions, mappings, iteration counts and r values -- equals
+    the plain path's bit for bit."""
+    import tipl_amd
+    outs = ("f2t", "to2from", "from2to", "warped")
+    kw = dict(nonlinear=True, reverse=True, outputs=outs)
+    plain = []
+    for f, m in cdm_mix:
+        print(f"plain bidirectional {f.shape}", flush=True)
+        plain.append(tipl_amd.register(ctx, f, m, **kw))
+    with tipl_amd.Server(0, slots=6, hbm_budget=16 << 30) as srv:
+        ts = [srv.submit(f, m, **kw) for f, m in cdm_mix]
+        got = [srv.wait(t) for t in ts]
+        st = srv.stats()
+    print(f"batched CDM: {st['cdm_fields']} fields in {st['cdm_steps']} steps", flush=True)
+    for a, b in zip(got, plain):
+        _same(a, b)
+    iters = sum(p["cdm_iterations"] for p in plain)
+    assert st["cdm_steps"] > 0
+    assert st["cdm_fields"] >= iters          # the forward ones at least
+    assert st["cdm_fields"] >= 1.3 * st["cdm_steps"]
+
+
+def test_cdm_batching_off(ctx, cdm_mix, monkeypatch):
+    """TIPLHIP_SERVER_CDM_BATCH=0 at server creation: each registration runs
+    its cdm() alone on its stream (no batched steps), same results."""
+    import tipl_amd
+    monkeypatch.setenv("TIPLHIP_SERVER_CDM_BATCH", "0")
+    sel = cdm_mix[:3]
+    plain = [tipl_amd.register(ctx, f, m, nonlinear=True) for f, m in sel]
+    with tipl_amd.Server(0, slots=3, hbm_budget=16 << 30) as srv:
+        ts = [srv.submit(f, m, nonlinear=True) for f, m in sel]
+        got = [srv.wait(t) for t in ts]
+        st = srv.stats()
+    assert st["cdm_steps"] == 0 and st["cdm_fields"] == 0
+    for a, b in zip(got, plain):
+        _same(a, b)

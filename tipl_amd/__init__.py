@@ -97,7 +97,8 @@ class ServerStats(C.Structure):
                 ("running", C.c_int64), ("queued", C.c_int64), ("batch_launches", C.c_int64),
                 ("batched_pairs", C.c_int64), ("hbm_in_use", C.c_size_t), ("hbm_peak", C.c_size_t),
                 ("latency_p50_ms", C.c_double), ("latency_p95_ms", C.c_double),
-                ("latency_p99_ms", C.c_double), ("wall_s", C.c_double), ("regs_per_s", C.c_double)]
+                ("latency_p99_ms", C.c_double), ("wall_s", C.c_double), ("regs_per_s", C.c_double),
+                ("cdm_steps", C.c_int64), ("cdm_fields", C.c_int64)]
 
 
 _lib = None

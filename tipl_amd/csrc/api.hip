@@ -447,6 +447,7 @@ tiplhip_ctx* aux_context(tiplhip_ctx* ctx)
         tiplhip_ctx* a = nullptr;
         if (tiplhip_ctx_create(ctx->device, &a) != TIPLHIP_OK)
             throw Error(TIPLHIP_ERR_DEVICE, tiplhip_last_error());
+        a->cdm_batch = ctx->cdm_batch;
         ctx->aux = a;
     }
     return ctx->aux;

@@ -22,10 +22,18 @@
 #include "field.hpp"
 #include "host_wait.hpp"
 #include "internal.hpp"
+#include "step_gate.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <exception>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <vector>
 
 namespace tiplhip {
@@ -34,11 +42,11 @@ namespace {
 // ---------------------------------------------------------------------------
 // compose_displacement: Js(x) = Is(x + d(x)); exact copy where d(x) == 0.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void compose_kernel(const uint8_t* __restrict__ Is,
-                                                      const float* __restrict__ d, FieldGeo g,
-                                                      uint8_t* __restrict__ Js)
+__device__ __forceinline__ void compose_body(const uint8_t* __restrict__ Is,
+                                             const float* __restrict__ d, const FieldGeo& g,
+                                             uint8_t* __restrict__ Js, const Block& blk)
 {
-    const Vox v = this_voxel(g);
+    const Vox v = voxel_of(g, blk);
     if (!v.ok) return;
     const float3 dv = ld3(d, v.p);
     if (dv.x == 0.0f && dv.y == 0.0f && dv.z == 0.0f) {
@@ -48,6 +56,13 @@ __global__ __launch_bounds__(256) void compose_kernel(const uint8_t* __restrict_
     const __amdgpu_buffer_rsrc_t r = make_rsrc(Is, g.n);
     VolGeo vg{g.w, g.h, g.d, g.wh, g.wm1, g.hm1, g.dm1};
     Js[v.p] = uint8_t(sample_u8(r, vg, float(v.x) + dv.x, float(v.y) + dv.y, float(v.z) + dv.z));
+}
+
+__global__ __launch_bounds__(256) void compose_kernel(const uint8_t* __restrict__ Is,
+                                                      const float* __restrict__ d, FieldGeo g,
+                                                      uint8_t* __restrict__ Js)
+{
+    compose_body(Is, d, g, Js, logical_block());
 }
 
 // ---------------------------------------------------------------------------
@@ -220,17 +235,17 @@ __device__ inline void regress_and_store(const uint8_t* __restrict__ Js, const F
 
 constexpr int kStage = (kPX * kPY + kTX * kTY - 1) / (kTX * kTY);  // staged bytes per thread
 
-__global__ __launch_bounds__(256) void gradient_r_box_kernel(const uint8_t* __restrict__ Js,
-                                                             const uint8_t* __restrict__ It,
-                                                             FieldGeo g, int accumulate,
-                                                             float* __restrict__ nd,
-                                                             double* __restrict__ partials, int zc)
+__device__ __forceinline__ void gradient_r_box_body(const uint8_t* __restrict__ Js,
+                                                    const uint8_t* __restrict__ It,
+                                                    const FieldGeo& g, int accumulate,
+                                                    float* __restrict__ nd,
+                                                    double* __restrict__ partials, int zc,
+                                                    const Block& blk)
 {
     __shared__ uint8_t pj[kPY][kPX], pi[kPY][kPX];
     __shared__ Sums5 xs[kPY][kTX];
     const int tx = int(threadIdx.x), ty = int(threadIdx.y);
     const int tid = ty * kTX + tx;
-    const Block blk = logical_block();
     const int x0 = blk.x * kTX, y0 = blk.y * kTY;
     const int z0 = blk.z * zc, z1 = min(z0 + zc, g.d);
     Sums5 ring[5];
@@ -350,7 +365,16 @@ __global__ __launch_bounds__(256) void gradient_r_box_kernel(const uint8_t* __re
     double c = wave_sum_f64(cost_sum);
     if ((tid & 63) == 0) part[tid >> 6] = c;
     __syncthreads();
-    if (tid == 0) partials[logical_block().lin] = (part[0] + part[1]) + (part[2] + part[3]);
+    if (tid == 0) partials[blk.lin] = (part[0] + part[1]) + (part[2] + part[3]);
+}
+
+__global__ __launch_bounds__(256) void gradient_r_box_kernel(const uint8_t* __restrict__ Js,
+                                                             const uint8_t* __restrict__ It,
+                                                             FieldGeo g, int accumulate,
+                                                             float* __restrict__ nd,
+                                                             double* __restrict__ partials, int zc)
+{
+    gradient_r_box_body(Js, It, g, accumulate, nd, partials, zc, logical_block());
 }
 
 // Planes per block of the box-sum gradient: kZC for large volumes; on
@@ -370,10 +394,9 @@ dim3 box_grid(const FieldGeo& g)
 }
 
 // Sums the block partials in a fixed order and stores float(sum / n).
-__global__ __launch_bounds__(1024) void cost_reduce_kernel(const double* __restrict__ partials,
-                                                           uint32_t count, double n,
-                                                           float* __restrict__ out,
-                                                           float* __restrict__ host_out)
+__device__ __forceinline__ void cost_reduce_body(const double* __restrict__ partials,
+                                                 uint32_t count, double n, float* __restrict__ out,
+                                                 float* __restrict__ host_out)
 {
     double acc = 0.0;
     for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) acc += partials[i];
@@ -389,10 +412,18 @@ __global__ __launch_bounds__(1024) void cost_reduce_kernel(const double* __restr
     }
 }
 
+__global__ __launch_bounds__(1024) void cost_reduce_kernel(const double* __restrict__ partials,
+                                                           uint32_t count, double n,
+                                                           float* __restrict__ out,
+                                                           float* __restrict__ host_out)
+{
+    cost_reduce_body(partials, count, n, out, host_out);
+}
+
 // Max of the per-block maxima (non-negative floats), as float bits.
-__global__ __launch_bounds__(1024) void max_reduce_kernel(const float* __restrict__ parts,
-                                                          uint32_t count,
-                                                          unsigned int* __restrict__ out)
+__device__ __forceinline__ void max_reduce_body(const float* __restrict__ parts, uint32_t count,
+                                                unsigned int* __restrict__ out,
+                                                unsigned int* __restrict__ host_out)
 {
     float m = 0.0f;
     for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) m = fmaxf(m, parts[i]);
@@ -404,7 +435,15 @@ __global__ __launch_bounds__(1024) void max_reduce_kernel(const float* __restric
         float r = 0.0f;
         for (uint32_t w = 0; w < (blockDim.x >> 6); ++w) r = fmaxf(r, part[w]);
         out[0] = __float_as_uint(r);
+        if (host_out) host_out[0] = __float_as_uint(r);
     }
+}
+
+__global__ __launch_bounds__(1024) void max_reduce_kernel(const float* __restrict__ parts,
+                                                          uint32_t count,
+                                                          unsigned int* __restrict__ out)
+{
+    max_reduce_body(parts, count, out, nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -612,16 +651,13 @@ __device__ inline void ring_offset(int r, int& i, int& j)
 }
 
 template <bool FIRST, bool MAXLEN>
-__global__ __launch_bounds__(kQX * kQY) void poisson2_kernel(const float* __restrict__ nd,
-                                                             const float* __restrict__ a,
-                                                             FieldGeo g, int zq,
-                                                             float* __restrict__ b,
-                                                             float* __restrict__ maxparts)
+__device__ __forceinline__ void poisson2_body(const float* __restrict__ nd, const float* __restrict__ a,
+                                              const FieldGeo& g, int zq, float* __restrict__ b,
+                                              float* __restrict__ maxparts, const Block& blk)
 {
     __shared__ float3 S[3][kS2Y][kS2X];
     __shared__ float3 T[3][kS1Y][kS1X];
     const int tx = int(threadIdx.x), ty = int(threadIdx.y), tid = ty * kQX + tx;
-    const Block blk = logical_block();
     const int x0 = blk.x * kQX, y0 = blk.y * kQY;
     const int z0 = blk.z * zq, z1 = min(z0 + zq, g.d);
     const long long n = g.n, w = g.w, wh = g.wh;
@@ -747,6 +783,16 @@ __global__ __launch_bounds__(kQX * kQY) void poisson2_kernel(const float* __rest
     }
 }
 
+template <bool FIRST, bool MAXLEN>
+__global__ __launch_bounds__(kQX * kQY) void poisson2_kernel(const float* __restrict__ nd,
+                                                             const float* __restrict__ a,
+                                                             FieldGeo g, int zq,
+                                                             float* __restrict__ b,
+                                                             float* __restrict__ maxparts)
+{
+    poisson2_body<FIRST, MAXLEN>(nd, a, g, zq, b, maxparts, logical_block());
+}
+
 // ---------------------------------------------------------------------------
 // new_d *= k, then accumulate_displacement(cur, new_d) in place:
 // new(x) = m(x + new(x)) - x with m(y) = cur(y) + y by trilinear
@@ -866,15 +912,16 @@ __device__ inline float3 accumulated_bf(__amdgpu_buffer_rsrc_t rcur, __amdgpu_bu
 // the two passes).  Measured at 182x218x182, whole iterations: 1558
 // iterations/s against 1546 with the two passes, 1490 with branching
 // evaluations and 1538 with two planes a step (four evaluations, 125
-// VGPRs, half the occupancy).  out must not alias cur or nd.
+// VGPRs, half the occupancy).  out must not alias cur or nd.  With keep,
+// cur is also copied into keep (the best field taking the running one
+// before the step, as the reference's loop does), keep aliasing neither.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kQX * kQY) void accumulate_smooth_kernel(
+__device__ __forceinline__ void accumulate_smooth_body(
     const float* __restrict__ cur, const float* __restrict__ nd, float k, float w_6, float w_1,
-    FieldGeo g, int zq, float* __restrict__ out)
+    const FieldGeo& g, int zq, float* __restrict__ out, float* __restrict__ keep, const Block& blk)
 {
     __shared__ float3 A[4][kS1Y][kS1X];
     const int tx = int(threadIdx.x), ty = int(threadIdx.y), tid = ty * kQX + tx;
-    const Block blk = logical_block();
     const int x0 = blk.x * kQX, y0 = blk.y * kQY;
     const int z0 = blk.z * zq, z1 = min(z0 + zq, g.d);
     const long long n = g.n, w = g.w, wh = g.wh;
@@ -919,8 +966,9 @@ __global__ __launch_bounds__(kQX * kQY) void accumulate_smooth_kernel(
             v.z = v.z + t[e].z;
         }
         const float3 c = ac[cj][ci];
-        st3(out, uint32_t((long long)m * wh + base + io),
-            make_float3(c.x * w_1 + v.x * w_6, c.y * w_1 + v.y * w_6, c.z * w_1 + v.z * w_6));
+        const uint32_t p = uint32_t((long long)m * wh + base + io);
+        st3(out, p, make_float3(c.x * w_1 + v.x * w_6, c.y * w_1 + v.y * w_6, c.z * w_1 + v.z * w_6));
+        if (keep) st3(keep, p, ld3(cur, p));
     };
     fill(z0 - 1);
     fill(z0);
@@ -930,6 +978,116 @@ __global__ __launch_bounds__(kQX * kQY) void accumulate_smooth_kernel(
         smooth(m);
         __syncthreads();
     }
+}
+
+__global__ __launch_bounds__(kQX * kQY) void accumulate_smooth_kernel(
+    const float* __restrict__ cur, const float* __restrict__ nd, float k, float w_6, float w_1,
+    FieldGeo g, int zq, float* __restrict__ out)
+{
+    accumulate_smooth_body(cur, nd, k, w_6, w_1, g, zq, out, nullptr, logical_block());
+}
+
+// ---------------------------------------------------------------------------
+// Multi-field launches.  One launch runs one step of the CDM iteration for
+// up to kMaxSegs fields -- the concurrent registrations of a server, each
+// with its own shape and buffers.  The launch's workgroups are the fields'
+// grids one after another (field s from begin[s]); each workgroup finds its
+// field and its logical block within the field's grid, and runs the same
+// body as the single-field kernel, so every field's results, its block
+// partials included, are those of a launch of its own.  The table travels
+// as a kernel argument (no upload).
+// ---------------------------------------------------------------------------
+constexpr int kMaxSegs = 8;
+
+struct CdmStep {
+    uint32_t nseg = 0, total = 0;
+    uint32_t begin[kMaxSegs];
+    uint32_t gx[kMaxSegs], gy[kMaxSegs];
+    int zq[kMaxSegs];
+    FieldGeo g[kMaxSegs];
+    const uint8_t* Is[kMaxSegs];
+    const uint8_t* It[kMaxSegs];
+    uint8_t* Js[kMaxSegs];
+    const float* cur[kMaxSegs];
+    float* nd[kMaxSegs];
+    const float* src[kMaxSegs];  // Jacobi: the sweeps' input
+    float* dst[kMaxSegs];        // Jacobi: output; accumulate+smooth: the new field
+    float* keep[kMaxSegs];       // accumulate+smooth: receives cur when set
+    double* partials[kMaxSegs];
+    float* maxparts[kMaxSegs];
+    float* cost[kMaxSegs];
+    float* host_cost[kMaxSegs];
+    unsigned* maxlen[kMaxSegs];
+    unsigned* host_maxlen[kMaxSegs];
+    uint32_t count[kMaxSegs];    // reductions: partials of the field
+    double n[kMaxSegs];
+    float k[kMaxSegs], w_6[kMaxSegs], w_1[kMaxSegs];
+};
+static_assert(sizeof(CdmStep) <= 4096, "CdmStep travels as a kernel argument (4 KB at most)");
+
+// This workgroup's field s and logical block within the field's grid
+// (hardware order remapped so that each XCD takes a contiguous range).
+__device__ inline Block seg_block(const CdmStep& st, int& s)
+{
+    const uint32_t L = xcd_remap(blockIdx.x, st.total);
+    int k = 0;
+#pragma unroll
+    for (int i = 1; i < kMaxSegs; ++i)
+        if (uint32_t(i) < st.nseg && st.begin[i] <= L) k = i;
+    s = k;
+    const uint32_t local = L - st.begin[k];
+    const uint32_t gx = st.gx[k], gy = st.gy[k];
+    Block b;
+    b.x = int(local % gx);
+    b.y = int((local / gx) % gy);
+    b.z = int(local / (gx * gy));
+    b.lin = local;
+    return b;
+}
+
+__global__ __launch_bounds__(256) void compose_seg_kernel(const CdmStep st)
+{
+    int s;
+    const Block b = seg_block(st, s);
+    compose_body(st.Is[s], st.cur[s], st.g[s], st.Js[s], b);
+}
+
+__global__ __launch_bounds__(256) void gradient_seg_kernel(const CdmStep st)
+{
+    int s;
+    const Block b = seg_block(st, s);
+    gradient_r_box_body(st.Js[s], st.It[s], st.g[s], 0, st.nd[s], st.partials[s], st.zq[s], b);
+}
+
+// one workgroup per field
+__global__ __launch_bounds__(1024) void cost_reduce_seg_kernel(const CdmStep st)
+{
+    const int s = int(blockIdx.x);
+    cost_reduce_body(st.partials[s], st.count[s], st.n[s], st.cost[s], st.host_cost[s]);
+}
+
+template <bool FIRST, bool MAXLEN>
+__global__ __launch_bounds__(kQX * kQY) void poisson2_seg_kernel(const CdmStep st)
+{
+    int s;
+    const Block b = seg_block(st, s);
+    poisson2_body<FIRST, MAXLEN>(st.nd[s], st.src[s], st.g[s], st.zq[s], st.dst[s], st.maxparts[s], b);
+}
+
+// one workgroup per field; fields without a max_len slot are skipped
+__global__ __launch_bounds__(1024) void max_reduce_seg_kernel(const CdmStep st)
+{
+    const int s = int(blockIdx.x);
+    if (!st.maxlen[s]) return;
+    max_reduce_body(st.maxparts[s], st.count[s], st.maxlen[s], st.host_maxlen[s]);
+}
+
+__global__ __launch_bounds__(kQX * kQY) void accumulate_smooth_seg_kernel(const CdmStep st)
+{
+    int s;
+    const Block b = seg_block(st, s);
+    accumulate_smooth_body(st.cur[s], st.nd[s], st.k[s], st.w_6[s], st.w_1[s], st.g[s], st.zq[s],
+                           st.dst[s], st.keep[s], b);
 }
 
 // cdm_smooth: dd = d*(1-s) + (s/6)*(neighbours +1, -1, +w, -w, +wh, -wh by
@@ -1175,6 +1333,473 @@ struct CdmRun {
 
 bool is_terminated(const CdmRun& r) { return r.terminated && *r.terminated; }
 
+// ---------------------------------------------------------------------------
+// Batched CDM iterations (the serving loop).  A registration whose context
+// is attached to a CdmBatcher hands the device work of each iteration of its
+// cdm() level loop to the batcher as one step; the batcher launches the
+// steps of up to kMaxSegs registrations together, one *_seg launch per
+// kernel, on its own stream.  Each registration keeps its own host control
+// (cost history, cdm_improved, smoothing schedule) and each field's results
+// are those of launches of its own, so a batched cdm() equals a plain one
+// bit for bit.
+// ---------------------------------------------------------------------------
+
+// One registration's level loop as the batcher sees it.  The owner sets
+// the step fields before CdmBatcher::run; the launch reads them under the
+// gate's lock.
+struct CdmJob : GateSlot {
+    FieldGeo g{};
+    dim3 tile, box, p2;       // grids: compose, gradient, Jacobi and accumulate+smooth
+    int zq_box = 0, zq_p2 = 0;
+    const uint8_t* It = nullptr;
+    const uint8_t* Is = nullptr;
+    uint8_t* Js = nullptr;
+    float* nd = nullptr;
+    float* best = nullptr;
+    float* cur = nullptr;     // the running field the iteration body reads
+    float* pa = nullptr;      // Jacobi buffers: the solved new_d ends in pa
+    float* pb = nullptr;
+    double* partials = nullptr;
+    float* maxparts = nullptr;
+    float* d_cost = nullptr;
+    unsigned* d_maxlen = nullptr;
+    float* h_cost = nullptr;  // coherent host slots (device addresses)
+    unsigned* h_maxlen = nullptr;
+    hipEvent_t ready = nullptr;  // on the owner's stream: the level's inputs are written
+    hipEvent_t done = nullptr;   // on the batch stream: the job's last step
+    bool waited = false;         // the lane's stream has waited for `ready`
+    int lane = 0;                // the batcher lane the job joined
+    // the step: accumulate+smooth from s_src into s_out (s_keep <- s_src when
+    // set) or plain copies, then the iteration body on cur
+    bool smooth = false;
+    const float* s_src = nullptr;
+    const float* s_nd = nullptr;
+    float* s_out = nullptr;
+    float* s_keep = nullptr;
+    float k = 0.0f, w_6 = 0.0f, w_1 = 0.0f;
+    bool copy_best = false;   // without smoothing: best <- cur, then
+    bool restore = false;     // cur <- best
+    bool need_theta = false;  // the last Jacobi launch forms max_len
+};
+
+// Fills the workgroup ranges of a multi-field launch from one grid per field.
+void set_grids(CdmStep& st, CdmJob* const* J, int n, dim3 CdmJob::*grid)
+{
+    uint32_t total = 0;
+    for (int s = 0; s < n; ++s) {
+        const dim3 d = J[s]->*grid;
+        st.begin[s] = total;
+        st.gx[s] = d.x;
+        st.gy[s] = d.y;
+        const uint64_t t = uint64_t(total) + uint64_t(d.x) * d.y * d.z;
+        TIPLHIP_REQUIRE(t < (uint64_t(1) << 31), "batched CDM launch too large");
+        total = uint32_t(t);
+    }
+    st.total = total;
+}
+
+// Queues one step of every job in J (at most kMaxSegs) on `stream`, in the
+// order of the reference's loop body: the previous iteration's update of the
+// running field (reg/cdm.hpp:447-469), then compose_displacement, the
+// gradient and cost, and cdm_solve_poisson's 12 sweeps (reg/cdm.hpp:420-462),
+// exactly as cdm_level queues them for one field.
+void launch_cdm_steps(hipStream_t stream, CdmJob* const* J, int n)
+{
+    TIPLHIP_REQUIRE(n >= 1 && n <= kMaxSegs, "1 to 8 fields per batched CDM step");
+    for (int s = 0; s < n; ++s) {
+        CdmJob& j = *J[s];
+        if (!j.waited) {
+            TIPLHIP_CHECK(hipStreamWaitEvent(stream, j.ready, 0));
+            j.waited = true;
+        }
+        if (j.smooth) continue;
+        const size_t fb = sizeof(float) * 3 * j.g.n;
+        if (j.copy_best)
+            TIPLHIP_CHECK(hipMemcpyAsync(j.best, j.cur, fb, hipMemcpyDeviceToDevice, stream));
+        if (j.restore)
+            TIPLHIP_CHECK(hipMemcpyAsync(j.cur, j.best, fb, hipMemcpyDeviceToDevice, stream));
+    }
+    {
+        CdmStep st;
+        CdmJob* sm[kMaxSegs];
+        int m = 0;
+        for (int s = 0; s < n; ++s)
+            if (J[s]->smooth) sm[m++] = J[s];
+        if (m) {
+            st.nseg = uint32_t(m);
+            set_grids(st, sm, m, &CdmJob::p2);
+            for (int s = 0; s < m; ++s) {
+                const CdmJob& j = *sm[s];
+                st.g[s] = j.g;
+                st.zq[s] = j.zq_p2;
+                st.cur[s] = j.s_src;
+                st.nd[s] = const_cast<float*>(j.s_nd);
+                st.dst[s] = j.s_out;
+                st.keep[s] = j.s_keep;
+                st.k[s] = j.k;
+                st.w_6[s] = j.w_6;
+                st.w_1[s] = j.w_1;
+            }
+            hipLaunchKernelGGL(accumulate_smooth_seg_kernel, dim3(st.total), dim3(kQX, kQY), 0, stream, st);
+            TIPLHIP_CHECK(hipGetLastError());
+        }
+    }
+    CdmStep st;
+    st.nseg = uint32_t(n);
+    bool theta = false;
+    for (int s = 0; s < n; ++s) {
+        const CdmJob& j = *J[s];
+        st.g[s] = j.g;
+        st.Is[s] = j.Is;
+        st.It[s] = j.It;
+        st.Js[s] = j.Js;
+        st.cur[s] = j.cur;
+        st.nd[s] = j.nd;
+        st.partials[s] = j.partials;
+        st.maxparts[s] = j.maxparts;
+        st.cost[s] = j.d_cost;
+        st.host_cost[s] = j.h_cost;
+        st.maxlen[s] = j.need_theta ? j.d_maxlen : nullptr;
+        st.host_maxlen[s] = j.h_maxlen;
+        st.n[s] = double(j.g.n);
+        theta = theta || j.need_theta;
+    }
+    set_grids(st, J, n, &CdmJob::tile);
+    hipLaunchKernelGGL(compose_seg_kernel, dim3(st.total), dim3(kTX, kTY), 0, stream, st);
+    TIPLHIP_CHECK(hipGetLastError());
+    set_grids(st, J, n, &CdmJob::box);
+    for (int s = 0; s < n; ++s) {
+        st.zq[s] = J[s]->zq_box;
+        st.count[s] = J[s]->box.x * J[s]->box.y * J[s]->box.z;
+    }
+    hipLaunchKernelGGL(gradient_seg_kernel, dim3(st.total), dim3(kTX, kTY), 0, stream, st);
+    TIPLHIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(cost_reduce_seg_kernel, dim3(st.nseg), dim3(1024), 0, stream, st);
+    TIPLHIP_CHECK(hipGetLastError());
+    set_grids(st, J, n, &CdmJob::p2);
+    for (int s = 0; s < n; ++s) {
+        st.zq[s] = J[s]->zq_p2;
+        st.count[s] = J[s]->p2.x * J[s]->p2.y * J[s]->p2.z;
+    }
+    // 12 sweeps, two per launch: launch i reads pa (i even) or pb and writes
+    // the other, so the solution ends in pa
+    for (int i = 0; i < 6; ++i) {
+        for (int s = 0; s < n; ++s) {
+            st.src[s] = (i & 1) ? J[s]->pb : J[s]->pa;
+            st.dst[s] = (i & 1) ? J[s]->pa : J[s]->pb;
+        }
+        const dim3 grid(st.total), block(kQX, kQY);
+        if (i == 0)
+            hipLaunchKernelGGL((poisson2_seg_kernel<true, false>), grid, block, 0, stream, st);
+        else if (i == 5 && theta)
+            hipLaunchKernelGGL((poisson2_seg_kernel<false, true>), grid, block, 0, stream, st);
+        else
+            hipLaunchKernelGGL((poisson2_seg_kernel<false, false>), grid, block, 0, stream, st);
+        TIPLHIP_CHECK(hipGetLastError());
+    }
+    if (theta) {
+        hipLaunchKernelGGL(max_reduce_seg_kernel, dim3(st.nseg), dim3(1024), 0, stream, st);
+        TIPLHIP_CHECK(hipGetLastError());
+    }
+}
+
+}  // namespace
+
+// The batch executor of one device: `lanes` streams, each with a StepGate
+// (at most kMaxSegs fields a batch, kGatherUs); a job keeps the lane it
+// joined (the one with the fewest members), whose stream orders its steps.
+// Levels above max_voxels run unbatched on their owner's stream.  Measured
+// on 32 uneven affine+CDM pairs served on 16 slots (pairs/s, two runs
+// each): unbatched 57.6-61.2; 1 lane (up to 8 fields a step, 5.5 on
+// average) 56.1-58.0 -- every registration held to the pace of the
+// largest field in its batch; 2 lanes 62.3-64.3; 3 lanes 64.6-65.6;
+// 4 lanes 65.6-68.8; 6 lanes 62.3; 8 lanes 61.6-61.9; 16 lanes (one field
+// a step) 58.9-61.3.  Leaving levels above 256K or 512K voxels unbatched
+// did not help (63.8-67.6).
+class CdmBatcher {
+public:
+    static constexpr int kGatherUs = 300;
+    static constexpr int kDefaultLanes = 4;
+    static constexpr size_t kDefaultMaxVoxels = size_t(1) << 25;
+
+    CdmBatcher(int device, int lanes, size_t max_voxels) : device_(device), max_voxels_(max_voxels)
+    {
+        TIPLHIP_REQUIRE(lanes >= 1 && lanes <= 16, "1 to 16 CDM batch lanes");
+        DeviceGuard dg(device);
+        lanes_.resize(size_t(lanes));
+        for (Lane& l : lanes_) {
+            TIPLHIP_CHECK(hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking));
+            hipStream_t st = l.stream;
+            l.gate.reset(new StepGate<CdmJob>(kMaxSegs, std::chrono::microseconds(kGatherUs),
+                                              [this, st](CdmJob* const* J, int n) {
+                                                  DeviceGuard g(device_);
+                                                  launch_cdm_steps(st, J, n);
+                                              }));
+        }
+    }
+    ~CdmBatcher()
+    {
+        int prev = -1;
+        (void)hipGetDevice(&prev);
+        (void)hipSetDevice(device_);
+        for (Lane& l : lanes_) {
+            if (!l.stream) continue;
+            (void)hipStreamSynchronize(l.stream);
+            (void)hipStreamDestroy(l.stream);
+        }
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+    CdmBatcher(const CdmBatcher&) = delete;
+    CdmBatcher& operator=(const CdmBatcher&) = delete;
+
+    int device() const { return device_; }
+    size_t max_voxels() const { return max_voxels_; }
+    hipStream_t stream(int lane) const { return lanes_[size_t(lane)].stream; }
+    // Makes j a member of the least occupied lane; returns the lane.
+    int join(CdmJob& j)
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            size_t best = 0;
+            for (size_t i = 1; i < lanes_.size(); ++i)
+                if (lanes_[i].members < lanes_[best].members) best = i;
+            ++lanes_[best].members;
+            j.lane = int(best);
+        }
+        lanes_[size_t(j.lane)].gate->join();
+        return j.lane;
+    }
+    // After the job's last step: the owner's stream waits for the lane's
+    // stream, so the owner's later work and frees follow every step.
+    void leave(CdmJob& j, hipStream_t owner)
+    {
+        Lane& l = lanes_[size_t(j.lane)];
+        l.gate->leave([&] {
+            TIPLHIP_CHECK(hipEventRecord(j.done, l.stream));
+            TIPLHIP_CHECK(hipStreamWaitEvent(owner, j.done, 0));
+        });
+        std::lock_guard<std::mutex> lk(mu_);
+        --l.members;
+    }
+    // Queues j's step; returns once it is launched (rethrows its launch error).
+    // A failed launch fails every job of its batch: their steps may be partly
+    // queued.
+    void run(CdmJob& j) { lanes_[size_t(j.lane)].gate->run(j); }
+    int64_t launches() const
+    {
+        int64_t n = 0;
+        for (const Lane& l : lanes_) n += l.gate->batches();
+        return n;
+    }
+    int64_t fields() const
+    {
+        int64_t n = 0;
+        for (const Lane& l : lanes_) n += l.gate->steps();
+        return n;
+    }
+
+private:
+    struct Lane {
+        hipStream_t stream = nullptr;
+        std::unique_ptr<StepGate<CdmJob>> gate;
+        int members = 0;
+    };
+    int device_;
+    size_t max_voxels_;
+    std::mutex mu_;
+    std::vector<Lane> lanes_;
+};
+
+void cdm_batcher_counts(const CdmBatcher& b, int64_t* steps, int64_t* fields)
+{
+    *steps = b.launches();
+    *fields = b.fields();
+}
+
+bool cdm_batching_enabled()
+{
+    const char* e = std::getenv("TIPLHIP_SERVER_CDM_BATCH");
+    return !(e && e[0] == '0');
+}
+
+// TIPLHIP_CDM_LANES and TIPLHIP_CDM_BATCH_VOXELS, read when a device's
+// batcher is created, override the lane count and the largest batched level.
+std::shared_ptr<CdmBatcher> shared_cdm_batcher(int device)
+{
+    static std::mutex mu;
+    static std::map<int, std::weak_ptr<CdmBatcher>> live;
+    std::lock_guard<std::mutex> lk(mu);
+    std::shared_ptr<CdmBatcher> b = live[device].lock();
+    if (!b) {
+        int lanes = CdmBatcher::kDefaultLanes;
+        size_t max_voxels = CdmBatcher::kDefaultMaxVoxels;
+        if (const char* e = std::getenv("TIPLHIP_CDM_LANES")) lanes = std::atoi(e);
+        if (const char* e = std::getenv("TIPLHIP_CDM_BATCH_VOXELS")) max_voxels = size_t(std::atoll(e));
+        b = std::make_shared<CdmBatcher>(device, lanes, max_voxels);
+        live[device] = b;
+    }
+    return b;
+}
+
+namespace {
+
+// A job's membership of the batcher for one level: events created, `ready`
+// recorded on the owner's stream and the job counted active; leave() (or
+// the destructor) ends it.
+struct CdmMembership {
+    CdmBatcher& b;
+    CdmJob& j;
+    hipStream_t owner;
+    bool joined = false;
+    CdmMembership(CdmBatcher& batcher, CdmJob& job, hipStream_t s) : b(batcher), j(job), owner(s)
+    {
+        TIPLHIP_CHECK(hipEventCreateWithFlags(&j.ready, hipEventDisableTiming));
+        TIPLHIP_CHECK(hipEventCreateWithFlags(&j.done, hipEventDisableTiming));
+        TIPLHIP_CHECK(hipEventRecord(j.ready, owner));
+        b.join(j);
+        joined = true;
+    }
+    void leave()
+    {
+        if (!joined) return;
+        joined = false;
+        b.leave(j, owner);
+    }
+    ~CdmMembership()
+    {
+        try {
+            leave();
+        } catch (...) {
+            // the lane is drained before the owner's buffers are freed
+            (void)hipStreamSynchronize(b.stream(j.lane));
+        }
+        if (j.ready) (void)hipEventDestroy(j.ready);
+        if (j.done) (void)hipEventDestroy(j.done);
+    }
+};
+
+// cdm_level's iteration loop for one modality with the 'r' gradient, its
+// device work run as batched steps.  The host control is cdm_level's, line
+// for line; a decision's copies and the field update are queued with the
+// next iteration's body (fused into accumulate+smooth where there is one).
+bool cdm_iterate_batched(CdmRun& run, const uint8_t* It, const uint8_t* Is, const FieldGeo& g,
+                         float* best_d)
+{
+    tiplhip_ctx* ctx = run.ctx;
+    CdmBatcher& batch = *ctx->cdm_batch;
+    TIPLHIP_REQUIRE(batch.device() == ctx->device, "CDM batcher of another device");
+    const size_t fb = sizeof(float) * 3 * g.n;
+    DevMem f0(ctx, fb), f1(ctx, fb), f2(ctx, fb), nd(ctx, fb), Js(ctx, g.n);
+    const uint32_t nblocks = block_count(g);
+    DevMem partials(ctx, sizeof(double) * nblocks), maxparts(ctx, sizeof(float) * nblocks);
+    DevMem scal(ctx, sizeof(float) * 16 + sizeof(unsigned int));
+    ctx->h_cdm.reserve(sizeof(float) * 16);
+    uint32_t* slots = ctx->h_cdm.as<uint32_t>();
+
+    CdmJob job;
+    job.g = g;
+    job.tile = tile_grid(g);
+    job.box = box_grid(g);
+    job.zq_box = box_zc(g);
+    job.zq_p2 = poisson2_zq(g);
+    job.p2 = poisson2_grid(g, job.zq_p2);
+    job.It = It;
+    job.Is = Is;
+    job.Js = Js.as<uint8_t>();
+    job.nd = nd.as<float>();
+    job.best = best_d;
+    job.cur = f0.as<float>();
+    job.pa = f1.as<float>();
+    job.pb = f2.as<float>();
+    job.partials = partials.as<double>();
+    job.maxparts = maxparts.as<float>();
+    job.d_cost = scal.as<float>();
+    job.d_maxlen = reinterpret_cast<unsigned int*>(scal.as<float>() + 16);
+    job.h_cost = static_cast<float*>(ctx->h_cdm.dev);
+    job.h_maxlen = static_cast<unsigned int*>(ctx->h_cdm.dev) + 1;
+    copy_d2d(ctx, job.cur, best_d, fb);
+
+    CdmMembership member(batch, job, ctx->stream);
+    // kernels of this job may be queued on either stream when a step throws
+    const hipStream_t lane = batch.stream(job.lane);
+    auto drain = [ctx, lane] {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamSynchronize(lane);
+    };
+    DrainOnUnwind<decltype(drain)> unwind(drain);
+    auto sync_batch = [lane] { TIPLHIP_CHECK(hipStreamSynchronize(lane)); };
+
+    float theta = 0.0f;
+    std::vector<float> cost;
+    float best_cost = 3.402823466e+38f;
+    float cur_smoothing = 0.8f;
+    // the last decision, carried out with the next step
+    bool copy_best = false, restore = false, update = false;
+    float k = 0.0f;
+    for (int index = 0; index < run.param.max_iterations; ++index) {
+        if (is_terminated(run)) {
+            run.stopped = true;
+            break;
+        }
+        ++run.iterations;
+        job.smooth = update && cur_smoothing != 0.0f;
+        job.copy_best = job.restore = false;
+        if (job.smooth) {
+            // accumulate+smooth of the running field (best when it was
+            // reset to it; best <- cur fused in) into the spare buffer,
+            // which becomes the running field
+            job.s_src = (restore && !copy_best) ? job.best : job.cur;
+            job.s_keep = copy_best ? job.best : nullptr;
+            job.s_nd = job.pa;
+            job.s_out = job.pb;
+            job.k = k;
+            job.w_6 = cur_smoothing / float(2.0f * 3.0f);
+            job.w_1 = 1.0f - cur_smoothing;
+            float* spare = job.pb;
+            job.pb = job.pa;
+            job.pa = job.cur;
+            job.cur = spare;
+        } else {
+            job.copy_best = copy_best;
+            job.restore = restore;
+        }
+        copy_best = restore = update = false;
+        job.need_theta = theta == 0.0f;
+        slots[0] = kCostPending;
+        slots[1] = kCostPending;
+        batch.run(job);
+        wait_for_slots(static_cast<const volatile uint32_t*>(slots), 1, kCostPending,
+                       std::chrono::microseconds(kCostSpinUs), sync_batch);
+        float sum_cost = 0.0f;
+        std::memcpy(&sum_cost, &slots[0], sizeof(float));
+        if (sum_cost == 0.0f) return false;
+        cost.push_back(sum_cost);
+        if (cost.back() < best_cost) {
+            best_cost = cost.back();
+            if (index) copy_best = true;
+        }
+        if (!cdm_improved(cost)) {
+            if (cur_smoothing <= run.param.smoothing) break;
+            restore = true;
+            cur_smoothing *= 0.5f;
+            cost.clear();
+        }
+        if (job.need_theta) {
+            wait_for_slots(static_cast<const volatile uint32_t*>(slots + 1), 1, kCostPending,
+                           std::chrono::microseconds(kCostSpinUs), sync_batch);
+            std::memcpy(&theta, &slots[1], sizeof(theta));
+        }
+        if (theta == 0.0f) break;
+        k = run.param.speed / theta;
+        update = true;
+    }
+    member.leave();
+    // a decision not yet carried out: only its best <- cur matters now
+    if (copy_best) copy_d2d(ctx, best_d, job.cur, fb);
+    TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return true;
+}
+
 // One cdm() call at shape dim.  best_d is a zeroed device field of vox(dim).
 bool cdm_level(CdmRun& run, const std::vector<const uint8_t*>& It,
                const std::vector<const uint8_t*>& Is, const int* dim, float* best_d,
@@ -1203,6 +1828,9 @@ bool cdm_level(CdmRun& run, const std::vector<const uint8_t*>& It,
         if (resolution > 1.0f) return true;
     }
     if (run.stopped) return true;
+    if (run.ctx->cdm_batch && run.nmod == 1 && run.param.gradient_type[0] != 'd' &&
+        !run.no_early_exit && n <= run.ctx->cdm_batch->max_voxels())
+        return cdm_iterate_batched(run, It[0], Is[0], g, best_d);
 
     // three fields rotate: the running field and the two Jacobi buffers;
     // accumulate+smooth writes the new running field into the Jacobi buffer

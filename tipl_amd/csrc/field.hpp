@@ -68,6 +68,18 @@ __device__ inline Block logical_block()
     return b;
 }
 
+// This thread's voxel of the 64 x 4 tile of block b.
+__device__ inline Vox voxel_of(const FieldGeo& g, const Block& b)
+{
+    Vox v;
+    v.x = b.x * kTX + int(threadIdx.x);
+    v.y = b.y * kTY + int(threadIdx.y);
+    v.z = b.z;
+    v.ok = v.x < g.w && v.y < g.h;
+    v.p = uint32_t(v.z) * g.wh + uint32_t(v.y) * uint32_t(g.w) + uint32_t(v.x);
+    return v;
+}
+
 __device__ inline Vox this_voxel(const FieldGeo& g)
 {
     const Block b = logical_block();

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <stdexcept>
@@ -23,6 +24,8 @@
 #pragma clang fp contract(off)
 
 namespace tiplhip {
+
+class CdmBatcher;
 
 #define TIPLHIP_CHECK(expr)                                                                      \
     do {                                                                                         \
@@ -150,6 +153,9 @@ struct tiplhip_ctx {
     bool use_graphs = false;         // see tiplhip_ctx_set_graphs
     std::map<std::string, hipGraphExec_t> graphs;  // by launch shape
     tiplhip_ctx* aux = nullptr;      // see aux_context
+    // when set, cdm() level loops on this context run as batched steps
+    // (cdm.hip), shared with the other contexts attached to the batcher
+    std::shared_ptr<tiplhip::CdmBatcher> cdm_batch;
     double kernel_ms = 0.0;
     int64_t kernel_launches = 0;
 };
@@ -211,6 +217,16 @@ void read_back(tiplhip_ctx* ctx, const void* dev, void* host, size_t bytes);
 // A second context on ctx's device (own stream), created on first use and
 // destroyed with ctx: the reverse CDM of a registration runs on it.
 tiplhip_ctx* aux_context(tiplhip_ctx* ctx);
+
+// The CDM batch executor of a device (cdm.hip), created on first use and
+// shared while any context holds it.  Contexts given it in cdm_batch run
+// their cdm() level loops (one modality, 'r' gradient) as batched steps.
+std::shared_ptr<CdmBatcher> shared_cdm_batcher(int device);
+// Batched steps the batcher has launched, and fields over them.
+void cdm_batcher_counts(const CdmBatcher& b, int64_t* steps, int64_t* fields);
+// Whether servers batch CDM iterations: unless TIPLHIP_SERVER_CDM_BATCH=0
+// when the server is created.
+bool cdm_batching_enabled();
 
 // filter::gaussian (filter/gaussian.hpp:18-89) of a device float image with
 // `channels` interleaved channels (1 or 3), ping-ponging between `a` and

@@ -15,8 +15,10 @@
 //     free launcher launches everything queued (up to 4096 candidates, one
 //     cost type) as one multi-pair launch -- registrations join and leave
 //     the batch at every optimiser step;
-//   * CDM iterations of different requests run on their workers' streams
-//     and overlap on the device.
+//   * the CDM iterations of all running registrations (both directions)
+//     are launched together: each iteration's device work is one step for
+//     the device's CdmBatcher (cdm.hip), which merges the steps of up to 8
+//     fields into one launch per kernel.
 // Batched costs are bit-identical to single-pair costs and the optimiser is
 // deterministic, so a served result equals tiplhip_register's for the same
 // request.
@@ -200,7 +202,15 @@ void register_pair(tipl::hip::context& ctx, const tiplhip_reg_request& req,
 // (stream) per worker slot and per batch launcher.
 struct HipBackend {
     using Context = tipl::hip::context;
-    static std::unique_ptr<Context> make(int device) { return std::make_unique<Context>(device); }
+    // Every worker context shares the device's CDM batcher, so the cdm()
+    // iterations of all running registrations go out as batched steps
+    // (TIPLHIP_SERVER_CDM_BATCH=0 at server creation turns this off).
+    static std::unique_ptr<Context> make(int device)
+    {
+        auto c = std::make_unique<Context>(device);
+        if (cdm_batching_enabled()) c->get()->cdm_batch = shared_cdm_batcher(device);
+        return c;
+    }
     static void register_pair(Context& ctx, const tiplhip_reg_request& req, const CostEvaluator& eval,
                               tiplhip_reg_result& res)
     {
@@ -224,9 +234,17 @@ struct HipBackend {
 
 struct tiplhip_server {
     tiplhip::ServerCore<tiplhip::HipBackend> core;
+    // the device's CDM batcher (shared with any other server on the device)
+    // and its counts when this server started
+    std::shared_ptr<tiplhip::CdmBatcher> cdm;
+    int64_t cdm_steps0 = 0, cdm_fields0 = 0;
     tiplhip_server(int device, int slots, int launchers, size_t budget)
         : core(device, slots, launchers, budget)
     {
+        if (tiplhip::cdm_batching_enabled()) {
+            cdm = tiplhip::shared_cdm_batcher(device);
+            tiplhip::cdm_batcher_counts(*cdm, &cdm_steps0, &cdm_fields0);
+        }
     }
 };
 
@@ -312,6 +330,12 @@ int tiplhip_server_stats_get(tiplhip_server* s, tiplhip_server_stats* st)
     return guarded([&] {
         TIPLHIP_REQUIRE(s != nullptr && st != nullptr, "null argument");
         s->core.stats(st);
+        st->cdm_steps = st->cdm_fields = 0;
+        if (s->cdm) {
+            tiplhip::cdm_batcher_counts(*s->cdm, &st->cdm_steps, &st->cdm_fields);
+            st->cdm_steps -= s->cdm_steps0;
+            st->cdm_fields -= s->cdm_fields0;
+        }
     });
 }
 
