@@ -310,3 +310,41 @@ def test_full_size_corr_sums(ctx, oracle, tissue_u8):
     for k, T in enumerate(Ts):
         r = oracle.resample(M, F.shape, T, crop=False)
         assert np.array_equal(got[k], oracle.corr_sums(F, r))
+
+
+def _inside_params(K, scale, seed):
+    rng = np.random.default_rng(seed)
+    P = random_params(rng, K, t=1.0, r=0.05, s=0.02, a=0.01)
+    P[:, 6:9] *= scale
+    return P
+
+
+@pytest.mark.parametrize("tshape", [(60, 64, 60), (256, 256, 256), (257, 256, 256)])
+def test_inside_columns_bit_exact(ctx, oracle, tshape):
+    """Correlation waves whose columns all map inside the `to` volume skip the
+    per-sample bounds test and form the cell index in FP32 (up to 2^24
+    voxels); the 256^3 case puts cell indices near 2^24, and 257x256x256
+    (over 2^24) takes the checked path.  Correlation sums, and the joint
+    histograms (256 and 64 bins) of the same transforms, bit-exact against
+    the oracle, for transforms with every column inside and for transforms
+    partly outside."""
+    fshape = (40, 48, 40) if tshape[0] == 60 else (64, 64, 64)
+    scale = 1.0 if tshape[0] == 60 else 3.9
+    A = smooth_volume(fshape, 21)
+    B = (smooth_volume(tshape, 22) if tshape[0] == 60 else
+         np.random.default_rng(23).integers(0, 256, tshape, dtype=np.uint8))
+    va, vb = ctx.volume(A), ctx.volume(B)
+    P = _inside_params(6, scale, 24)
+    P[4, 0:3] = [25.0, -30.0, 12.0]          # partly outside
+    P[5, 0:3] = [0.0, 0.0, 200.0]            # mostly outside
+    Ts = np.stack([_T(oracle, p, fshape, tshape) for p in P])
+    for bins in (256, 64):
+        got = ctx.mi_hist(va, vb, Ts, bins=bins)
+        for k, T in enumerate(Ts):
+            assert np.array_equal(got[k], oracle.mi_hist(A, B, T, bins=bins)), (bins, k)
+    got = ctx.corr_sums(va, vb, Ts)
+    for k, T in enumerate(Ts):
+        r = oracle.resample(B, fshape, T, crop=False)
+        assert np.array_equal(got[k], oracle.corr_sums(A, r)), k
+    vb.free()
+    va.free()

@@ -25,6 +25,7 @@
 #include <chrono>
 #include <cmath>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 namespace tiplhip {
@@ -209,6 +210,24 @@ __device__ inline float4 z_products(const float4* __restrict__ ztab, uint32_t ro
     }
 }
 
+// Whether every sample of this wave's columns lies inside the `to` volume,
+// so the march can skip the per-sample bounds test (sample2_u8_packed_in).
+// For fixed (x, y) each rounded step of ((hx + z*r2) + s) is monotone in z,
+// so a column's positions lie between those of its first and last plane:
+// the column is inside when both ends are.  Also requires w*h*d <= 2^24
+// (small_to) for the exact FP32 cell index.  Wave-uniform.
+template <bool ZT>
+__device__ inline bool column_inside(const BatchGeo& g, const ColumnSums& cs, const float* T,
+                                     const float4* __restrict__ ztab, uint32_t zrow, const Item& m,
+                                     bool small_to)
+{
+    if (!small_to) return false;
+    const float4 a = z_products<ZT>(ztab, zrow, m.z0, T), b = z_products<ZT>(ztab, zrow, m.z1 - 1, T);
+    const bool in = inside_u8(g.to, (cs.hx + a.x) + T[9], (cs.hy + a.y) + T[10], (cs.hz + a.z) + T[11]) &&
+                    inside_u8(g.to, (cs.hx + b.x) + T[9], (cs.hy + b.y) + T[10], (cs.hz + b.z) + T[11]);
+    return __ballot(in) == __ballot(1);
+}
+
 __global__ __launch_bounds__(256) void ztab_kernel(const float* __restrict__ Ts, uint32_t K,
                                                    uint32_t zstride, float4* __restrict__ ztab)
 {
@@ -254,6 +273,8 @@ __global__ __launch_bounds__(256) void corr_sums_kernel(const Segment* __restric
     const uint32_t it0 = chunk * g.items_per_block;
     const uint32_t it1 = min(it0 + g.items_per_block, g.items);
     const uint32_t fwh = uint32_t(g.fw) * uint32_t(g.fh);
+    const bool small_to = g.to.wh * uint64_t(uint32_t(g.to.d)) <= (uint64_t(1) << 24);
+    const float tw = float(g.to.w), twh = float(g.to.wh);
     uint32_t sy = 0, syy = 0, sxy = 0;
     for (uint32_t it = it0 + wave; it < it1; it += nwaves) {
         const Item m = decode(g, it, lane);
@@ -277,23 +298,38 @@ __global__ __launch_bounds__(256) void corr_sums_kernel(const Segment* __restric
             }
             continue;
         }
-        uint32_t z = m.z0;
         // two planes per step: both samples' gathers are in flight together
-        for (; z + 1 < m.z1; z += 2, fidx += 2 * fwh) {
-            const uint32_t xa = load_u8(rf, fidx, 0), xb = load_u8(rf, fidx + fwh, 0);
-            const float4 za = z_products<ZT>(ztab, zrow, z, T), zb = z_products<ZT>(ztab, zrow, z + 1, T);
-            uint32_t ya, yb;
-            sample2_u8_packed(r8, g.to, (cs.hx + za.x) + T[9], (cs.hy + za.y) + T[10],
-                              (cs.hz + za.z) + T[11], (cs.hx + zb.x) + T[9], (cs.hy + zb.y) + T[10],
-                              (cs.hz + zb.z) + T[11], ya, yb);
-            add(xa, ya);
-            add(xb, yb);
-        }
-        if (z < m.z1) {
-            const float4 zp = z_products<ZT>(ztab, zrow, z, T);
-            add(load_u8(rf, fidx, 0), sample_u8_packed(r8, g.to, (cs.hx + zp.x) + T[9],
-                                                       (cs.hy + zp.y) + T[10], (cs.hz + zp.z) + T[11]));
-        }
+        auto march = [&](auto inside_tag) {
+            constexpr bool kIn = decltype(inside_tag)::value;
+            uint32_t z = m.z0;
+            for (; z + 1 < m.z1; z += 2, fidx += 2 * fwh) {
+                const uint32_t xa = load_u8(rf, fidx, 0), xb = load_u8(rf, fidx + fwh, 0);
+                const float4 za = z_products<ZT>(ztab, zrow, z, T), zb = z_products<ZT>(ztab, zrow, z + 1, T);
+                uint32_t ya, yb;
+                if constexpr (kIn)
+                    sample2_u8_packed_in(r8, tw, twh, (cs.hx + za.x) + T[9], (cs.hy + za.y) + T[10],
+                                         (cs.hz + za.z) + T[11], (cs.hx + zb.x) + T[9],
+                                         (cs.hy + zb.y) + T[10], (cs.hz + zb.z) + T[11], ya, yb);
+                else
+                    sample2_u8_packed(r8, g.to, (cs.hx + za.x) + T[9], (cs.hy + za.y) + T[10],
+                                      (cs.hz + za.z) + T[11], (cs.hx + zb.x) + T[9],
+                                      (cs.hy + zb.y) + T[10], (cs.hz + zb.z) + T[11], ya, yb);
+                add(xa, ya);
+                add(xb, yb);
+            }
+            if (z < m.z1) {
+                const float4 zp = z_products<ZT>(ztab, zrow, z, T);
+                add(load_u8(rf, fidx, 0), sample_u8_packed(r8, g.to, (cs.hx + zp.x) + T[9],
+                                                           (cs.hy + zp.y) + T[10], (cs.hz + zp.z) + T[11]));
+            }
+        };
+        // measured at 182x218x182 with 64 candidates: 0.672-0.692 -> 0.657-0.663
+        // ms (candidates up to 10 degrees and 8 voxels off), 0.617-0.633 ->
+        // 0.601-0.616 ms (candidates within 1 degree and 1 voxel)
+        if (column_inside<ZT>(g, cs, T, ztab, zrow, m, small_to))
+            march(std::true_type{});
+        else
+            march(std::false_type{});
     }
     __shared__ unsigned long long red[3][4];
     unsigned long long a = wave_sum_u64(sy), b = wave_sum_u64(syy), c = wave_sum_u64(sxy);
@@ -435,8 +471,11 @@ __global__ __launch_bounds__(MiTraits<BINS>::kThreads) void mi_hist_kernel(
                 atomicAdd(&lds[Tr::word(bin)], Tr::inc(bin));
             }
         };
+        // Two planes per step: both samples' gathers are in flight together.
+        // (The correlation kernel's all-inside march does not pay here: with
+        // the histogram's atomics in the loop it measured 1-2% slower, with
+        // every column inside or not.)
         uint32_t z = m.z0;
-        // two planes per step: both samples' gathers are in flight together
         for (; z + 1 < m.z1; z += 2, fidx += 2 * fwh) {
             const uint32_t fva = load_u8(rf, fidx, 0), fvb = load_u8(rf, fidx + fwh, 0);
             const float4 za = z_products<ZT>(ztab, zrow, z, T), zb = z_products<ZT>(ztab, zrow, z + 1, T);

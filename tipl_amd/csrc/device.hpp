@@ -186,6 +186,57 @@ __device__ inline void sample2_u8_packed(__amdgpu_buffer_rsrc_t r8, const VolGeo
     tb = inb ? vb : 0u;
 }
 
+// The sampler's bounds test alone: 0 <= x < w-1, and y, z alike (the same
+// unsigned compares as sample_u8_packed, with the same -0.0 proviso).
+__device__ inline bool inside_u8(const VolGeo& g, float x, float y, float z)
+{
+    return (__float_as_uint(x) < __float_as_uint(g.wm1)) &
+           (__float_as_uint(y) < __float_as_uint(g.hm1)) &
+           (__float_as_uint(z) < __float_as_uint(g.dm1));
+}
+
+// sample2_u8_packed for two samples the caller knows are inside the volume:
+// no bounds test and no select, and the cell index formed exactly in FP32.
+// For 0 <= x < 2^23, v_fract gives x - floor(x) exactly, so x - fract(x) is
+// floor(x); with w*h*d <= 2^24 every product and sum of
+// (floor(z)*wh + floor(y)*w) + floor(x) is an integer below 2^24, so the two
+// fused multiply-adds are exact and one conversion replaces three
+// conversions and two 24-bit multiplies.  fw, fwh: w and w*h as floats.
+// The interpolation is sample_u8_packed's, operation for operation.
+__device__ inline void sample2_u8_packed_in(__amdgpu_buffer_rsrc_t r8, float fw, float fwh,
+                                            float xa, float ya, float za, float xb, float yb,
+                                            float zb, uint32_t& ta, uint32_t& tb)
+{
+    const float pxa = __builtin_amdgcn_fractf(xa), pya = __builtin_amdgcn_fractf(ya),
+                pza = __builtin_amdgcn_fractf(za);
+    const float pxb = __builtin_amdgcn_fractf(xb), pyb = __builtin_amdgcn_fractf(yb),
+                pzb = __builtin_amdgcn_fractf(zb);
+    const float ia = __builtin_fmaf(za - pza, fwh, __builtin_fmaf(ya - pya, fw, xa - pxa));
+    const float ib = __builtin_fmaf(zb - pzb, fwh, __builtin_fmaf(yb - pyb, fw, xb - pxb));
+    const auto qa = __builtin_amdgcn_raw_buffer_load_b64(r8, int(cvt_u32_sat(ia) * 8u), 0, 0);
+    const auto qb = __builtin_amdgcn_raw_buffer_load_b64(r8, int(cvt_u32_sat(ib) * 8u), 0, 0);
+    auto blend = [](uint32_t lo, uint32_t hi, float p0, float p1, float p2) {
+        const float n0 = 1.0f - p0;
+        const float n1 = 1.0f - p1;
+        const float n2 = 1.0f - p2;
+        const float n0n1 = n0 * n1;
+        const float p0n1 = p0 * n1;
+        const float n0p1 = n0 * p1;
+        const float p0p1 = p0 * p1;
+        float acc = float(lo & 0xffu) * (n0n1 * n2);
+        acc = acc + float((lo >> 8) & 0xffu) * (p0n1 * n2);
+        acc = acc + float((lo >> 16) & 0xffu) * (n0p1 * n2);
+        acc = acc + float(lo >> 24) * (p0p1 * n2);
+        acc = acc + float(hi & 0xffu) * (n0n1 * p2);
+        acc = acc + float((hi >> 8) & 0xffu) * (p0n1 * p2);
+        acc = acc + float((hi >> 16) & 0xffu) * (n0p1 * p2);
+        acc = acc + float(hi >> 24) * (p0p1 * p2);
+        return uint32_t(acc);
+    };
+    ta = blend(qa[0], qa[1], pxa, pya, pza);
+    tb = blend(qb[0], qb[1], pxb, pyb, pzb);
+}
+
 // Hardware dispatches workgroup b of a 1-D grid to XCD b % 8.  This remap
 // gives each XCD a contiguous range of logical workgroups so that the
 // volume region a range touches stays in that XCD's L2.  Bijective on
