@@ -385,19 +385,17 @@ __global__ void corr_finalize_kernel(const Segment* __restrict__ segs, uint32_t 
 //               in different LDS banks: 0.80-0.81 -> 0.73 ms for 64
 //               candidates at 182x218x182.  The same padding of the 256-bin
 //               rows (129 words, 129 KB) was 1.4% slower.
-// With 64 bins adds are wave-aggregated: the lanes whose bin equals the
-// first active lane's bin (the background bin, mostly) are counted with one
-// ballot and one LDS atomic; the other lanes add individually.  With 256
-// bins values spread over many more bins and plain LDS atomics are faster
-// (measured: 1.08 ms against 1.22 ms for 64 candidates at 182x218x182).  At
-// the end the non-zero words are flushed into the candidate's global int32
-// histogram.
+// Every sample is one plain LDS atomic.  Aggregating a wave's lanes that
+// share the first active lane's bin (one ballot, one atomic) paid off while
+// the 64-bin rows were unpadded (0.97 against 1.18 ms); with padded rows
+// plain atomics are faster (0.68-0.69 against 0.72-0.73 ms), and with 256
+// bins they always were (1.08 against 1.22 ms).  At the end the non-zero
+// words are flushed into the candidate's global int32 histogram.
 // ---------------------------------------------------------------------------
 template <int BINS>
 struct MiTraits;
 template <>
 struct MiTraits<256> {
-    static constexpr bool kAggregate = false;  // values spread over many bins
     static constexpr int kWords = 32768;
     static constexpr int kStride = 128;        // words per row in LDS (no padding)
     static constexpr int kLdsWords = 32768;
@@ -408,7 +406,6 @@ struct MiTraits<256> {
 };
 template <>
 struct MiTraits<64> {
-    static constexpr bool kAggregate = true;  // 64 x 64 bins: runs of equal bins are common
     static constexpr int kWords = 4096;
     static constexpr int kStride = 65;         // words per row in LDS: one pad word
     static constexpr int kLdsWords = 64 * 65;
@@ -450,21 +447,10 @@ __global__ __launch_bounds__(MiTraits<BINS>::kThreads) void mi_hist_kernel(
         const ColumnSums cs = column_sums(T, float(m.x), float(m.y));
         uint32_t fidx = m.z0 * fwh + m.y * uint32_t(g.fw) + m.x;
         auto count = [&](uint32_t fv, uint32_t tv) {
-            if constexpr (Tr::kAggregate) {
-                // 64 bins: the byte address of word (fv >> 2) * 65 + (tv >> 2)
-                // (rows padded to 65 words, so the bins of one column in
-                // different rows fall in different LDS banks)
+            if constexpr (BINS == 64) {
+                // the byte address of word (fv >> 2) * 65 + (tv >> 2)
                 const uint32_t a = __umul24(fv & 0xFCu, 65u) + (tv & 0xFCu);
-                const uint32_t a0 = __builtin_amdgcn_readfirstlane(a);
-                const bool same = a == a0;
-                const unsigned long long mk = __ballot(same);
-                uint32_t* const w = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(lds) + a);
-                if (same) {
-                    const uint32_t leader = uint32_t(__ffsll((long long)mk) - 1);
-                    if (lane == leader) atomicAdd(w, uint32_t(__popcll(mk)));
-                } else {
-                    atomicAdd(w, 1u);
-                }
+                atomicAdd(reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(lds) + a), 1u);
             } else {
                 const uint32_t bin = (fv << Tr::kBand) | tv;
                 atomicAdd(&lds[Tr::word(bin)], Tr::inc(bin));
