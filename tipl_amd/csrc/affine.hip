@@ -457,9 +457,8 @@ __global__ __launch_bounds__(MiTraits<BINS>::kThreads) void mi_hist_kernel(
             }
         };
         // Two planes per step: both samples' gathers are in flight together.
-        // (The correlation kernel's all-inside march does not pay here: with
-        // the histogram's atomics in the loop it measured 1-2% slower, with
-        // every column inside or not.)
+        // (The correlation kernel's all-inside march does not pay here: 1-2%
+        // slower with the wave-aggregated atomics, no faster with plain ones.)
         uint32_t z = m.z0;
         for (; z + 1 < m.z1; z += 2, fidx += 2 * fwh) {
             const uint32_t fva = load_u8(rf, fidx, 0), fvb = load_u8(rf, fidx + fwh, 0);
