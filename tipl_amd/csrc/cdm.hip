@@ -1735,6 +1735,7 @@ bool cdm_iterate_batched(CdmRun& run, const uint8_t* It, const uint8_t* Is, cons
     float cur_smoothing = 0.8f;
     // the last decision, carried out with the next step
     bool copy_best = false, restore = false, update = false;
+    bool failed = false;
     float k = 0.0f;
     for (int index = 0; index < run.param.max_iterations; ++index) {
         if (is_terminated(run)) {
@@ -1772,7 +1773,12 @@ bool cdm_iterate_batched(CdmRun& run, const uint8_t* It, const uint8_t* Is, cons
                        std::chrono::microseconds(kCostSpinUs), sync_batch);
         float sum_cost = 0.0f;
         std::memcpy(&sum_cost, &slots[0], sizeof(float));
-        if (sum_cost == 0.0f) return false;
+        // zero cost: cdm() fails (its step's later kernels still run, and
+        // may store into the slots: the exit below waits for them)
+        if (sum_cost == 0.0f) {
+            failed = true;
+            break;
+        }
         cost.push_back(sum_cost);
         if (cost.back() < best_cost) {
             best_cost = cost.back();
@@ -1795,9 +1801,11 @@ bool cdm_iterate_batched(CdmRun& run, const uint8_t* It, const uint8_t* Is, cons
     }
     member.leave();
     // a decision not yet carried out: only its best <- cur matters now
-    if (copy_best) copy_d2d(ctx, best_d, job.cur, fb);
+    if (copy_best && !failed) copy_d2d(ctx, best_d, job.cur, fb);
+    // every step has finished (and stored into the slots) before the
+    // context's next call resets them
     TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream));
-    return true;
+    return !failed;
 }
 
 // One cdm() call at shape dim.  best_d is a zeroed device field of vox(dim).
