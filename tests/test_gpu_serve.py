@@ -267,3 +267,24 @@ def test_cdm_batching_off(ctx, cdm_mix, monkeypatch):
     assert st["cdm_steps"] == 0 and st["cdm_fields"] == 0
     for a, b in zip(got, plain):
         _same(a, b)
+
+
+def test_cdm_batching_mixed_levels_and_fallbacks(ctx, cdm_mix, monkeypatch):
+    """Only levels up to 20000 voxels batched (TIPLHIP_CDM_BATCH_VOXELS), on
+    two lanes: each registration runs its coarse levels as batched steps and
+    its finer levels alone; a request with the 'd' gradient runs unbatched
+    throughout.  Every result equals the plain path's bit for bit."""
+    import tipl_amd
+    monkeypatch.setenv("TIPLHIP_CDM_BATCH_VOXELS", "20000")
+    monkeypatch.setenv("TIPLHIP_CDM_LANES", "2")
+    reqs = [(f, m, {}) for f, m in cdm_mix[:4]] + [(cdm_mix[4][0], cdm_mix[4][1], {"gradient_type": "d"})]
+    plain = [tipl_amd.register(ctx, f, m, nonlinear=True, **kw) for f, m, kw in reqs]
+    with tipl_amd.Server(0, slots=5, hbm_budget=16 << 30) as srv:
+        ts = [srv.submit(f, m, nonlinear=True, **kw) for f, m, kw in reqs]
+        got = [srv.wait(t) for t in ts]
+        st = srv.stats()
+    print(f"batched CDM: {st['cdm_fields']} fields in {st['cdm_steps']} steps", flush=True)
+    for a, b in zip(got, plain):
+        _same(a, b)
+    iters = sum(p["cdm_iterations"] for p in plain)
+    assert 0 < st["cdm_fields"] < iters
