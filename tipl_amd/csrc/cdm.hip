@@ -1528,7 +1528,11 @@ public:
         DeviceGuard dg(device);
         lanes_.resize(size_t(lanes));
         for (Lane& l : lanes_) {
-            TIPLHIP_CHECK(hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking));
+            const hipError_t e = hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking);
+            if (e != hipSuccess) {
+                destroy_streams();  // the destructor does not run for a throwing constructor
+                TIPLHIP_CHECK(e);
+            }
             hipStream_t st = l.stream;
             l.gate.reset(new StepGate<CdmJob>(kMaxSegs, std::chrono::microseconds(kGatherUs),
                                               [this, st](CdmJob* const* J, int n) {
@@ -1537,18 +1541,7 @@ public:
                                               }));
         }
     }
-    ~CdmBatcher()
-    {
-        int prev = -1;
-        (void)hipGetDevice(&prev);
-        (void)hipSetDevice(device_);
-        for (Lane& l : lanes_) {
-            if (!l.stream) continue;
-            (void)hipStreamSynchronize(l.stream);
-            (void)hipStreamDestroy(l.stream);
-        }
-        if (prev >= 0) (void)hipSetDevice(prev);
-    }
+    ~CdmBatcher() { destroy_streams(); }
     CdmBatcher(const CdmBatcher&) = delete;
     CdmBatcher& operator=(const CdmBatcher&) = delete;
 
@@ -1604,6 +1597,19 @@ private:
         std::unique_ptr<StepGate<CdmJob>> gate;
         int members = 0;
     };
+    void destroy_streams()
+    {
+        int prev = -1;
+        (void)hipGetDevice(&prev);
+        (void)hipSetDevice(device_);
+        for (Lane& l : lanes_) {
+            if (!l.stream) continue;
+            (void)hipStreamSynchronize(l.stream);
+            (void)hipStreamDestroy(l.stream);
+            l.stream = nullptr;
+        }
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
     int device_;
     size_t max_voxels_;
     std::mutex mu_;
