@@ -288,3 +288,23 @@ def test_cdm_batching_mixed_levels_and_fallbacks(ctx, cdm_mix, monkeypatch):
         _same(a, b)
     iters = sum(p["cdm_iterations"] for p in plain)
     assert 0 < st["cdm_fields"] < iters
+
+
+def test_cdm_batching_one_lane_full_batches(ctx, cdm_mix, monkeypatch):
+    """One lane (TIPLHIP_CDM_LANES=1): four bidirectional registrations put up
+    to eight fields -- the most a step launch takes -- into each batched step;
+    results equal the plain path's bit for bit."""
+    import tipl_amd
+    monkeypatch.setenv("TIPLHIP_CDM_LANES", "1")
+    outs = ("f2t", "to2from", "from2to", "warped")
+    kw = dict(nonlinear=True, reverse=True, outputs=outs)
+    sel = [cdm_mix[0], cdm_mix[2], cdm_mix[4], cdm_mix[1]]
+    plain = [tipl_amd.register(ctx, f, m, **kw) for f, m in sel]
+    with tipl_amd.Server(0, slots=4, hbm_budget=16 << 30) as srv:
+        ts = [srv.submit(f, m, **kw) for f, m in sel]
+        got = [srv.wait(t) for t in ts]
+        st = srv.stats()
+    print(f"batched CDM: {st['cdm_fields']} fields in {st['cdm_steps']} steps", flush=True)
+    for a, b in zip(got, plain):
+        _same(a, b)
+    assert st["cdm_fields"] >= 3 * st["cdm_steps"]
