@@ -51,7 +51,7 @@ def time_steps(ctx, fn, steps, warmup):
     return time.perf_counter() - t0
 
 
-# The GPU box allots 16 host cores to each GPU; the CPU legs use at most that
+# Each GPU comes with a share of 16 host cores; the CPU legs use at most that
 # many threads (fewer if the process may run on fewer cores).
 CPU_SHARE = 16
 
