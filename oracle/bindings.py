@@ -127,7 +127,118 @@ class _Warp:
         return float(self._corr_r(x, y, x.size, int(bool(masked))))
 
 
-class Oracle(_Lib, _Warp):
+class _Cdm2:
+    """The 2-D CDM pieces (cdm<out,uint8,2>), shared by both libraries.
+    Images are C-ordered (h, w) uint8 arrays; fields (h, w, 2) float32."""
+
+    def _bind_cdm2(self, prefix: str):
+        P = C.POINTER
+        self._c2_prefix = prefix
+        self._c2_sample = self._fn(prefix + "sample_u8", C.c_uint8, _u8p, _i32p, C.c_float, C.c_float)
+        self._c2_down = self._fn(prefix + "downsample_u8", None, _u8p, _i32p, _u8p, _i32p)
+        self._c2_up = self._fn(prefix + "upsample_f2", None, _f32p, _i32p, _f32p, _i32p)
+        self._c2_compose = self._fn(prefix + "compose_displacement", None, _u8p, _i32p, _f32p, _u8p)
+        self._c2_grad = self._fn(prefix + "cdm_gradient", C.c_float, _u8p, _u8p, _i32p, C.c_char, _f32p)
+        self._c2_acc = self._fn(prefix + "accumulate_displacement", None, _f32p, _f32p, _i32p)
+        self._c2_smooth = self._fn(prefix + "cdm_smooth", None, _f32p, _f32p, _i32p, C.c_float)
+        if prefix == "orc2_":
+            self._c2_poisson = self._fn("orc2_cdm_poisson", None, _f32p, _i32p, C.c_int)
+            self._c2_maxlen = self._fn("orc2_cdm_max_length", C.c_float, _f32p, C.c_size_t)
+            self._c2_cdm = self._fn("orc2_cdm", C.c_int, P(C.c_void_p), P(C.c_void_p), C.c_int,
+                                    _i32p, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int,
+                                    C.c_char, _f32p, P(C.c_int))
+        else:
+            self._c2_poisson = self._fn("tref2_cdm_poisson", None, _f32p, _i32p)
+            self._c2_maxlen = self._fn("tref2_cdm_max_length", C.c_float, _f32p, _i32p)
+            self._c2_cdm = self._fn("tref2_cdm", C.c_int, P(C.c_void_p), P(C.c_void_p), C.c_int,
+                                    _i32p, C.c_float, C.c_float, C.c_float, C.c_int, C.c_char,
+                                    _f32p)
+
+    @staticmethod
+    def _d2(a) -> np.ndarray:
+        """x-fastest dims {w, h} of a (h, w) image or (h, w, 2) field."""
+        return np.array([a.shape[1], a.shape[0]], np.int32)
+
+    def sample2(self, img, x, y) -> int:
+        img = np.ascontiguousarray(img, np.uint8)
+        return int(self._c2_sample(img, self._d2(img), x, y))
+
+    def downsample2(self, v) -> np.ndarray:
+        v = np.ascontiguousarray(v, np.uint8)
+        out = np.zeros(((v.shape[0] + 1) // 2, (v.shape[1] + 1) // 2), np.uint8)
+        od = np.zeros(2, np.int32)
+        self._c2_down(v, self._d2(v), out, od)
+        return out
+
+    def upsample2(self, field, out_shape) -> np.ndarray:
+        field = np.ascontiguousarray(field, np.float32)
+        out = np.zeros(tuple(out_shape) + (2,), np.float32)
+        self._c2_up(field, self._d2(field), out, np.array(tuple(out_shape)[::-1], np.int32))
+        return out
+
+    def compose2(self, Is, d) -> np.ndarray:
+        Is = np.ascontiguousarray(Is, np.uint8)
+        out = np.zeros_like(Is)
+        self._c2_compose(Is, self._d2(Is), np.ascontiguousarray(d, np.float32), out)
+        return out
+
+    def gradient2(self, Js, It, kind="r"):
+        Js = np.ascontiguousarray(Js, np.uint8)
+        It = np.ascontiguousarray(It, np.uint8)
+        nd = np.zeros(Js.shape + (2,), np.float32)
+        c = self._c2_grad(Js, It, self._d2(Js), kind.encode(), nd)
+        return nd, float(c)
+
+    def poisson2(self, nd) -> np.ndarray:
+        """The reference's 12 Jacobi sweeps."""
+        nd = np.array(nd, np.float32, order="C")
+        if self._c2_prefix == "orc2_":
+            self._c2_poisson(nd, self._d2(nd), 12)
+        else:
+            self._c2_poisson(nd, self._d2(nd))
+        return nd
+
+    def max_length2(self, d) -> float:
+        d = np.ascontiguousarray(d, np.float32)
+        if self._c2_prefix == "orc2_":
+            return float(self._c2_maxlen(d, d.size // 2))
+        return float(self._c2_maxlen(d, self._d2(d)))
+
+    def accumulate2(self, cur, nd) -> np.ndarray:
+        cur = np.ascontiguousarray(cur, np.float32)
+        nd = np.array(nd, np.float32, order="C")
+        self._c2_acc(cur, nd, self._d2(cur))
+        return nd
+
+    def smooth2(self, d, dd, s) -> np.ndarray:
+        d = np.ascontiguousarray(d, np.float32)
+        dd = np.array(dd, np.float32, order="C")
+        self._c2_smooth(d, dd, self._d2(d), s)
+        return dd
+
+    def cdm2(self, It, Is, resolution=2.0, speed=0.3, smoothing=0.05, min_dimension=16,
+             gradient_type="r", max_iterations=200):
+        """cdm() on 2-D images; returns (ok, field, iterations or None)."""
+        if isinstance(It, np.ndarray):
+            It, Is = [It], [Is]
+        It = [np.ascontiguousarray(v, np.uint8) for v in It]
+        Is = [np.ascontiguousarray(v, np.uint8) for v in Is]
+        n = len(It)
+        pt = (C.c_void_p * n)(*[v.ctypes.data for v in It])
+        ps = (C.c_void_p * n)(*[v.ctypes.data for v in Is])
+        field = np.zeros(It[0].shape + (2,), np.float32)
+        if self._c2_prefix == "orc2_":
+            iters = C.c_int(0)
+            ok = self._c2_cdm(pt, ps, n, self._d2(It[0]), resolution, speed, smoothing,
+                              min_dimension, max_iterations, gradient_type.encode(), field,
+                              C.byref(iters))
+            return bool(ok), field, iters.value
+        ok = self._c2_cdm(pt, ps, n, self._d2(It[0]), resolution, speed, smoothing, min_dimension,
+                          gradient_type.encode(), field)
+        return bool(ok), field, None
+
+
+class Oracle(_Lib, _Warp, _Cdm2):
     """The C restatement; every method mirrors one reference routine."""
 
     def __init__(self, path: str = ORACLE_SO):
@@ -168,6 +279,7 @@ class Oracle(_Lib, _Warp):
                              C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, C.c_char, _f32p,
                              P(C.c_int))
         self._bind_warp("orc_")
+        self._bind_cdm2("orc2_")
         self._fov = self._fn("orc_match_fov", None, _u8p, _i32p, _f32p, _i32p)
 
     def match_fov(self, It, T, moving_shape) -> np.ndarray:
@@ -363,7 +475,7 @@ class Oracle(_Lib, _Warp):
         return bool(ok), field, iters.value
 
 
-class Reference(_Lib, _Warp):
+class Reference(_Lib, _Warp, _Cdm2):
     """The reference headers compiled through oracle/ref_shim.cpp."""
 
     def __init__(self, path: str = REF_SO):
@@ -406,6 +518,7 @@ class Reference(_Lib, _Warp):
         self._cdm_iters = self._fn("tref_cdm_iterations", C.c_double, _u8p, _u8p, _i32p, C.c_int,
                                    C.c_void_p)
         self._bind_warp("tref_")
+        self._bind_cdm2("tref2_")
         self._fov = self._fn("tref_match_fov", None, _u8p, _i32p, _f32p, _i32p, _f32p, _f32p)
         _ip = C.POINTER(C.c_int)
         self._nii_f = self._fn("tref_nifti_read_f32", C.c_int, C.c_char_p, C.c_int, _f32p,

@@ -956,6 +956,442 @@ ORC_API int orc_cdm(const uint8_t* const* It, const uint8_t* const* Is, int nmod
     return ok;
 }
 
+/* ------------------------------------------------------------------------ */
+/* 2-D CDM (cdm<out,uint8,2>, cdm_cuda<out,uint8,2>: reg/cdm_cu.hpp:12-17)   */
+/* ------------------------------------------------------------------------ */
+/* Images are x-fastest, dim[2] = {w, h}; fields are packed float2 per pixel
+ * (tipl::image<2,tipl::vector<2>>).  Each function is the 2-D case of the
+ * same reference template as its 3-D counterpart above. */
+
+static size_t pix(const int* dim) { return (size_t)dim[0] * (size_t)dim[1]; }
+
+/* interpolator::linear<2>::get_location numerical/interpolation.hpp:283-306
+ * (the same outside rules as the 3-D locate). */
+typedef struct {
+    size_t idx[4];
+    float w[4];
+} orc_bil;
+
+static int orc_bil_locate(const int* dim, float x, float y, orc_bil* t)
+{
+    if (x < 0.0f || y < 0.0f)
+        return 0;
+    if (isnan(x) || isnan(y))
+        return 0;
+    if (x >= 16777216.0f || y >= 16777216.0f)
+        return 0;
+    size_t ix = (size_t)x, iy = (size_t)y, w = (size_t)dim[0];
+    if (ix + 1 >= w || iy + 1 >= (size_t)dim[1])
+        return 0;
+    t->idx[0] = iy * w + ix;
+    t->idx[1] = t->idx[0] + 1;
+    t->idx[2] = t->idx[0] + w;
+    t->idx[3] = t->idx[2] + 1;
+    float p0 = x - (float)ix;
+    float p1 = y - (float)iy;
+    float n0 = 1.0f - p0;
+    float n1 = 1.0f - p1;
+    t->w[0] = n0 * n1;
+    t->w[1] = p0 * n1;
+    t->w[2] = n0 * p1;
+    t->w[3] = p0 * p1;
+    return 1;
+}
+
+/* weighted_sum (numerical/interpolation.hpp:61-73) of uint8 data, then the
+ * uint8 assignment (:25-35). */
+static uint8_t orc_bil_u8(const uint8_t* I, const orc_bil* t)
+{
+    float r = (float)I[t->idx[0]] * t->w[0];
+    for (int k = 1; k < 4; ++k) {
+        float v = (float)I[t->idx[k]];
+        v *= t->w[k];
+        r += v;
+    }
+    return r <= 0.0f ? 0 : (uint8_t)r;
+}
+
+/* weighted_sum of vector<2,float> data. */
+static void orc_bil_f2(const float* F, const orc_bil* t, float* out)
+{
+    for (int c = 0; c < 2; ++c) {
+        float r = F[t->idx[0] * 2 + c] * t->w[0];
+        for (int k = 1; k < 4; ++k) {
+            float v = F[t->idx[k] * 2 + c];
+            v *= t->w[k];
+            r += v;
+        }
+        out[c] = r;
+    }
+}
+
+/* tipl::estimate<linear>(I, pos) for a 2-D uint8 image: 0 when outside. */
+ORC_API uint8_t orc2_sample_u8(const uint8_t* I, const int* dim, float x, float y)
+{
+    orc_bil t;
+    if (!orc_bil_locate(dim, x, y, &t))
+        return 0;
+    return orc_bil_u8(I, &t);
+}
+
+/* downsample_with_padding numerical/resampling.hpp:575-613 in 2-D: shape
+ * (s+1)/2, the uint16 sum of the 2x2 block (a missing far-edge tap replaced by
+ * the tap that drops the missing axes) shifted right by 2. */
+ORC_API void orc2_downsample_u8(const uint8_t* in, const int* dim, uint8_t* out, int* odim)
+{
+    int ow = (dim[0] + 1) / 2, oh = (dim[1] + 1) / 2;
+    odim[0] = ow; odim[1] = oh;
+    size_t w = dim[0];
+    size_t shift[4] = {0, 1, w, 1 + w};
+    size_t o = 0;
+    for (int y = 0; y < oh; ++y)
+        for (int x = 0; x < ow; ++x, ++o) {
+            int x2 = 2 * x, y2 = 2 * y;
+            size_t base = (size_t)y2 * w + x2;
+            int has = 0;
+            if (x2 + 1 < dim[0]) has += 1;
+            if (y2 + 1 < dim[1]) has += 2;
+            uint8_t buf[4];
+            uint16_t sum = buf[0] = in[base];
+            for (int i = 1; i < 4; ++i) {
+                int h = has & i;
+                buf[i] = (h == i) ? in[base + shift[i]] : buf[h];
+                sum += buf[i];
+            }
+            out[o] = (uint8_t)(sum >> 2);
+        }
+}
+
+/* upsample_with_padding<linear> (numerical/resampling.hpp:849-858) of a
+ * float2 field: out(p) = in(p * 0.5) bilinearly; outside samples stay 0. */
+ORC_API void orc2_upsample_f2(const float* in, const int* idim, float* out, const int* odim)
+{
+    memset(out, 0, sizeof(float) * 2 * pix(odim));
+    size_t o = 0;
+    for (int y = 0; y < odim[1]; ++y)
+        for (int x = 0; x < odim[0]; ++x, ++o) {
+            float p0 = (float)x, p1 = (float)y;
+            p0 *= 0.5f; p1 *= 0.5f;
+            orc_bil t;
+            if (orc_bil_locate(idim, p0, p1, &t))
+                orc_bil_f2(in, &t, out + o * 2);
+        }
+}
+
+/* compose_displacement numerical/dif.hpp:90-107 in 2-D. */
+ORC_API void orc2_compose_displacement(const uint8_t* Is, const int* dim, const float* d,
+                                       uint8_t* Js)
+{
+    size_t i = 0;
+    for (int y = 0; y < dim[1]; ++y)
+        for (int x = 0; x < dim[0]; ++x, ++i) {
+            const float* v = d + i * 2;
+            if (v[0] == 0.0f && v[1] == 0.0f) {
+                Js[i] = Is[i];
+                continue;
+            }
+            float p0 = (float)x, p1 = (float)y;
+            p0 += v[0]; p1 += v[1];
+            Js[i] = orc2_sample_u8(Is, dim, p0, p1);
+        }
+}
+
+/* gradient_at numerical/numerical.hpp:112-117 (the 2-D overload). */
+static void orc2_gradient_at(const uint8_t* I, const int* dim, int x, int y, size_t pos, float* g)
+{
+    size_t w = dim[0];
+    g[0] = (x + 1 < dim[0] && x) ? (float)((int)I[pos + 1] - (int)I[pos - 1]) : 0.0f;
+    g[1] = (y + 1 < dim[1] && y) ? (float)((int)I[pos + w] - (int)I[pos - w]) : 0.0f;
+}
+
+/* cdm_get_gradient_r_imp reg/cdm.hpp:46-66 with the 2-D
+ * get_window_at_width<2> (numerical/window.hpp:73-84, a 5x5 window clipped
+ * at the edges) and linear_regression (numerical/statistics.hpp:694-713). */
+static void orc2_grad_r_pixel(const uint8_t* Js, const uint8_t* It, const int* dim, int x, int y,
+                              size_t pos, float* nd, float* cost)
+{
+    if (Js[pos] == 0 || It[pos] == 0)
+        return;
+    int x0 = x - 2 < 0 ? 0 : x - 2, x1 = x + 2 >= dim[0] ? dim[0] - 1 : x + 2;
+    int y0 = y - 2 < 0 ? 0 : y - 2, y1 = y + 2 >= dim[1] ? dim[1] - 1 : y + 2;
+    int64_t sx = 0, sxx = 0, sy = 0, syy = 0, sxy = 0, n = 0;
+    for (int yy = y0; yy <= y1; ++yy)
+        for (int xx = x0; xx <= x1; ++xx) {
+            size_t q = (size_t)yy * dim[0] + xx;
+            int64_t a = Js[q], b = It[q];
+            sx += a; sxx += a * a; sy += b; syy += b * b; sxy += a * b; ++n;
+        }
+    double dn = (double)n;
+    double mean_x = (double)sx / dn;
+    double mean_y = (double)sy / dn;
+    double x_var = (double)sxx / dn - mean_x * mean_x;
+    double y_var = (double)syy / dn - mean_y * mean_y;
+    float a, b, r2;
+    if (x_var == 0.0 || y_var == 0.0) {
+        a = 0.0f;
+        b = (float)mean_y;
+        r2 = 0.0f;
+    } else {
+        double cov = (double)sxy / dn - mean_x * mean_y;
+        a = (float)(cov / x_var);
+        b = (float)(mean_y - (double)a * mean_x);
+        r2 = (float)(cov * cov / x_var / y_var);
+    }
+    if (a > 0.0f) {
+        float g[2];
+        orc2_gradient_at(Js, dim, x, y, pos, g);
+        float s = (float)Js[pos] * a;
+        s = s + b;
+        s = s - (float)It[pos];
+        s = r2 * s;
+        nd[pos * 2 + 0] = 0.0f + g[0] * s;
+        nd[pos * 2 + 1] = 0.0f + g[1] * s;
+        cost[pos] = -r2;
+    }
+}
+
+/* cdm_get_gradient_d_imp reg/cdm.hpp:68-78 in 2-D. */
+static void orc2_grad_d_pixel(const uint8_t* Js, const uint8_t* It, const int* dim, int x, int y,
+                              size_t pos, float* nd, float* cost)
+{
+    float g[2];
+    orc2_gradient_at(Js, dim, x, y, pos, g);
+    int dif = (int)Js[pos] - (int)It[pos];
+    nd[pos * 2 + 0] = 0.0f + g[0] * (float)dif;
+    nd[pos * 2 + 1] = 0.0f + g[1] * (float)dif;
+    cost[pos] = (float)(dif * dif);
+}
+
+/* cdm_get_gradient reg/cdm.hpp:84-106 in 2-D (the FP64 mean summed in pixel
+ * order, as orc_cdm_gradient). */
+ORC_API float orc2_cdm_gradient(const uint8_t* Js, const uint8_t* It, const int* dim, char type,
+                                float* new_d)
+{
+    size_t n = pix(dim);
+    memset(new_d, 0, sizeof(float) * 2 * n);
+    float* cost = (float*)calloc(n ? n : 1, sizeof(float));
+    if (!cost) return NAN;
+    size_t i = 0;
+    for (int y = 0; y < dim[1]; ++y)
+        for (int x = 0; x < dim[0]; ++x, ++i) {
+            if (type == 'd')
+                orc2_grad_d_pixel(Js, It, dim, x, y, i, new_d, cost);
+            else
+                orc2_grad_r_pixel(Js, It, dim, x, y, i, new_d, cost);
+        }
+    double s = 0.0;
+    for (size_t k = 0; k < n; ++k) s += (double)cost[k];
+    free(cost);
+    return n ? (float)(s / (double)n) : 0.0f;
+}
+
+/* cdm_solve_poisson reg/cdm.hpp:158-198 in 2-D: solve = new_d * (-1/4), then
+ * `sweeps` Jacobi sweeps of the 5-point flat-index stencil (neighbours in the
+ * order -1, +1, -w, +w; missing ones dropped at the ends of the array). */
+ORC_API void orc2_cdm_poisson(float* new_d, const int* dim, int sweeps)
+{
+    size_t n = pix(dim);
+    float* a = (float*)malloc(sizeof(float) * 2 * (n ? n : 1));
+    float* b = (float*)malloc(sizeof(float) * 2 * (n ? n : 1));
+    if (!a || !b) { free(a); free(b); return; }
+    const float inv_d2 = -0.5f / 2.0f;
+    for (size_t i = 0; i < 2 * n; ++i) a[i] = new_d[i] * inv_d2;
+    long w = dim[0], size = (long)n;
+    for (int it = 0; it < sweeps; ++it) {
+        for (long p = 0; p < size; ++p)
+            for (int c = 0; c < 2; ++c) {
+                float v = new_d[p * 2 + c];
+                if (p >= 1) v -= a[(p - 1) * 2 + c];
+                if (p < size - 1) v -= a[(p + 1) * 2 + c];
+                if (p >= w) v -= a[(p - w) * 2 + c];
+                if (p < size - w) v -= a[(p + w) * 2 + c];
+                v *= inv_d2;
+                b[p * 2 + c] = v;
+            }
+        float* t = a; a = b; b = t;
+    }
+    memcpy(new_d, a, sizeof(float) * 2 * n);
+    free(a);
+    free(b);
+}
+
+/* cdm_max_displacement_length reg/cdm.hpp:269-284 with vector<2>::length()
+ * utility/pixel_index.hpp:340: std::sqrt of the float x*x+y*y. */
+ORC_API float orc2_cdm_max_length(const float* d, size_t n)
+{
+    float m = 0.0f;
+    for (size_t i = 0; i < n; ++i) {
+        float s = d[i * 2] * d[i * 2] + d[i * 2 + 1] * d[i * 2 + 1];
+        float l = sqrtf(s);
+        if (l > m) m = l;
+    }
+    return m;
+}
+
+/* accumulate_displacement numerical/dif.hpp:166-186 in 2-D. */
+ORC_API void orc2_accumulate_displacement(const float* cur, float* nd, const int* dim)
+{
+    size_t n = pix(dim);
+    float* map = (float*)malloc(sizeof(float) * 2 * (n ? n : 1));
+    if (!map) return;
+    size_t i = 0;
+    for (int y = 0; y < dim[1]; ++y)
+        for (int x = 0; x < dim[0]; ++x, ++i) {
+            map[i * 2 + 0] = cur[i * 2 + 0] + (float)x;
+            map[i * 2 + 1] = cur[i * 2 + 1] + (float)y;
+        }
+    i = 0;
+    for (int y = 0; y < dim[1]; ++y)
+        for (int x = 0; x < dim[0]; ++x, ++i) {
+            float* v = nd + i * 2;
+            int moved = 0;
+            if (v[0] != 0.0f || v[1] != 0.0f) {
+                float p0 = v[0] + (float)x, p1 = v[1] + (float)y;
+                orc_bil t;
+                if (orc_bil_locate(dim, p0, p1, &t)) {
+                    float r[2];
+                    orc_bil_f2(map, &t, r);
+                    v[0] = r[0] - (float)x;
+                    v[1] = r[1] - (float)y;
+                    moved = 1;
+                }
+            }
+            if (!moved) {
+                v[0] = cur[i * 2 + 0];
+                v[1] = cur[i * 2 + 1];
+            }
+        }
+    free(map);
+}
+
+/* cdm_smooth reg/cdm.hpp:302-342 in 2-D: dd = d*(1-s) + (s/4)*(the four
+ * flat-index neighbours, added in the order +1, -1, +w, -w).  Does nothing
+ * when smoothing == 0. */
+ORC_API void orc2_cdm_smooth(const float* d, float* dd, const int* dim, float smoothing)
+{
+    if (smoothing == 0.0f)
+        return;
+    size_t n = pix(dim), w = dim[0];
+    float w_6 = smoothing / (2.0f * 2.0f);
+    float w_1 = 1.0f - smoothing;
+    for (size_t i = 0; i < n; ++i)
+        for (int c = 0; c < 2; ++c) {
+            float v = 0.0f;
+            if (i + 1 < n) v += d[(i + 1) * 2 + c];
+            if (i >= 1) v += d[(i - 1) * 2 + c];
+            if (i + w < n) v += d[(i + w) * 2 + c];
+            if (i >= w) v += d[(i - w) * 2 + c];
+            v *= w_6;
+            dd[i * 2 + c] = d[i * 2 + c] * w_1 + v;
+        }
+}
+
+/* One call of the 2-D cdm() (reg/cdm.hpp:373-474), as orc_cdm_rec. */
+static int orc2_cdm_rec(const uint8_t* const* It, const uint8_t* const* Is, int nmod,
+                        const int* dim, float* best_d, orc_cdm_param p, int* iterations)
+{
+    size_t n = pix(dim);
+    int mn = dim[0] < dim[1] ? dim[0] : dim[1];
+    if (mn > p.min_dimension) {
+        int rdim[2] = {(dim[0] + 1) / 2, (dim[1] + 1) / 2};
+        size_t rn = pix(rdim);
+        uint8_t* buf = (uint8_t*)malloc(2 * (size_t)nmod * rn);
+        float* rd = (float*)calloc(2 * rn, sizeof(float));
+        const uint8_t* rIt[16];
+        const uint8_t* rIs[16];
+        if (!buf || !rd || nmod > 16) { free(buf); free(rd); return 0; }
+        for (int i = 0; i < nmod; ++i) {
+            int od[2];
+            orc2_downsample_u8(It[i], dim, buf + (size_t)(2 * i) * rn, od);
+            orc2_downsample_u8(Is[i], dim, buf + (size_t)(2 * i + 1) * rn, od);
+            rIt[i] = buf + (size_t)(2 * i) * rn;
+            rIs[i] = buf + (size_t)(2 * i + 1) * rn;
+        }
+        orc_cdm_param p2 = p;
+        p2.resolution /= 2.0f;
+        int ok = orc2_cdm_rec(rIt, rIs, nmod, rdim, rd, p2, iterations);
+        free(buf);
+        if (!ok) { free(rd); return 0; }
+        for (size_t k = 0; k < 2 * rn; ++k) rd[k] *= 2.0f;
+        orc2_upsample_f2(rd, rdim, best_d, dim);
+        free(rd);
+        if (p.resolution > 1.0f)
+            return 1;
+    }
+
+    float* cur = (float*)malloc(sizeof(float) * 2 * (n ? n : 1));
+    float* nd = (float*)malloc(sizeof(float) * 2 * (n ? n : 1));
+    float* dd = (float*)malloc(sizeof(float) * 2 * (n ? n : 1));
+    uint8_t* Js = (uint8_t*)malloc(n ? n : 1);
+    if (!cur || !nd || !dd || !Js) { free(cur); free(nd); free(dd); free(Js); return 0; }
+    memcpy(cur, best_d, sizeof(float) * 2 * n);
+    float theta = 0.0f;
+    float cost[16];
+    int ncost = 0;
+    float best_cost = 3.402823466e+38f;
+    float cur_smoothing = 0.8f;
+    int ok = 1;
+    for (int index = 0; index < p.max_iterations; ++index) {
+        ++*iterations;
+        float sum_cost = 0.0f;
+        for (int i = 0; i < nmod; ++i) {
+            orc2_compose_displacement(Is[i], dim, cur, Js);
+            float c = orc2_cdm_gradient(Js, It[i], dim, p.gradient_type, i == 0 ? nd : dd);
+            sum_cost += c;
+            if (sum_cost == 0.0f) { ok = 0; goto done; }
+            if (i)
+                for (size_t k = 0; k < 2 * n; ++k) nd[k] += dd[k];
+        }
+        cost[ncost++] = sum_cost / (float)nmod;
+        if (cost[ncost - 1] < best_cost) {
+            best_cost = cost[ncost - 1];
+            if (index)
+                memcpy(best_d, cur, sizeof(float) * 2 * n);
+        }
+        if (!orc_cdm_improved(cost, &ncost)) {
+            if (cur_smoothing <= p.smoothing)
+                break;
+            memcpy(cur, best_d, sizeof(float) * 2 * n);
+            cur_smoothing *= 0.5f;
+            ncost = 0;
+        }
+        orc2_cdm_poisson(nd, dim, 12);
+        if (theta == 0.0f)
+            theta = orc2_cdm_max_length(nd, n);
+        if (theta == 0.0f)
+            break;
+        float k = p.speed / theta;
+        for (size_t j = 0; j < 2 * n; ++j) nd[j] *= k;
+        orc2_accumulate_displacement(cur, nd, dim);
+        orc2_cdm_smooth(nd, cur, dim, cur_smoothing);
+    }
+done:
+    free(cur); free(nd); free(dd); free(Js);
+    return ok;
+}
+
+/* tipl::reg::cdm for 2-D uint8 images (reg/cdm.hpp:373-474; the reference's
+ * GPU instantiation cdm_cuda<out,uint8,2>, reg/cdm_cu.hpp:12-17, runs the same
+ * template).  `field` receives pix(dim) float2.  Returns 1 on success, 0
+ * when the reference returns false. */
+ORC_API int orc2_cdm(const uint8_t* const* It, const uint8_t* const* Is, int nmod, const int* dim,
+                     float resolution, float speed, float smoothing, int min_dimension,
+                     int max_iterations, char gradient_type, float* field, int* iterations)
+{
+    orc_cdm_param p;
+    p.resolution = resolution;
+    p.speed = speed;
+    p.smoothing = smoothing;
+    p.min_dimension = min_dimension;
+    p.max_iterations = max_iterations;
+    p.gradient_type = gradient_type;
+    memset(field, 0, sizeof(float) * 2 * pix(dim));
+    int it = 0;
+    int ok = orc2_cdm_rec(It, Is, nmod, dim, field, p, &it);
+    if (iterations) *iterations = it;
+    return ok;
+}
+
 /* filter::gaussian filter/gaussian.hpp:18-89 on a float image of `ch`
  * interleaved channels (1 = scalar image, 3 = vector<3> field): `bandwidth`
  * rounds of the separable [1 2 1] stencil along x, then y, then z, each pass

@@ -36,6 +36,21 @@ tipl::transformation_matrix<float, 3> mat(const float* T)
 }
 
 using field_t = tipl::image<3, tipl::vector<3>>;
+using field2_t = tipl::image<2, tipl::vector<2>>;
+
+tipl::shape<2> shp2(const int* d) { return tipl::shape<2>(d[0], d[1]); }
+
+field2_t to_field2(const float* f, const int* d)
+{
+    field2_t F(shp2(d));
+    std::memcpy(&F[0][0], f, sizeof(float) * 2 * F.size());
+    return F;
+}
+
+void from_field2(const field2_t& F, float* f)
+{
+    std::memcpy(f, reinterpret_cast<const float*>(F.data()), sizeof(float) * 2 * F.size());
+}
 
 // Log sink for mm_reg<out_type>, whose show_r writes unconditionally.
 struct null_out {
@@ -353,6 +368,102 @@ int tref_cdm(const uint8_t* const* It, const uint8_t* const* Is, int nmod, const
     bool terminated = false;
     bool ok = tipl::reg::cdm(vt, vs, D, terminated, p);
     if (ok) from_field(D, field);
+    return ok ? 1 : 0;
+}
+
+// ---- the 2-D instantiations (cdm<out,uint8,2>) ----------------------------
+
+uint8_t tref2_sample_u8(const uint8_t* I, const int* dim, float x, float y)
+{
+    tipl::const_pointer_image<2, uint8_t> P(I, shp2(dim));
+    uint8_t v = 0;
+    tipl::estimate<tipl::interpolation::linear>(P, tipl::vector<2>(x, y), v);
+    return v;
+}
+
+void tref2_downsample_u8(const uint8_t* in, const int* dim, uint8_t* out, int* odim)
+{
+    tipl::const_pointer_image<2, uint8_t> I(in, shp2(dim));
+    tipl::image<2, uint8_t> O;
+    tipl::downsample_with_padding(I, O);
+    odim[0] = O.width();
+    odim[1] = O.height();
+    std::memcpy(out, O.data(), O.size());
+}
+
+void tref2_upsample_f2(const float* in, const int* idim, float* out, const int* odim)
+{
+    field2_t F = to_field2(in, idim);
+    tipl::upsample_with_padding(F, shp2(odim));
+    from_field2(F, out);
+}
+
+void tref2_compose_displacement(const uint8_t* Is, const int* dim, const float* d, uint8_t* Js)
+{
+    tipl::const_pointer_image<2, uint8_t> I(Is, shp2(dim));
+    field2_t D = to_field2(d, dim);
+    auto J = tipl::compose_displacement(I, D);
+    std::memcpy(Js, J.data(), J.size());
+}
+
+float tref2_cdm_gradient(const uint8_t* Js, const uint8_t* It, const int* dim, char type,
+                         float* new_d)
+{
+    tipl::image<2, uint8_t> J(Js, shp2(dim));
+    tipl::const_pointer_image<2, uint8_t> T(It, shp2(dim));
+    field2_t D;
+    float c = tipl::reg::cdm_get_gradient(J, T, D, (unsigned char)type);
+    from_field2(D, new_d);
+    return c;
+}
+
+void tref2_cdm_poisson(float* new_d, const int* dim)
+{
+    field2_t D = to_field2(new_d, dim);
+    bool terminated = false;
+    tipl::reg::cdm_solve_poisson(D, terminated);
+    from_field2(D, new_d);
+}
+
+float tref2_cdm_max_length(const float* d, const int* dim)
+{
+    field2_t D = to_field2(d, dim);
+    return tipl::reg::cdm_max_displacement_length(D);
+}
+
+void tref2_accumulate_displacement(const float* cur, float* nd, const int* dim)
+{
+    field2_t C = to_field2(cur, dim), N = to_field2(nd, dim);
+    tipl::accumulate_displacement(C, N);
+    from_field2(N, nd);
+}
+
+void tref2_cdm_smooth(const float* d, float* dd, const int* dim, float smoothing)
+{
+    field2_t D = to_field2(d, dim), DD = to_field2(dd, dim);
+    tipl::reg::cdm_smooth(D, DD, smoothing);
+    from_field2(DD, dd);
+}
+
+int tref2_cdm(const uint8_t* const* It, const uint8_t* const* Is, int nmod, const int* dim,
+              float resolution, float speed, float smoothing, int min_dimension,
+              char gradient_type, float* field)
+{
+    std::vector<tipl::const_pointer_image<2, uint8_t>> vt, vs;
+    for (int i = 0; i < nmod; ++i) {
+        vt.emplace_back(It[i], shp2(dim));
+        vs.emplace_back(Is[i], shp2(dim));
+    }
+    tipl::reg::cdm_param p;
+    p.resolution = resolution;
+    p.speed = speed;
+    p.smoothing = smoothing;
+    p.min_dimension = (unsigned int)min_dimension;
+    p.gradient_type = std::string(size_t(nmod), gradient_type);
+    field2_t D;
+    bool terminated = false;
+    bool ok = tipl::reg::cdm(vt, vs, D, terminated, p);
+    if (ok) from_field2(D, field);
     return ok ? 1 : 0;
 }
 
