@@ -315,6 +315,37 @@ int tiplhip_cdm_smooth(tiplhip_ctx* ctx, const float* d, float* dd, const int di
 int tiplhip_upsample_field(tiplhip_ctx* ctx, const float* in, const int in_dim[3], float* out,
                            const int out_dim[3]);                        /* numerical/resampling.hpp:849-858 */
 
+/* ---- 2-D CDM -------------------------------------------------------------- */
+
+/* tipl::reg::cdm for 2-D uint8 images: the reference's second GPU plug,
+ * cdm_cuda<out,uint8,2> (declared reg/cdm.hpp:525-531, instantiated
+ * reg/cdm_cu.hpp:12-17), which runs the cdm() template (reg/cdm.hpp:373-474)
+ * at dimension 2.  It and Is are n host images of dim[0] x dim[1] pixels
+ * (x-fastest); field (host) receives dim[0]*dim[1] float2.  param as for
+ * tiplhip_cdm (the fixed schedule, levels > 0, is 3-D only); terminated and
+ * the return codes as for tiplhip_cdm. */
+int tiplhip_cdm2d(tiplhip_ctx* ctx, const uint8_t* const* It, const uint8_t* const* Is, int n,
+                  const int dim[2], const tiplhip_cdm_param* param, float* field,
+                  const volatile uint8_t* terminated, int* iterations);
+
+/* Single 2-D CDM steps on host buffers, for parity checks of each kernel
+ * against the reference routine it restates at dimension 2. */
+int tiplhip_cdm2d_compose(tiplhip_ctx* ctx, const uint8_t* Is, const int dim[2], const float* d,
+                          uint8_t* Js);                                   /* numerical/dif.hpp:90-107 */
+int tiplhip_cdm2d_gradient(tiplhip_ctx* ctx, const uint8_t* Js, const uint8_t* It,
+                           const int dim[2], char type, float* new_d,
+                           float* cost);                                  /* reg/cdm.hpp:46-106 */
+int tiplhip_cdm2d_poisson(tiplhip_ctx* ctx, float* new_d, const int dim[2], int sweeps,
+                          float* max_length);                             /* reg/cdm.hpp:158-296 */
+int tiplhip_cdm2d_accumulate(tiplhip_ctx* ctx, const float* cur, float* new_d, const int dim[2],
+                             float scale);                                /* numerical/dif.hpp:166-186 */
+int tiplhip_cdm2d_smooth(tiplhip_ctx* ctx, const float* d, float* dd, const int dim[2],
+                         float smoothing);                                /* reg/cdm.hpp:302-342 */
+int tiplhip_cdm2d_downsample(tiplhip_ctx* ctx, const uint8_t* in, const int dim[2],
+                             uint8_t* out);                               /* numerical/resampling.hpp:575-613 */
+int tiplhip_cdm2d_upsample(tiplhip_ctx* ctx, const float* in, const int in_dim[2], float* out,
+                           const int out_dim[2]);                         /* numerical/resampling.hpp:849-858 */
+
 /* tipl::filter::gaussian(image, bandwidth) (filter/gaussian.hpp:18-105), in
  * place on a host float image of dim[0]*dim[1]*dim[2] voxels with `channels`
  * interleaved floats each (1: scalar image, 3: vector<3> field).  Bit-exact:

@@ -42,6 +42,9 @@ EXPORTS = (
     "tiplhip_inv_displacement_to_mapping", "tiplhip_compose_mapping",
     "tiplhip_jacobian_determinant_dis", "tiplhip_jacobian_determinant", "tiplhip_match_fov",
     "tiplhip_correlation", "tiplhip_nifti_read_info", "tiplhip_nifti_read", "tiplhip_nifti_write",
+    "tiplhip_cdm2d", "tiplhip_cdm2d_compose", "tiplhip_cdm2d_gradient", "tiplhip_cdm2d_poisson",
+    "tiplhip_cdm2d_accumulate", "tiplhip_cdm2d_smooth", "tiplhip_cdm2d_downsample",
+    "tiplhip_cdm2d_upsample",
 )
 
 
@@ -153,6 +156,14 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "tiplhip_accumulate_displacement": (ip, [vp, vp, vp, P(ip), C.c_float]),
         "tiplhip_cdm_smooth": (ip, [vp, vp, vp, P(ip), C.c_float]),
         "tiplhip_upsample_field": (ip, [vp, vp, P(ip), vp, P(ip)]),
+        "tiplhip_cdm2d": (ip, [vp, vp, vp, ip, P(ip), P(CdmParam), vp, vp, P(ip)]),
+        "tiplhip_cdm2d_compose": (ip, [vp, vp, P(ip), vp, vp]),
+        "tiplhip_cdm2d_gradient": (ip, [vp, vp, vp, P(ip), C.c_char, vp, P(C.c_float)]),
+        "tiplhip_cdm2d_poisson": (ip, [vp, vp, P(ip), ip, P(C.c_float)]),
+        "tiplhip_cdm2d_accumulate": (ip, [vp, vp, vp, P(ip), C.c_float]),
+        "tiplhip_cdm2d_smooth": (ip, [vp, vp, vp, P(ip), C.c_float]),
+        "tiplhip_cdm2d_downsample": (ip, [vp, vp, P(ip), vp]),
+        "tiplhip_cdm2d_upsample": (ip, [vp, vp, P(ip), vp, P(ip)]),
         "tiplhip_gaussian": (ip, [vp, vp, P(ip), ip, C.c_uint]),
         "tiplhip_cost_batch_pairs": (ip, [vp, P(PairCosts), ip, ip, ip]),
         "tiplhip_volume_resample": (ip, [vp, vp, P(ip), vp, ip, P(vp)]),
@@ -197,6 +208,12 @@ def _dims(shape) -> "C.Array":
     """x-fastest {w,h,d} from a C-ordered (z, y, x) numpy shape."""
     z, y, x = shape
     return (C.c_int * 3)(x, y, z)
+
+
+def _dims2(shape) -> "C.Array":
+    """x-fastest {w,h} from a C-ordered (y, x) numpy shape."""
+    y, x = shape
+    return (C.c_int * 2)(x, y)
 
 
 def _ptr(a: np.ndarray) -> int:
@@ -571,6 +588,79 @@ class Context:
         out = np.zeros(tuple(out_shape) + (3,), np.float32)
         _check(load_library().tiplhip_upsample_field(self.handle, _ptr(field), _dims(field.shape[:3]),
                                                      _ptr(out), _dims(out_shape)))
+        return out
+
+    # -- 2-D CDM (cdm<out,uint8,2>): (h, w) uint8 images, (h, w, 2) fields --
+    def cdm2d(self, It, Is, param: Optional[CdmParam] = None,
+              terminated: Optional[C.c_uint8] = None):
+        """tipl::reg::cdm on 2-D uint8 host images (one or a list of
+        modalities); returns (status, field, iterations)."""
+        if isinstance(It, np.ndarray):
+            It, Is = [It], [Is]
+        It = [np.ascontiguousarray(v, np.uint8) for v in It]
+        Is = [np.ascontiguousarray(v, np.uint8) for v in Is]
+        n = len(It)
+        at = (C.c_void_p * n)(*[v.ctypes.data for v in It])
+        as_ = (C.c_void_p * n)(*[v.ctypes.data for v in Is])
+        field = np.zeros(It[0].shape + (2,), np.float32)
+        iters = C.c_int(0)
+        p = param if param is not None else self.cdm_param()
+        rc = load_library().tiplhip_cdm2d(self.handle, at, as_, n, _dims2(It[0].shape), C.byref(p),
+                                          _ptr(field),
+                                          C.byref(terminated) if terminated is not None else None,
+                                          C.byref(iters))
+        return rc, field, iters.value
+
+    def compose2d(self, Is: np.ndarray, d: np.ndarray) -> np.ndarray:
+        Is = np.ascontiguousarray(Is, np.uint8)
+        d = np.ascontiguousarray(d, np.float32)
+        out = np.empty_like(Is)
+        _check(load_library().tiplhip_cdm2d_compose(self.handle, _ptr(Is), _dims2(Is.shape), _ptr(d),
+                                                     _ptr(out)))
+        return out
+
+    def gradient2d(self, Js: np.ndarray, It: np.ndarray, kind: str = "r"):
+        Js = np.ascontiguousarray(Js, np.uint8)
+        It = np.ascontiguousarray(It, np.uint8)
+        nd = np.zeros(Js.shape + (2,), np.float32)
+        c = C.c_float(0)
+        _check(load_library().tiplhip_cdm2d_gradient(self.handle, _ptr(Js), _ptr(It), _dims2(Js.shape),
+                                                      kind.encode(), _ptr(nd), C.byref(c)))
+        return nd, c.value
+
+    def poisson2d(self, nd: np.ndarray, sweeps: int = 12):
+        nd = np.array(nd, np.float32, order="C")
+        ml = C.c_float(0)
+        _check(load_library().tiplhip_cdm2d_poisson(self.handle, _ptr(nd), _dims2(nd.shape[:2]), sweeps,
+                                                     C.byref(ml)))
+        return nd, ml.value
+
+    def accumulate2d(self, cur: np.ndarray, nd: np.ndarray, scale: float = 1.0) -> np.ndarray:
+        cur = np.ascontiguousarray(cur, np.float32)
+        nd = np.array(nd, np.float32, order="C")
+        _check(load_library().tiplhip_cdm2d_accumulate(self.handle, _ptr(cur), _ptr(nd),
+                                                        _dims2(cur.shape[:2]), C.c_float(scale)))
+        return nd
+
+    def smooth2d(self, d: np.ndarray, dd: np.ndarray, smoothing: float) -> np.ndarray:
+        d = np.ascontiguousarray(d, np.float32)
+        dd = np.array(dd, np.float32, order="C")
+        _check(load_library().tiplhip_cdm2d_smooth(self.handle, _ptr(d), _ptr(dd), _dims2(d.shape[:2]),
+                                                    C.c_float(smoothing)))
+        return dd
+
+    def downsample2d(self, img: np.ndarray) -> np.ndarray:
+        img = np.ascontiguousarray(img, np.uint8)
+        out = np.empty(((img.shape[0] + 1) // 2, (img.shape[1] + 1) // 2), np.uint8)
+        _check(load_library().tiplhip_cdm2d_downsample(self.handle, _ptr(img), _dims2(img.shape),
+                                                        _ptr(out)))
+        return out
+
+    def upsample2d(self, field: np.ndarray, out_shape) -> np.ndarray:
+        field = np.ascontiguousarray(field, np.float32)
+        out = np.zeros(tuple(out_shape) + (2,), np.float32)
+        _check(load_library().tiplhip_cdm2d_upsample(self.handle, _ptr(field), _dims2(field.shape[:2]),
+                                                      _ptr(out), _dims2(out_shape)))
         return out
 
     def gaussian(self, img: np.ndarray, bandwidth: int = 1) -> np.ndarray:

@@ -15,7 +15,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtiplhip.so")
-SOURCES = ["api.hip", "affine.hip", "cdm.hip", "filter.hip", "prep.hip", "warp.hip", "server.cpp", "nifti.cpp"]
+SOURCES = ["api.hip", "affine.hip", "cdm.hip", "cdm2d.hip", "filter.hip", "prep.hip", "warp.hip", "server.cpp",
+           "nifti.cpp"]
 
 # -ffp-contract=off: the trilinear paths must round every FP32 operation in
 # source order to reproduce the reference's uint8 truncation bit for bit.
