@@ -2,8 +2,8 @@
 //
 // Mirrors the parts of the reference's containers that tipl::reg::linear and
 // tipl::reg::cdm take and return (utility/shape.hpp, utility/pixel_index.hpp:
-// 410-640, utility/basic_image.hpp:136-317, 616-693, 930-947): shape<3>,
-// vector<3,T>, image<3,T> (x fastest, std::vector storage),
+// 259-640, utility/basic_image.hpp:136-317, 616-693, 930-947): shape<2|3>,
+// vector<2|3,T>, image<dim,T> (x fastest, std::vector storage),
 // pointer_image / const_pointer_image (non-owning views) and make_shared.
 #ifndef TIPL_HIP_IMAGE_HPP
 #define TIPL_HIP_IMAGE_HPP
@@ -41,8 +41,59 @@ public:
     bool operator!=(const shape& r) const { return !(*this == r); }
 };
 
+template <>
+class shape<2> {
+public:
+    static constexpr int dimension = 2;
+    unsigned int dim[2] = {0, 0};
+    shape() = default;
+    shape(unsigned int w, unsigned int h) : dim{w, h} {}
+    template <class T>
+    explicit shape(const T* p) : dim{unsigned(p[0]), unsigned(p[1])} {}
+    unsigned int operator[](int i) const { return dim[i]; }
+    unsigned int& operator[](int i) { return dim[i]; }
+    unsigned int width() const { return dim[0]; }
+    unsigned int height() const { return dim[1]; }
+    size_t plane_size() const { return size_t(dim[0]) * dim[1]; }
+    size_t size() const { return size_t(dim[0]) * dim[1]; }
+    const unsigned int* begin() const { return dim; }
+    const unsigned int* end() const { return dim + 2; }
+    bool operator==(const shape& r) const { return std::equal(dim, dim + 2, r.dim); }
+    bool operator!=(const shape& r) const { return !(*this == r); }
+};
+
 template <int dim, typename T = float>
 class vector;
+
+// tipl::vector<2,T>: two packed components (the 2-D displacement field pixel).
+template <typename T>
+class vector<2, T> {
+public:
+    static constexpr int dimension = 2;
+    using value_type = T;
+    T v[2] = {T(0), T(0)};
+    vector() = default;
+    vector(T x, T y) : v{x, y} {}
+    template <class U>
+    explicit vector(const U* p) : v{T(p[0]), T(p[1])} {}
+    T operator[](int i) const { return v[i]; }
+    T& operator[](int i) { return v[i]; }
+    T* begin() { return v; }
+    T* end() { return v + 2; }
+    const T* begin() const { return v; }
+    const T* end() const { return v + 2; }
+    vector& operator*=(T s)
+    {
+        for (T& c : v) c *= s;
+        return *this;
+    }
+    vector& operator+=(const vector& r)
+    {
+        for (int i = 0; i < 2; ++i) v[i] += r.v[i];
+        return *this;
+    }
+    double length() const { return std::sqrt(v[0] * v[0] + v[1] * v[1]); }
+};
 
 // tipl::vector<3,T>: three packed components (the displacement field voxel).
 template <typename T>
@@ -76,28 +127,26 @@ public:
 
 // a displacement field is handed to the library as a packed float array
 static_assert(sizeof(vector<3, float>) == 3 * sizeof(float), "vector<3> must be three packed floats");
+static_assert(sizeof(vector<2, float>) == 2 * sizeof(float), "vector<2> must be two packed floats");
 
+// Owning x-fastest image (depth() exists for dim 3 only).
 template <int dim, typename T>
-class image;
-
-// Owning x-fastest image.
-template <typename T>
-class image<3, T> {
+class image {
 public:
-    static constexpr int dimension = 3;
+    static constexpr int dimension = dim;
     using value_type = T;
-    using buffer_type = image<3, T>;
+    using buffer_type = image<dim, T>;
     image() = default;
-    explicit image(const tipl::shape<3>& s) : sp_(s), data_(s.size()) {}
-    image(const T* p, const tipl::shape<3>& s) : sp_(s), data_(p, p + s.size()) {}
-    const tipl::shape<3>& shape() const { return sp_; }
+    explicit image(const tipl::shape<dim>& s) : sp_(s), data_(s.size()) {}
+    image(const T* p, const tipl::shape<dim>& s) : sp_(s), data_(p, p + s.size()) {}
+    const tipl::shape<dim>& shape() const { return sp_; }
     unsigned int width() const { return sp_.width(); }
     unsigned int height() const { return sp_.height(); }
     unsigned int depth() const { return sp_.depth(); }
     size_t plane_size() const { return sp_.plane_size(); }
     size_t size() const { return data_.size(); }
     bool empty() const { return data_.empty(); }
-    void resize(const tipl::shape<3>& s)
+    void resize(const tipl::shape<dim>& s)
     {
         sp_ = s;
         data_.resize(s.size());
@@ -105,7 +154,7 @@ public:
     void clear()
     {
         data_.clear();
-        sp_ = tipl::shape<3>();
+        sp_ = tipl::shape<dim>();
     }
     T* data() { return data_.data(); }
     const T* data() const { return data_.data(); }
@@ -122,7 +171,7 @@ public:
     }
 
 private:
-    tipl::shape<3> sp_;
+    tipl::shape<dim> sp_;
     std::vector<T> data_;
 };
 
@@ -174,16 +223,16 @@ private:
     tipl::shape<dim> sp_;
 };
 
-template <typename T>
-const_pointer_image<3, T> make_shared(const image<3, T>& I)
+template <int dim, typename T>
+const_pointer_image<dim, T> make_shared(const image<dim, T>& I)
 {
-    return const_pointer_image<3, T>(I.data(), I.shape());
+    return const_pointer_image<dim, T>(I.data(), I.shape());
 }
 
-template <typename T>
-pointer_image<3, T> make_shared(image<3, T>& I)
+template <int dim, typename T>
+pointer_image<dim, T> make_shared(image<dim, T>& I)
 {
-    return pointer_image<3, T>(I.data(), I.shape());
+    return pointer_image<dim, T>(I.data(), I.shape());
 }
 
 }  // namespace tipl

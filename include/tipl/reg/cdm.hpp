@@ -59,43 +59,71 @@ inline tiplhip_cdm_param to_c(const cdm_param& p)
 }  // namespace detail
 
 // tipl::reg::cdm (reg/cdm.hpp:373-474): It / Is are lists of modalities of
-// one shape; dis receives the displacement field (x-fastest float3).
+// one shape; dis receives the displacement field (x-fastest, packed
+// vector<dim>).  Volumes go through tiplhip_cdm; 2-D uint8 images (the
+// reference's cdm_cuda<out,uint8,2>, reg/cdm_cu.hpp:12-17) through
+// tiplhip_cdm2d.
 template <typename out_type = void, typename value_type, int dim>
 bool cdm(const std::vector<const_pointer_image<dim, value_type>>& It,
          const std::vector<const_pointer_image<dim, value_type>>& Is,
          image<dim, vector<dim>>& dis, bool& terminated, cdm_param param = cdm_param())
 {
-    static_assert(dim == 3, "the MI355X path registers 3-D volumes");
+    static_assert(dim == 3 || dim == 2, "cdm registers 2-D or 3-D images");
     const size_t n = std::min(It.size(), Is.size());
     if (n == 0) throw std::invalid_argument("cdm: empty image list");
     hip::context& ctx = hip::thread_context();
-    std::vector<std::unique_ptr<hip::volume>> vt, vs;
-    std::vector<const tiplhip_volume*> pt, ps;
-    for (size_t i = 0; i < n; ++i) {
-        if constexpr (std::is_same_v<value_type, unsigned char>) {
-            vt.emplace_back(new hip::volume(ctx, It[i].data(), It[i].shape()));
-            vs.emplace_back(new hip::volume(ctx, Is[i].data(), Is[i].shape()));
-        } else {
-            vt.emplace_back(new hip::volume(ctx, It[i].data(), It[i].shape(), 255.99f));
-            vs.emplace_back(new hip::volume(ctx, Is[i].data(), Is[i].shape(), 255.99f));
+    if constexpr (dim == 2) {
+        // the reference's 2-D GPU instantiation, cdm_cuda<out,uint8,2>
+        // (reg/cdm_cu.hpp:12-17), takes uint8 images
+        static_assert(std::is_same_v<value_type, unsigned char>, "2-D cdm takes uint8 images");
+        std::vector<const uint8_t*> pt, ps;
+        for (size_t i = 0; i < n; ++i) {
+            pt.push_back(It[i].data());
+            ps.push_back(Is[i].data());
         }
-        pt.push_back(vt.back()->get());
-        ps.push_back(vs.back()->get());
+        const int d2[2] = {int(It[0].width()), int(It[0].height())};
+        const tiplhip_cdm_param cp = detail::to_c(param);
+        dis.resize(It[0].shape());
+        static_assert(sizeof(bool) == 1, "the terminate flag is read as one byte");
+        int iterations = 0;
+        const int rc = tiplhip_cdm2d(ctx.get(), pt.data(), ps.data(), int(n), d2, &cp, &dis[0][0],
+                                     reinterpret_cast<const volatile uint8_t*>(&terminated),
+                                     &iterations);
+        if constexpr (!std::is_void_v<out_type>)
+            out_type() << "nonlinear registration using gpu: " << iterations << " iterations";
+        if (rc == TIPLHIP_ERR_FAILED) return false;
+        if (rc == TIPLHIP_ERR_TERMINATED) return true;
+        hip::check(rc);
+        return true;
+    } else {
+        std::vector<std::unique_ptr<hip::volume>> vt, vs;
+        std::vector<const tiplhip_volume*> pt, ps;
+        for (size_t i = 0; i < n; ++i) {
+            if constexpr (std::is_same_v<value_type, unsigned char>) {
+                vt.emplace_back(new hip::volume(ctx, It[i].data(), It[i].shape()));
+                vs.emplace_back(new hip::volume(ctx, Is[i].data(), Is[i].shape()));
+            } else {
+                vt.emplace_back(new hip::volume(ctx, It[i].data(), It[i].shape(), 255.99f));
+                vs.emplace_back(new hip::volume(ctx, Is[i].data(), Is[i].shape(), 255.99f));
+            }
+            pt.push_back(vt.back()->get());
+            ps.push_back(vs.back()->get());
+        }
+        const tiplhip_cdm_param cp = detail::to_c(param);
+        dis.resize(It[0].shape());
+        // the flag is polled in place once per iteration (a bool is one byte)
+        static_assert(sizeof(bool) == 1, "the terminate flag is read as one byte");
+        const volatile uint8_t* stop = reinterpret_cast<const volatile uint8_t*>(&terminated);
+        int iterations = 0;
+        const int rc = tiplhip_cdm(ctx.get(), pt.data(), ps.data(), int(n), &cp, &dis[0][0], stop,
+                                   &iterations);
+        if constexpr (!std::is_void_v<out_type>)
+            out_type() << "nonlinear registration using gpu: " << iterations << " iterations";
+        if (rc == TIPLHIP_ERR_FAILED) return false;
+        if (rc == TIPLHIP_ERR_TERMINATED) return true;
+        hip::check(rc);
+        return true;
     }
-    const tiplhip_cdm_param cp = detail::to_c(param);
-    dis.resize(It[0].shape());
-    // the flag is polled in place once per iteration (a bool is one byte)
-    static_assert(sizeof(bool) == 1, "the terminate flag is read as one byte");
-    const volatile uint8_t* stop = reinterpret_cast<const volatile uint8_t*>(&terminated);
-    int iterations = 0;
-    const int rc = tiplhip_cdm(ctx.get(), pt.data(), ps.data(), int(n), &cp, &dis[0][0], stop,
-                               &iterations);
-    if constexpr (!std::is_void_v<out_type>)
-        out_type() << "nonlinear registration using gpu: " << iterations << " iterations";
-    if (rc == TIPLHIP_ERR_FAILED) return false;
-    if (rc == TIPLHIP_ERR_TERMINATED) return true;
-    hip::check(rc);
-    return true;
 }
 
 // tipl::reg::cdm_common (reg/cdm.hpp:534-558).

@@ -6,6 +6,8 @@
 //       parameters and the final cost.
 //   reg_cli cdm <It.u8> <Is.u8> <w> <h> <d> <out.f32>
 //       tipl::reg::cdm_common, writes the displacement field (w*h*d*3 floats).
+//   reg_cli cdm2d <It.u8> <Is.u8> <w> <h> <out.f32>
+//       tipl::reg::cdm_common on 2-D images, writes the field (w*h*2 floats).
 //   reg_cli gaussian <in.f32> <w> <h> <d> <channels> <bandwidth> <out.f32>
 //       tipl::filter::gaussian on a float image (1) or vector<3> field (3).
 //   reg_cli subject <in.f32> <w> <h> <d> <out.u8>
@@ -147,6 +149,21 @@ int main(int argc, char** argv)
             o << vs << R << I;
             if (!o) throw std::runtime_error(o.error_msg);
             return 0;
+        }
+        if (argc >= 7 && std::string(argv[1]) == "cdm2d") {
+            const tipl::shape<2> s2(unsigned(std::atoi(argv[4])), unsigned(std::atoi(argv[5])));
+            auto a = read_u8(argv[2], s2.size());
+            auto b = read_u8(argv[3], s2.size());
+            tipl::image<2, tipl::vector<2>> dis;
+            bool terminated = false;
+            const bool ok = tipl::reg::cdm_common<void>(
+                std::vector<tipl::const_pointer_image<2, unsigned char>>{{a.data(), s2}},
+                std::vector<tipl::const_pointer_image<2, unsigned char>>{{b.data(), s2}}, dis,
+                terminated);
+            std::ofstream out(argv[6], std::ios::binary);
+            out.write(reinterpret_cast<const char*>(&dis[0][0]),
+                      std::streamsize(sizeof(float) * 2 * dis.size()));
+            return ok ? 0 : 1;
         }
         if (argc < 8) {
             std::cerr << "usage: reg_cli linear|cdm ...\n";

@@ -72,6 +72,26 @@ def test_cdm_cli_matches_c_abi(tmp_path, ctx):
     assert np.array_equal(f_cli.view(np.uint32), f.view(np.uint32))
 
 
+def test_cdm2d_cli_matches_c_abi(tmp_path, ctx):
+    """tipl::reg::cdm_common on image<2,uint8> through the C++ mirror equals
+    the C ABI's tiplhip_cdm2d bit for bit."""
+    from scipy.ndimage import shift
+    from test_oracle_cdm2d import image2
+    It = image2((90, 110), 22, sigma=2.0)
+    Is = shift(It.astype(np.float32), (-1.5, 2.0), order=1).astype(np.uint8)
+    a, b, out = tmp_path / "t.u8", tmp_path / "s.u8", tmp_path / "f.f32"
+    It.tofile(a)
+    Is.tofile(b)
+    h, w = It.shape
+    r = subprocess.run([CLI, "cdm2d", str(a), str(b), str(w), str(h), str(out)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    f_cli = np.fromfile(out, np.float32).reshape(It.shape + (2,))
+    rc, f, _ = ctx.cdm2d(It, Is)
+    assert rc == 0 and np.abs(f).max() > 0.1
+    assert np.array_equal(f_cli.view(np.uint32), f.view(np.uint32))
+
+
 @pytest.mark.parametrize("channels,bandwidth", [(1, 1), (3, 8), (1, 3)])
 def test_gaussian_cli_matches_oracle(tmp_path, oracle, channels, bandwidth):
     """tipl::filter::gaussian from the C++ mirror equals the oracle bit for bit."""

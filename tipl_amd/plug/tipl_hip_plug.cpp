@@ -17,10 +17,8 @@
 //     as the reference's optimisers call their functors);
 //   * cdm_cuda<out, unsigned char, 3>: the whole cdm() on the device through
 //     tiplhip_cdm;
-//   * cdm_cuda<out, unsigned char, 2>: libtiplhip has no 2-D CDM; this
-//     specialisation runs the reference's own CPU cdm (reg/cdm.hpp:373-474),
-//     so a program that instantiates the 2-D plug still links and behaves as
-//     the reference does without a GPU.
+//   * cdm_cuda<out, unsigned char, 2>: the whole 2-D cdm() on the device
+//     through tiplhip_cdm2d.
 //
 // Build (as __graft_entry__.build() does, via oracle/Makefile):
 //   g++ -std=c++17 -DTIPL_USE_CUDA -I<reference root> -I<repo>/include -fPIC \
@@ -181,14 +179,49 @@ bool cdm_cuda<tipl::out, unsigned char, 3>(
     }
 }
 
-// No 2-D CDM on the device: the reference's own CPU cdm().
+// The 2-D cdm() on the device (reg/cdm.hpp:373-474 at dimension 2, through
+// tiplhip_cdm2d), with the 3-D plug's return rules.
 template <>
 bool cdm_cuda<tipl::out, unsigned char, 2>(
     const std::vector<tipl::const_pointer_image<2, unsigned char>>& It,
     const std::vector<tipl::const_pointer_image<2, unsigned char>>& Is,
     tipl::image<2, tipl::vector<2>>& d, bool& terminated, const cdm_param& param)
 {
-    return tipl::reg::cdm<tipl::out>(It, Is, d, terminated, param);
+    tiplhip_ctx* ctx = nullptr;
+    try {
+        const size_t n = std::min(It.size(), Is.size());
+        if (n == 0 || n > 8) throw std::runtime_error("cdm_cuda: 1 to 8 modalities");
+        hip_check(tiplhip_ctx_create(0, &ctx));
+        const int dim[2] = {int(It[0].width()), int(It[0].height())};
+        std::vector<const uint8_t*> pt, ps;
+        for (size_t i = 0; i < n; ++i) {
+            pt.push_back(&It[i][0]);
+            ps.push_back(&Is[i][0]);
+        }
+        tiplhip_cdm_param p;
+        tiplhip_cdm_param_default(&p);
+        p.resolution = param.resolution;
+        p.speed = param.speed;
+        p.smoothing = param.smoothing;
+        p.min_dimension = int(param.min_dimension);
+        for (size_t i = 0; i < 8; ++i)
+            p.gradient_type[i] = i < param.gradient_type.size() ? param.gradient_type[i] : 'r';
+        d.resize(It[0].shape());
+        int iterations = 0;
+        const int rc = tiplhip_cdm2d(ctx, pt.data(), ps.data(), int(n), dim, &p, &d[0][0],
+                                     reinterpret_cast<const volatile uint8_t*>(&terminated),
+                                     &iterations);
+        tiplhip_ctx_destroy(ctx);
+        ctx = nullptr;
+        if (rc == TIPLHIP_ERR_FAILED) return false;
+        if (rc != TIPLHIP_OK && rc != TIPLHIP_ERR_TERMINATED)
+            throw std::runtime_error(std::string("tiplhip: ") + tiplhip_last_error());
+        return true;
+    } catch (std::runtime_error& er) {
+        if (ctx) tiplhip_ctx_destroy(ctx);
+        tipl::out() << "❌️" << er.what();
+        return false;
+    }
 }
 
 }  // namespace tipl::reg
