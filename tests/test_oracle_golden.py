@@ -77,3 +77,31 @@ def test_prep_vectors(oracle):
     big = np.random.default_rng(77).gamma(2.0, 100.0, 182 * 218 * 182).astype(np.float32)
     assert np.float32(oracle.median(big)) == G["prep_median_full"]
     assert np.array_equal(oracle.normalize(G["q16_in"], 255.99), G["q16_out"])
+
+
+G2 = np.load(os.path.join(os.path.dirname(__file__), "golden", "cdm2d_vectors.npz"))
+
+
+def test_cdm2d_vectors(oracle):
+    """The 2-D CDM (cdm<out,uint8,2>) steps and one whole run against the
+    vectors tests/golden/make_golden_cdm2d.py wrote from the compiled reference."""
+    assert np.array_equal(oracle.downsample2(G2["down_in"]), G2["down_out"])
+    assert np.array_equal(_bits(oracle.upsample2(G2["up_in"], G2["up_out"].shape[:2])),
+                          _bits(G2["up_out"]))
+    Js = oracle.compose2(G2["Is"], G2["d"])
+    assert np.array_equal(Js, G2["Js"])
+    gr, cr = oracle.gradient2(Js, G2["It"], "r")
+    gd, cd = oracle.gradient2(Js, G2["It"], "d")
+    assert np.array_equal(_bits(gr), _bits(G2["grad_r"]))
+    assert np.array_equal(_bits(gd), _bits(G2["grad_d"]))
+    assert cr == pytest.approx(float(G2["cost_r"]), rel=1e-6)
+    assert cd == pytest.approx(float(G2["cost_d"]), rel=1e-6)
+    p = oracle.poisson2(G2["nd"])
+    assert np.array_equal(_bits(p), _bits(G2["poisson"]))
+    assert np.float32(oracle.max_length2(p)) == G2["maxlen"]
+    assert np.array_equal(_bits(oracle.accumulate2(G2["d"], p)), _bits(G2["accumulate"]))
+    assert np.array_equal(_bits(oracle.smooth2(p, G2["d"], 0.4)), _bits(G2["smooth"]))
+    ok, f, _ = oracle.cdm2(G2["full_It"], G2["full_Is"])
+    assert ok == bool(G2["full_ok"])
+    rms = float(np.sqrt(((f - G2["full_field"]) ** 2).sum(-1).mean()))
+    assert rms <= 1e-6, rms
