@@ -435,6 +435,45 @@ def cdm_config2(ctx):
     return out
 
 
+def cdm2d_bench(ctx, threads, with_cpu, reps=5):
+    """The 2-D CDM (cdm<uint8,2>, the reference's cdm_cuda<out,uint8,2>):
+    whole multi-level cdm() calls on a smooth slice and its smoothly warped
+    copy, on the GPU, and -- with the CPU legs -- the reference's own CPU
+    cdm() on the same pair with the host's threads (one call)."""
+    from scipy.ndimage import gaussian_filter, map_coordinates
+    out = {}
+    for shape in ((218, 182), (512, 512)):
+        rng = np.random.default_rng(5000 + shape[0])
+        a = gaussian_filter(rng.random(shape), 3.0)
+        a = ((a - a.min()) / (a.max() - a.min()) * 255.99).astype(np.uint8)
+        yy, xx = np.meshgrid(np.arange(shape[0]), np.arange(shape[1]), indexing="ij")
+        w = [gaussian_filter(rng.standard_normal(shape), 12.0) for _ in range(2)]
+        w = [2.0 * v / np.abs(v).max() for v in w]
+        b = map_coordinates(a.astype(np.float32), [yy + w[0], xx + w[1]], order=1)
+        b = np.clip(b, 0, 255).astype(np.uint8)
+        ctx.cdm2d(a, b)  # warm-up
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            rc, f, iters = ctx.cdm2d(a, b)
+        dt = (time.perf_counter() - t0) / reps
+        e = {"status": rc, "iterations": iters, "ms_per_call": round(dt * 1e3, 3),
+             "iter_per_s": round(iters / dt, 1)}
+        if with_cpu:
+            from oracle.bindings import load_reference
+            ref = load_reference()
+            if ref is not None:
+                ref.set_threads(threads)
+                t0 = time.perf_counter()
+                ok, fr, _ = ref.cdm2(a, b)
+                dtr = time.perf_counter() - t0
+                e["cpu_reference_ms_per_call"] = round(dtr * 1e3, 1)
+                e["cpu_threads"] = threads
+                e["speedup_vs_cpu"] = round(dtr / dt, 1)
+                e["field_rms_vs_reference"] = float(np.sqrt(((f - fr) ** 2).sum(-1).mean()))
+        out[f"{shape[1]}x{shape[0]}"] = e
+    return out
+
+
 def mm_reg_pipeline(ctx):
     """mm_reg's nonlinear half on the 160x192x160 warped pair with a given
     (identity) affine: J and r_linear, It_match_fov, cdm in both directions
@@ -611,9 +650,13 @@ def main():
         breg = batch_registration_bench(rpairs)
         log(f"batched registrations {breg}")
 
+    host = cpu_info()
+    log("2-D cdm()")
+    cdm2d = cdm2d_bench(ctx, host["threads"], not args.no_cpu)
+    log(f"cdm 2-D {cdm2d}")
+
     cpu = cpu_corr = cpu_cdm = None
     cpu_1core = {}
-    host = cpu_info()
     if not args.no_cpu:
         log(f"CPU reference baselines on {host}")
         F8 = vf.read()
@@ -674,6 +717,7 @@ def main():
         "speedup_same_computation": speedup or None,
         "hbm_copy_gbps": round(copy_gbps, 1),
         "cdm_config2": cdm2,
+        "cdm_2d": cdm2d,
         "mm_reg_pipeline": mmreg,
         "multi_pair": mp_res,
         "serving": serve,
