@@ -27,9 +27,11 @@
 #include "cdm_control.hpp"
 #include "device.hpp"
 #include "field.hpp"
+#include "host_wait.hpp"
 #include "internal.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -57,6 +59,11 @@ Geo2 geo2(int w, int h)
 
 constexpr int kBX = 64, kBY = 4;
 constexpr uint32_t kThreads2 = kBX * kBY;
+
+// A quiet-NaN float pattern the cost reduction never writes, and how long
+// the host spins for an iteration's costs before it synchronises the stream.
+constexpr uint32_t kCostPending2 = 0x7FC0C057u;
+constexpr int kCostSpinUs2 = 2000;
 
 dim3 grid2(const Geo2& g)
 {
@@ -321,17 +328,22 @@ __global__ __launch_bounds__(kThreads2) void gradient2_kernel(const uint8_t* __r
     if (threadIdx.x == 0 && threadIdx.y == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = s;
 }
 
-// float(mean(cost_map)): the block sums added in block order, then / n.
+// float(mean(cost_map)): the block sums added in block order, then / n;
+// also stored into host-visible memory when host_out is set.
 __global__ __launch_bounds__(kThreads2) void cost2_final_kernel(const double* __restrict__ part,
                                                                 uint32_t nparts, uint32_t n,
-                                                                float* __restrict__ out)
+                                                                float* __restrict__ out,
+                                                                float* __restrict__ host_out)
 {
     __shared__ double sh[kThreads2];
     const uint32_t t = threadIdx.y * kBX + threadIdx.x;
     double s = 0.0;
     for (uint32_t i = t; i < nparts; i += kThreads2) s += part[i];
     const double tot = block_sum_f64(s, sh);
-    if (t == 0) *out = float(tot / double(n));
+    if (t == 0) {
+        *out = float(tot / double(n));
+        if (host_out) *host_out = float(tot / double(n));
+    }
 }
 
 // One Jacobi sweep of cdm_solve_poisson in 2-D: v = new_d - (the flat-index
@@ -380,6 +392,84 @@ __global__ __launch_bounds__(kThreads2) void poisson2d_kernel(const float* __res
     v.x *= inv_d2;
     v.y *= inv_d2;
     st2(dst, p, v);
+}
+
+// Two Jacobi sweeps in one launch over a flat-index segment [a, a + L) of
+// the field.  The stencil reaches p +- 1 and p +- w, so the first sweep is
+// evaluated on [a - w, a + L + w) from the previous solve on
+// [a - 2w, a + L + 2w), both held in LDS.  Positions outside the array hold
+// +0.0: subtracting +0.0 leaves a value unchanged bit for bit (x - (+0) == x,
+// -0 included), so every neighbour is subtracted, in the reference's order,
+// and the dropped ones change nothing.  `first`: the previous solve is
+// nd * -1/4 (the initial solve).  Dynamic LDS: (2L + 6w) float2.
+__global__ __launch_bounds__(kThreads2) void poisson2d_x2_kernel(const float* __restrict__ nd,
+                                                                 const float* __restrict__ src,
+                                                                 uint32_t n, uint32_t w, uint32_t L,
+                                                                 int first, float* __restrict__ dst)
+{
+    extern __shared__ float2 sm2[];
+    constexpr float inv_d2 = -0.5f / 2.0f;
+    const int64_t a = int64_t(blockIdx.x) * int64_t(L);
+    const int64_t lo0 = a - 2 * int64_t(w), len0 = int64_t(L) + 4 * int64_t(w);
+    const int64_t lo1 = a - int64_t(w), len1 = int64_t(L) + 2 * int64_t(w);
+    float2* s0 = sm2;
+    float2* s1 = sm2 + len0;
+    const uint32_t t = threadIdx.y * kBX + threadIdx.x;
+    for (int64_t i = t; i < len0; i += kThreads2) {
+        const int64_t q = lo0 + i;
+        float2 v = make_float2(0.0f, 0.0f);
+        if (q >= 0 && q < int64_t(n)) {
+            if (first) {
+                v = ld2(nd, uint32_t(q));
+                v.x *= inv_d2;
+                v.y *= inv_d2;
+            } else {
+                v = ld2(src, uint32_t(q));
+            }
+        }
+        s0[i] = v;
+    }
+    __syncthreads();
+    const int64_t d0 = lo1 - lo0;  // s0 index of s1's first position (= w)
+    for (int64_t i = t; i < len1; i += kThreads2) {
+        const int64_t q = lo1 + i;
+        float2 v = make_float2(0.0f, 0.0f);
+        if (q >= 0 && q < int64_t(n)) {
+            const int64_t k = d0 + i;
+            v = ld2(nd, uint32_t(q));
+            const float2 a1 = s0[k - 1], a2 = s0[k + 1], a3 = s0[k - int64_t(w)], a4 = s0[k + int64_t(w)];
+            v.x -= a1.x;
+            v.y -= a1.y;
+            v.x -= a2.x;
+            v.y -= a2.y;
+            v.x -= a3.x;
+            v.y -= a3.y;
+            v.x -= a4.x;
+            v.y -= a4.y;
+            v.x *= inv_d2;
+            v.y *= inv_d2;
+        }
+        s1[i] = v;
+    }
+    __syncthreads();
+    for (int64_t i = t; i < int64_t(L); i += kThreads2) {
+        const int64_t p = a + i;
+        if (p >= int64_t(n)) break;
+        const int64_t k = int64_t(w) + i;  // s1 index of p
+        float2 v = ld2(nd, uint32_t(p));
+        const float2 a1 = s1[k - 1], a2 = s1[k + 1], a3 = s1[k - int64_t(w)], a4 = s1[k + int64_t(w)];
+        v.x -= a1.x;
+        v.y -= a1.y;
+        v.x -= a2.x;
+        v.y -= a2.y;
+        v.x -= a3.x;
+        v.y -= a3.y;
+        v.x -= a4.x;
+        v.y -= a4.y;
+        v.x *= inv_d2;
+        v.y *= inv_d2;
+        st2(dst, uint32_t(p), v);
+    }
 }
 
 // cdm_max_displacement_length: the largest float(sqrt(x*x + y*y)) as float
@@ -499,27 +589,48 @@ void launch_compose2(tiplhip_ctx* ctx, const uint8_t* Is, const float* d, const 
     TIPLHIP_CHECK(hipGetLastError());
 }
 
-// The gradient of one modality into nd and its float cost into *cost (device).
+// The gradient of one modality into nd and its float cost into *cost
+// (device) and, when host_cost is set, into host-visible memory.
 void launch_gradient2(tiplhip_ctx* ctx, const uint8_t* Js, const uint8_t* It, const Geo2& g,
-                      char type, bool accumulate, float* nd, double* part, float* cost)
+                      char type, bool accumulate, float* nd, double* part, float* cost,
+                      float* host_cost = nullptr)
 {
     hipLaunchKernelGGL(gradient2_kernel, grid2(g), dim3(kBX, kBY), 0, ctx->stream, Js, It, g, type,
                        accumulate ? 1 : 0, nd, part);
     TIPLHIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(cost2_final_kernel, dim3(1), dim3(kBX, kBY), 0, ctx->stream, part, blocks2(g),
-                       g.n, cost);
+                       g.n, cost, host_cost);
     TIPLHIP_CHECK(hipGetLastError());
 }
 
-// `sweeps` Jacobi sweeps ping-ponging a/b; returns the buffer holding the result.
+// Flat outputs per two-sweep launch, and the widest image whose tile
+// ((2L + 6w) float2) stays within 64 KB of LDS; wider images sweep once per
+// launch.
+constexpr uint32_t kSweepTile = 1024;
+constexpr uint32_t kMaxX2Width = 1024;
+
+// `sweeps` Jacobi sweeps ping-ponging a/b (two per launch where the tile
+// fits); returns the buffer holding the result.
 float* launch_poisson2(tiplhip_ctx* ctx, const float* nd, float* a, float* b, const Geo2& g, int sweeps)
 {
     float* src = a;
     float* dst = a;
-    for (int i = 0; i < sweeps; ++i) {
+    const bool x2 = uint32_t(g.w) <= kMaxX2Width;
+    int done = 0;
+    for (int i = 0; done < sweeps; ++i) {
         dst = (i & 1) ? b : a;
-        hipLaunchKernelGGL(poisson2d_kernel, grid2(g), dim3(kBX, kBY), 0, ctx->stream, nd, src, g,
-                           i == 0 ? 1 : 0, dst);
+        if (x2 && sweeps - done >= 2) {
+            const uint32_t w = uint32_t(g.w);
+            const size_t lds = sizeof(float2) * (2 * size_t(kSweepTile) + 6 * size_t(w));
+            hipLaunchKernelGGL(poisson2d_x2_kernel, dim3((g.n + kSweepTile - 1) / kSweepTile),
+                               dim3(kBX, kBY), lds, ctx->stream, nd, src, g.n, w, kSweepTile,
+                               done == 0 ? 1 : 0, dst);
+            done += 2;
+        } else {
+            hipLaunchKernelGGL(poisson2d_kernel, grid2(g), dim3(kBX, kBY), 0, ctx->stream, nd, src, g,
+                               done == 0 ? 1 : 0, dst);
+            done += 1;
+        }
         TIPLHIP_CHECK(hipGetLastError());
         src = dst;
     }
@@ -609,6 +720,12 @@ bool cdm2_level(Cdm2Run& run, const std::vector<const uint8_t*>& It,
     DevMem cur(ctx, fb), nd(ctx, fb), acc(ctx, fb), pa(ctx, fb), pb(ctx, fb), Js(ctx, g.n);
     DevMem part(ctx, sizeof(double) * blocks2(g)), costs(ctx, sizeof(float) * size_t(nmod)),
         word(ctx, sizeof(unsigned int));
+    // the cost reductions also store into ctx->h_cdm: if a step throws while
+    // one is queued, drain the stream before the slots (and the buffers
+    // above) can be reused
+    auto drain = [ctx] { (void)hipStreamSynchronize(ctx->stream); };
+    DrainOnUnwind<decltype(drain)> unwind(drain);
+    ctx->h_cdm.reserve(sizeof(float) * 16);
     copy2(ctx, cur.p, best, fb);
     float theta = 0.0f;
     std::vector<float> cost;
@@ -621,13 +738,26 @@ bool cdm2_level(Cdm2Run& run, const std::vector<const uint8_t*>& It,
             break;
         }
         ++run.iterations;
+        // Each modality's cost also goes to coherent host memory, whose slots
+        // start as a NaN pattern the reduction never writes; the host spins
+        // on them (a stream synchronisation after kCostSpinUs2).  The Jacobi
+        // sweeps do not depend on the host's decision (the reference solves
+        // the same new_d whatever cdm_improved says), so they are queued
+        // before the wait and run during the round trip.
+        uint32_t* slots = ctx->h_cdm.as<uint32_t>();
+        std::fill(slots, slots + nmod, kCostPending2);
+        float* h_cost = static_cast<float*>(ctx->h_cdm.dev);
         for (int i = 0; i < nmod; ++i) {
             launch_compose2(ctx, Is[size_t(i)], cur.as<float>(), g, Js.as<uint8_t>());
             const char type = run.param.gradient_type[i] == 'd' ? 'd' : 'r';
             launch_gradient2(ctx, Js.as<uint8_t>(), It[size_t(i)], g, type, i > 0, nd.as<float>(),
-                             part.as<double>(), costs.as<float>() + i);
+                             part.as<double>(), costs.as<float>() + i, h_cost + i);
         }
-        read_back(ctx, costs.p, c.data(), sizeof(float) * size_t(nmod));
+        float* solved = launch_poisson2(ctx, nd.as<float>(), pa.as<float>(), pb.as<float>(), g, 12);
+        wait_for_slots(static_cast<const volatile uint32_t*>(slots), size_t(nmod), kCostPending2,
+                       std::chrono::microseconds(kCostSpinUs2),
+                       [ctx] { TIPLHIP_CHECK(hipStreamSynchronize(ctx->stream)); });
+        for (int i = 0; i < nmod; ++i) std::memcpy(&c[size_t(i)], &slots[i], sizeof(float));
         float sum_cost = 0.0f;
         for (int i = 0; i < nmod; ++i) {
             sum_cost += c[size_t(i)];
@@ -644,7 +774,6 @@ bool cdm2_level(Cdm2Run& run, const std::vector<const uint8_t*>& It,
             cur_smoothing *= 0.5f;
             cost.clear();
         }
-        float* solved = launch_poisson2(ctx, nd.as<float>(), pa.as<float>(), pb.as<float>(), g, 12);
         if (theta == 0.0f) theta = max_length2(ctx, solved, g, word.as<unsigned int>());
         if (theta == 0.0f) break;
         const float k = run.param.speed / theta;
