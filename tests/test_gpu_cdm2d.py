@@ -39,11 +39,11 @@ def test_gradient2d_bit_exact(ctx, oracle, shape, kind):
     assert c == c_o
 
 
-@pytest.mark.parametrize("sweeps", [1, 2, 3, 12])
-@pytest.mark.parametrize("shape", SHAPES + [(6, 1100), (2, 1024), (700, 3)])
+@pytest.mark.parametrize("sweeps", [1, 2, 3, 5, 12])
+@pytest.mark.parametrize("shape", SHAPES + [(6, 1100), (2, 1024), (700, 3), (9, 500), (11, 384)])
 def test_poisson2d_bit_exact(ctx, oracle, shape, sweeps):
-    """Two sweeps per launch over flat-index tiles up to 1024 pixels wide,
-    one per launch beyond: both bit-exact, odd counts included."""
+    """Four sweeps per launch up to 384 pixels wide, two up to 768, one
+    beyond, over flat-index tiles: all bit-exact, odd counts included."""
     nd = field2(shape, 5, 1.0, 0.0)
     got, ml = ctx.poisson2d(nd, sweeps)
     want = nd.copy()

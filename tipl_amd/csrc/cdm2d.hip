@@ -394,15 +394,18 @@ __global__ __launch_bounds__(kThreads2) void poisson2d_kernel(const float* __res
     st2(dst, p, v);
 }
 
-// Two Jacobi sweeps in one launch over a flat-index segment [a, a + L) of
-// the field.  The stencil reaches p +- 1 and p +- w, so the first sweep is
-// evaluated on [a - w, a + L + w) from the previous solve on
-// [a - 2w, a + L + 2w), both held in LDS.  Positions outside the array hold
-// +0.0: subtracting +0.0 leaves a value unchanged bit for bit (x - (+0) == x,
-// -0 included), so every neighbour is subtracted, in the reference's order,
-// and the dropped ones change nothing.  `first`: the previous solve is
-// nd * -1/4 (the initial solve).  Dynamic LDS: (2L + 6w) float2.
-__global__ __launch_bounds__(kThreads2) void poisson2d_x2_kernel(const float* __restrict__ nd,
+// K Jacobi sweeps in one launch over a flat-index segment [a, a + L) of the
+// field.  The stencil reaches p +- 1 and p +- w, so sweep s (1..K) is
+// evaluated on [a - (K-s)w, a + L + (K-s)w) from sweep s-1's values one w
+// wider on each side; the input solve (sweep 0) covers [a - Kw, a + L + Kw).
+// Two LDS buffers of L + 2Kw values alternate.  Positions outside the array
+// hold +0.0: subtracting +0.0 leaves a value unchanged bit for bit
+// (x - (+0) == x, -0 included), so every neighbour is subtracted, in the
+// reference's order, and the dropped ones change nothing.  `first`: the
+// input solve is nd * -1/4 (the initial solve).  Dynamic LDS:
+// 2 (L + 2Kw) float2.
+template <int K>
+__global__ __launch_bounds__(kThreads2) void poisson2d_xk_kernel(const float* __restrict__ nd,
                                                                  const float* __restrict__ src,
                                                                  uint32_t n, uint32_t w, uint32_t L,
                                                                  int first, float* __restrict__ dst)
@@ -410,65 +413,56 @@ __global__ __launch_bounds__(kThreads2) void poisson2d_x2_kernel(const float* __
     extern __shared__ float2 sm2[];
     constexpr float inv_d2 = -0.5f / 2.0f;
     const int64_t a = int64_t(blockIdx.x) * int64_t(L);
-    const int64_t lo0 = a - 2 * int64_t(w), len0 = int64_t(L) + 4 * int64_t(w);
-    const int64_t lo1 = a - int64_t(w), len1 = int64_t(L) + 2 * int64_t(w);
-    float2* s0 = sm2;
-    float2* s1 = sm2 + len0;
+    const int64_t W = int64_t(w), span = int64_t(L) + 2 * K * W;
+    float2* buf[2] = {sm2, sm2 + span};
     const uint32_t t = threadIdx.y * kBX + threadIdx.x;
-    for (int64_t i = t; i < len0; i += kThreads2) {
-        const int64_t q = lo0 + i;
-        float2 v = make_float2(0.0f, 0.0f);
-        if (q >= 0 && q < int64_t(n)) {
-            if (first) {
+    {
+        const int64_t lo = a - K * W;
+        for (int64_t i = t; i < span; i += kThreads2) {
+            const int64_t q = lo + i;
+            float2 v = make_float2(0.0f, 0.0f);
+            if (q >= 0 && q < int64_t(n)) {
+                if (first) {
+                    v = ld2(nd, uint32_t(q));
+                    v.x *= inv_d2;
+                    v.y *= inv_d2;
+                } else {
+                    v = ld2(src, uint32_t(q));
+                }
+            }
+            buf[0][i] = v;
+        }
+    }
+#pragma unroll
+    for (int s = 1; s <= K; ++s) {
+        __syncthreads();
+        const float2* in = buf[(s - 1) & 1];
+        float2* out = buf[s & 1];
+        const int64_t lo = a - (K - s) * W, len = int64_t(L) + 2 * (K - s) * W;
+        for (int64_t i = t; i < len; i += kThreads2) {
+            const int64_t q = lo + i;
+            if (s == K && q >= int64_t(n)) break;
+            float2 v = make_float2(0.0f, 0.0f);
+            if (q >= 0 && q < int64_t(n)) {
+                const int64_t k = i + W;  // in's index of q
                 v = ld2(nd, uint32_t(q));
+                const float2 a1 = in[k - 1], a2 = in[k + 1], a3 = in[k - W], a4 = in[k + W];
+                v.x -= a1.x;
+                v.y -= a1.y;
+                v.x -= a2.x;
+                v.y -= a2.y;
+                v.x -= a3.x;
+                v.y -= a3.y;
+                v.x -= a4.x;
+                v.y -= a4.y;
                 v.x *= inv_d2;
                 v.y *= inv_d2;
-            } else {
-                v = ld2(src, uint32_t(q));
             }
+            if (s == K)
+                st2(dst, uint32_t(q), v);
+            else
+                out[i] = v;
         }
-        s0[i] = v;
-    }
-    __syncthreads();
-    const int64_t d0 = lo1 - lo0;  // s0 index of s1's first position (= w)
-    for (int64_t i = t; i < len1; i += kThreads2) {
-        const int64_t q = lo1 + i;
-        float2 v = make_float2(0.0f, 0.0f);
-        if (q >= 0 && q < int64_t(n)) {
-            const int64_t k = d0 + i;
-            v = ld2(nd, uint32_t(q));
-            const float2 a1 = s0[k - 1], a2 = s0[k + 1], a3 = s0[k - int64_t(w)], a4 = s0[k + int64_t(w)];
-            v.x -= a1.x;
-            v.y -= a1.y;
-            v.x -= a2.x;
-            v.y -= a2.y;
-            v.x -= a3.x;
-            v.y -= a3.y;
-            v.x -= a4.x;
-            v.y -= a4.y;
-            v.x *= inv_d2;
-            v.y *= inv_d2;
-        }
-        s1[i] = v;
-    }
-    __syncthreads();
-    for (int64_t i = t; i < int64_t(L); i += kThreads2) {
-        const int64_t p = a + i;
-        if (p >= int64_t(n)) break;
-        const int64_t k = int64_t(w) + i;  // s1 index of p
-        float2 v = ld2(nd, uint32_t(p));
-        const float2 a1 = s1[k - 1], a2 = s1[k + 1], a3 = s1[k - int64_t(w)], a4 = s1[k + int64_t(w)];
-        v.x -= a1.x;
-        v.y -= a1.y;
-        v.x -= a2.x;
-        v.y -= a2.y;
-        v.x -= a3.x;
-        v.y -= a3.y;
-        v.x -= a4.x;
-        v.y -= a4.y;
-        v.x *= inv_d2;
-        v.y *= inv_d2;
-        st2(dst, uint32_t(p), v);
     }
 }
 
@@ -603,28 +597,33 @@ void launch_gradient2(tiplhip_ctx* ctx, const uint8_t* Js, const uint8_t* It, co
     TIPLHIP_CHECK(hipGetLastError());
 }
 
-// Flat outputs per two-sweep launch, and the widest image whose tile
-// ((2L + 6w) float2) stays within 64 KB of LDS; wider images sweep once per
-// launch.
+// Flat outputs per multi-sweep launch; a launch runs K = 4 sweeps while the
+// two LDS buffers (2 (L + 8w) float2) fit in 64 KB, K = 2 up to twice that
+// width, and one sweep per launch beyond.
 constexpr uint32_t kSweepTile = 1024;
-constexpr uint32_t kMaxX2Width = 1024;
+constexpr uint32_t kMaxX4Width = 384;
+constexpr uint32_t kMaxX2Width = 768;
 
-// `sweeps` Jacobi sweeps ping-ponging a/b (two per launch where the tile
-// fits); returns the buffer holding the result.
+// `sweeps` Jacobi sweeps ping-ponging a/b (several per launch where the
+// tile fits); returns the buffer holding the result.
 float* launch_poisson2(tiplhip_ctx* ctx, const float* nd, float* a, float* b, const Geo2& g, int sweeps)
 {
     float* src = a;
     float* dst = a;
-    const bool x2 = uint32_t(g.w) <= kMaxX2Width;
+    const uint32_t w = uint32_t(g.w);
+    const dim3 tiles((g.n + kSweepTile - 1) / kSweepTile);
     int done = 0;
     for (int i = 0; done < sweeps; ++i) {
         dst = (i & 1) ? b : a;
-        if (x2 && sweeps - done >= 2) {
-            const uint32_t w = uint32_t(g.w);
-            const size_t lds = sizeof(float2) * (2 * size_t(kSweepTile) + 6 * size_t(w));
-            hipLaunchKernelGGL(poisson2d_x2_kernel, dim3((g.n + kSweepTile - 1) / kSweepTile),
-                               dim3(kBX, kBY), lds, ctx->stream, nd, src, g.n, w, kSweepTile,
-                               done == 0 ? 1 : 0, dst);
+        if (w <= kMaxX4Width && sweeps - done >= 4) {
+            const size_t lds = sizeof(float2) * 2 * (size_t(kSweepTile) + 8 * size_t(w));
+            hipLaunchKernelGGL(poisson2d_xk_kernel<4>, tiles, dim3(kBX, kBY), lds, ctx->stream, nd, src,
+                               g.n, w, kSweepTile, done == 0 ? 1 : 0, dst);
+            done += 4;
+        } else if (w <= kMaxX2Width && sweeps - done >= 2) {
+            const size_t lds = sizeof(float2) * 2 * (size_t(kSweepTile) + 4 * size_t(w));
+            hipLaunchKernelGGL(poisson2d_xk_kernel<2>, tiles, dim3(kBX, kBY), lds, ctx->stream, nd, src,
+                               g.n, w, kSweepTile, done == 0 ? 1 : 0, dst);
             done += 2;
         } else {
             hipLaunchKernelGGL(poisson2d_kernel, grid2(g), dim3(kBX, kBY), 0, ctx->stream, nd, src, g,
