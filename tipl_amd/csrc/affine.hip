@@ -445,6 +445,9 @@ __global__ __launch_bounds__(MiTraits<BINS>::kThreads) void mi_hist_kernel(
     const uint32_t it0 = chunk * g.items_per_block;
     const uint32_t it1 = min(it0 + g.items_per_block, g.items);
     const uint32_t fwh = uint32_t(g.fw) * uint32_t(g.fh);
+    // the exact FP32 cell index needs w*h*d <= 2^24 (sample2_u8_packed_in)
+    const bool small_to = g.to.wh * uint64_t(uint32_t(g.to.d)) <= (uint64_t(1) << 24);
+    const float tw = float(g.to.w), twh = float(g.to.wh);
     for (uint32_t it = it0 + wave; it < it1; it += nwaves) {
         const Item m = decode(g, it, lane);
         if (!m.ok) continue;
@@ -463,20 +466,36 @@ __global__ __launch_bounds__(MiTraits<BINS>::kThreads) void mi_hist_kernel(
         // Two planes per step: both samples' gathers are in flight together.
         // (The correlation kernel's all-inside march does not pay here: 1-2%
         // slower with the wave-aggregated atomics, no faster with plain ones.)
+        // The 256-bin kernel forms the cell index in FP32 when the `to`
+        // volume allows it (sample2_u8_packed_fi).
         uint32_t z = m.z0;
-        for (; z + 1 < m.z1; z += 2, fidx += 2 * fwh) {
-            const uint32_t fva = load_u8(rf, fidx, 0), fvb = load_u8(rf, fidx + fwh, 0);
-            const float4 za = z_products<ZT>(ztab, zrow, z, T), zb = z_products<ZT>(ztab, zrow, z + 1, T);
-            const float pxa = (cs.hx + za.x) + T[9];
-            const float pya = (cs.hy + za.y) + T[10];
-            const float pza = (cs.hz + za.z) + T[11];
-            const float pxb = (cs.hx + zb.x) + T[9];
-            const float pyb = (cs.hy + zb.y) + T[10];
-            const float pzb = (cs.hz + zb.z) + T[11];
-            uint32_t tva, tvb;
-            sample2_u8_packed(r8, g.to, pxa, pya, pza, pxb, pyb, pzb, tva, tvb);
-            count(fva, tva);
-            count(fvb, tvb);
+        auto march = [&](auto fi_tag) {
+            constexpr bool kFi = decltype(fi_tag)::value;
+            for (; z + 1 < m.z1; z += 2, fidx += 2 * fwh) {
+                const uint32_t fva = load_u8(rf, fidx, 0), fvb = load_u8(rf, fidx + fwh, 0);
+                const float4 za = z_products<ZT>(ztab, zrow, z, T), zb = z_products<ZT>(ztab, zrow, z + 1, T);
+                const float pxa = (cs.hx + za.x) + T[9];
+                const float pya = (cs.hy + za.y) + T[10];
+                const float pza = (cs.hz + za.z) + T[11];
+                const float pxb = (cs.hx + zb.x) + T[9];
+                const float pyb = (cs.hy + zb.y) + T[10];
+                const float pzb = (cs.hz + zb.z) + T[11];
+                uint32_t tva, tvb;
+                if constexpr (kFi)
+                    sample2_u8_packed_fi(r8, g.to, tw, twh, pxa, pya, pza, pxb, pyb, pzb, tva, tvb);
+                else
+                    sample2_u8_packed(r8, g.to, pxa, pya, pza, pxb, pyb, pzb, tva, tvb);
+                count(fva, tva);
+                count(fvb, tvb);
+            }
+        };
+        if constexpr (BINS == 256) {
+            if (small_to)
+                march(std::true_type{});
+            else
+                march(std::false_type{});
+        } else {
+            march(std::false_type{});
         }
         if (z < m.z1) {
             const uint32_t fv = load_u8(rf, fidx, 0);

@@ -237,6 +237,29 @@ __device__ inline void sample2_u8_packed_in(__amdgpu_buffer_rsrc_t r8, float fw,
     tb = blend(qb[0], qb[1], pxb, pyb, pzb);
 }
 
+// sample2_u8_packed with the cell index formed in FP32 as in
+// sample2_u8_packed_in (floor as x - fract(x), two exact fused multiply-adds,
+// one conversion), for a `to` volume of at most 2^24 voxels.  An outside
+// sample gets an arbitrary index (the buffer's range check makes any offset
+// safe) and is then forced to 0, as in sample2_u8_packed.  Measured at
+// 182x218x182 with 64 candidates: the 256-bin histogram kernel 1-3% faster,
+// the 64-bin one 4.5% slower (which keeps sample2_u8_packed).
+__device__ inline void sample2_u8_packed_fi(__amdgpu_buffer_rsrc_t r8, const VolGeo& g, float fw,
+                                            float fwh, float xa, float ya, float za, float xb,
+                                            float yb, float zb, uint32_t& ta, uint32_t& tb)
+{
+    const bool ina = (__float_as_uint(xa) < __float_as_uint(g.wm1)) &
+                     (__float_as_uint(ya) < __float_as_uint(g.hm1)) &
+                     (__float_as_uint(za) < __float_as_uint(g.dm1));
+    const bool inb = (__float_as_uint(xb) < __float_as_uint(g.wm1)) &
+                     (__float_as_uint(yb) < __float_as_uint(g.hm1)) &
+                     (__float_as_uint(zb) < __float_as_uint(g.dm1));
+    uint32_t va, vb;
+    sample2_u8_packed_in(r8, fw, fwh, xa, ya, za, xb, yb, zb, va, vb);
+    ta = ina ? va : 0u;
+    tb = inb ? vb : 0u;
+}
+
 // Hardware dispatches workgroup b of a 1-D grid to XCD b % 8.  This remap
 // gives each XCD a contiguous range of logical workgroups so that the
 // volume region a range touches stays in that XCD's L2.  Bijective on
