@@ -18,10 +18,13 @@
 //                               numerical/resampling.hpp:575-673, 849-891
 //   cdm / cdm_cuda              reg/cdm.hpp:373-532
 //
-// A slice is 10^4..10^6 pixels, so every step is one plain kernel with one
-// thread per pixel on 64 x 4 tiles; the loop control (cost history,
-// cdm_improved, smoothing schedule) runs on the host from the modality costs
-// read back once per iteration, as in the 3-D loop.  Fields are packed float2
+// A slice is 10^4..10^6 pixels, a fraction of the chip, so the loop is bound
+// by launches: each step is one kernel with one thread per pixel on 64 x 4
+// tiles, except the Jacobi solve, which runs four (or two) sweeps per launch
+// over flat-index tiles held in LDS.  The loop control (cost history,
+// cdm_improved, smoothing schedule) runs on the host from the modality costs,
+// which the cost reduction stores into coherent host memory while the Jacobi
+// sweeps are already queued, as in the 3-D loop.  Fields are packed float2
 // per pixel, x-fastest (the reference's image<2,vector<2>>).  Every FP32
 // operation is a separately rounded operation in the reference's order.
 #include "cdm_control.hpp"
