@@ -1299,7 +1299,7 @@ void cost_noalloc(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip_vo
     LaunchShape shape = default_shape(cost, bins);
     auto it = ctx->tuned.find(tune_key(from, to, K, cost, bins));
     if (it != ctx->tuned.end()) shape = it->second;
-    Segment sg;
+    Segment sg{};  // every field zero until set below
     sg.g = batch_geometry(from, to, K, shape.zc);
     const double work = double(sg.g.items) * double(K);
     const double target = shape.per_cu == 0
