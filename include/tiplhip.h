@@ -80,8 +80,11 @@ int tiplhip_ctx_destroy(tiplhip_ctx* ctx);
 int tiplhip_ctx_synchronize(tiplhip_ctx* ctx);
 
 /* Device time accounting: when enabled, the affine cost entry points record
- * HIP events around their main kernel on the context stream and accumulate
- * the elapsed time.  name is "affine_cost" (the dominant kernel). */
+ * HIP events around their main kernel on the context stream; the host does
+ * not wait for them, so a timed run keeps its launch rate.  The pairs are
+ * resolved (waiting for the last one) when tiplhip_ctx_kernel_time reads the
+ * totals, and every 512 launches.  Enabling or disabling resets the totals
+ * and drops pairs not yet read. */
 int tiplhip_ctx_set_kernel_timing(tiplhip_ctx* ctx, int enable);
 int tiplhip_ctx_kernel_time(tiplhip_ctx* ctx, double* total_ms, int64_t* launches);
 

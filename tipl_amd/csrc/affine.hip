@@ -738,20 +738,30 @@ void check_volume(const tiplhip_ctx* ctx, const tiplhip_volume* v, const char* w
                     std::string(what) + " plane exceeds 2^24 voxels");
 }
 
+// Kernel timing brackets each timed launch with a pair of events and does
+// not wait for them, so timed runs keep their throughput; timing_flush
+// resolves the pairs (at most kTimingPairs are held before a flush).
+constexpr size_t kTimingPairs = 512;
+
 void timing_begin(tiplhip_ctx* ctx)
 {
-    if (ctx->kernel_timing) TIPLHIP_CHECK(hipEventRecord(ctx->k0, ctx->stream));
+    if (!ctx->kernel_timing) return;
+    if (ctx->kt_pending == kTimingPairs) timing_flush(ctx);
+    if (ctx->kt_events.size() < 2 * (ctx->kt_pending + 1)) {
+        for (int i = 0; i < 2; ++i) {
+            hipEvent_t e = nullptr;
+            TIPLHIP_CHECK(hipEventCreate(&e));
+            ctx->kt_events.push_back(e);
+        }
+    }
+    TIPLHIP_CHECK(hipEventRecord(ctx->kt_events[2 * ctx->kt_pending], ctx->stream));
 }
 
 void timing_end(tiplhip_ctx* ctx)
 {
     if (!ctx->kernel_timing) return;
-    TIPLHIP_CHECK(hipEventRecord(ctx->k1, ctx->stream));
-    TIPLHIP_CHECK(hipEventSynchronize(ctx->k1));
-    float ms = 0.0f;
-    TIPLHIP_CHECK(hipEventElapsedTime(&ms, ctx->k0, ctx->k1));
-    ctx->kernel_ms += ms;
-    ctx->kernel_launches += 1;
+    TIPLHIP_CHECK(hipEventRecord(ctx->kt_events[2 * ctx->kt_pending + 1], ctx->stream));
+    ++ctx->kt_pending;
 }
 
 void zero_u64(tiplhip_ctx* ctx, void* p, size_t n)
@@ -816,14 +826,16 @@ constexpr double kMaxItemsPerBlock = 256.0;
 // Measured on the device over 48^3 .. 182x218x182 pairs with 12 and 64
 // candidates: the 256-bin histogram (128 KB of LDS, one 1024-thread
 // workgroup per CU) is fastest with about one workgroup per CU -- fewer,
-// fuller workgroups amortise the LDS clear and flush -- and the 64-bin one
-// with about eight.  (Splitting the 256-bin histogram into 2 or 4 ranges
+// fuller workgroups amortise the LDS clear and flush -- while the 64-bin and
+// correlation kernels gain from finer slices up to 32-64 per CU (64-bin MI at
+// 182x218x182, 64 candidates: 0.76-0.79 ms with 8, 0.65 with 16, 0.60 with
+// 32; the same at 48^3-128^3 within noise).  (Splitting the 256-bin histogram into 2 or 4 ranges
 // of `from` rows, one workgroup each with a half or a quarter of the LDS,
 // was slower at every size from 64^3 with 24 or more candidates: 1.4x and
 // 2.6x at 182x218x182, 64 candidates.)
 LaunchShape default_shape(int cost, int bins)
 {
-    return LaunchShape{16u, hist256(cost, bins) ? 1u : 8u};
+    return LaunchShape{16u, hist256(cost, bins) ? 1u : 32u};
 }
 
 TuneKey tune_key(const tiplhip_volume* from, const tiplhip_volume* to, int K, int cost, int bins)
@@ -1491,36 +1503,62 @@ int tiplhip_autotune(tiplhip_ctx* ctx, const tiplhip_volume* from, const tiplhip
         }
         volume_ensure_packed(ctx, const_cast<tiplhip_volume*>(t));
         const bool wide = hist256(cost, bins);  // 1024-thread workgroups
+        // measured at 182x218x182 with 64 candidates: the 64-bin MI and
+        // correlation kernels keep getting faster up to 48-64 workgroups per
+        // CU (finer slices of the work; 0.647 -> 0.584 ms from 16 to 64 for
+        // MI), the 256-bin one is indifferent from 0 to 4
         const uint32_t zcs[] = {8u, 16u, 32u};
-        const uint32_t pcs_narrow[] = {2u, 4u, 8u, 16u, 0u}, pcs_wide[] = {1u, 2u, 4u, 8u, 0u};
-        const uint32_t* pcs = wide ? pcs_wide : pcs_narrow;
-        LaunchShape best = default_shape(cost, bins);
-        double best_t = 1e30;
+        const std::vector<uint32_t> pcs = wide ? std::vector<uint32_t>{1u, 2u, 4u, 8u, 0u}
+                                               : std::vector<uint32_t>{2u, 4u, 8u, 16u, 32u, 48u, 64u, 0u};
+        timing_flush(ctx);  // the caller's own pending launches count for the caller
         const bool keep_timing = ctx->kernel_timing;
         const double keep_ms = ctx->kernel_ms;
         const int64_t keep_n = ctx->kernel_launches;
         ctx->kernel_timing = true;
-        for (uint32_t z : zcs)
-            for (int p = 0; p < 5; ++p) {
-                const LaunchShape shape{z, pcs[p]};
-                const Staged st = stage_jobs(ctx, {PairJob{f, t, K}}, Tk.data(),
-                                             crop.empty() ? nullptr : crop.data(), cost, bins, &shape);
+        // `n` timed launches of one shape (after one that warms up), appended
+        // to `times`
+        auto time_shape = [&](const LaunchShape& shape, int n, std::vector<double>& times) {
+            const Staged st = stage_jobs(ctx, {PairJob{f, t, K}}, Tk.data(),
+                                         crop.empty() ? nullptr : crop.data(), cost, bins, &shape);
+            for (int r = 0; r <= n; ++r) {
                 ctx->kernel_ms = 0.0;
-                ctx->kernel_launches = 0;
-                for (int r = 0; r <= reps; ++r) {  // the first launch warms up
-                    if (r == 1) {
-                        ctx->kernel_ms = 0.0;
-                        ctx->kernel_launches = 0;
-                    }
-                    if (cost == TIPLHIP_COST_CORR) launch_corr_sums(ctx, st);
-                    else launch_mi_hist(ctx, st, bins);
-                }
-                const double avg = ctx->kernel_ms / double(std::max<int64_t>(1, ctx->kernel_launches));
-                if (avg < best_t) {
-                    best_t = avg;
-                    best = shape;
-                }
+                if (cost == TIPLHIP_COST_CORR) launch_corr_sums(ctx, st);
+                else launch_mi_hist(ctx, st, bins);
+                timing_flush(ctx);
+                if (r > 0) times.push_back(ctx->kernel_ms);
             }
+        };
+        auto median = [](std::vector<double> v) {
+            std::sort(v.begin(), v.end());
+            return v[v.size() / 2];
+        };
+        // Two passes against the noise of single launches (clocks drift by a
+        // few percent within a run): every shape by the median of `reps`
+        // launches, then the four best again, interleaved, by the median of
+        // all their launches.  A lone minimum over two dozen shapes picks
+        // lucky samples: (8, 48) measured 0.58 ms once and ran at 0.67 ms.
+        std::vector<std::pair<double, LaunchShape>> first;
+        for (uint32_t z : zcs)
+            for (uint32_t pc : pcs) {
+                std::vector<double> v;
+                time_shape(LaunchShape{z, pc}, reps, v);
+                first.emplace_back(median(v), LaunchShape{z, pc});
+            }
+        std::sort(first.begin(), first.end(),
+                  [](const auto& a, const auto& b) { return a.first < b.first; });
+        const size_t finalists = std::min<size_t>(4, first.size());
+        std::vector<std::vector<double>> again(finalists);
+        for (int round = 0; round < 3; ++round)
+            for (size_t i = 0; i < finalists; ++i) time_shape(first[i].second, reps, again[i]);
+        LaunchShape best = first[0].second;
+        double best_t = 1e30;
+        for (size_t i = 0; i < finalists; ++i) {
+            const double m = median(again[i]);
+            if (m < best_t) {
+                best_t = m;
+                best = first[i].second;
+            }
+        }
         ctx->kernel_timing = keep_timing;
         ctx->kernel_ms = keep_ms;
         ctx->kernel_launches = keep_n;

@@ -432,6 +432,20 @@ void volume_ensure_packed(tiplhip_ctx* ctx, tiplhip_volume* vol)
     vol->d_pack = static_cast<uint64_t*>(p);
 }
 
+void timing_flush(tiplhip_ctx* ctx)
+{
+    if (ctx->kt_pending == 0) return;
+    const size_t n = ctx->kt_pending;
+    ctx->kt_pending = 0;
+    TIPLHIP_CHECK(hipEventSynchronize(ctx->kt_events[2 * n - 1]));
+    for (size_t i = 0; i < n; ++i) {
+        float ms = 0.0f;
+        TIPLHIP_CHECK(hipEventElapsedTime(&ms, ctx->kt_events[2 * i], ctx->kt_events[2 * i + 1]));
+        ctx->kernel_ms += ms;
+        ctx->kernel_launches += 1;
+    }
+}
+
 void read_back(tiplhip_ctx* ctx, const void* dev, void* host, size_t bytes)
 {
     ctx->h_out.reserve(bytes);
@@ -608,8 +622,6 @@ int tiplhip_ctx_create(int device, tiplhip_ctx** out)
             TIPLHIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
             TIPLHIP_CHECK(hipEventCreate(&c->t0));
             TIPLHIP_CHECK(hipEventCreate(&c->t1));
-            TIPLHIP_CHECK(hipEventCreate(&c->k0));
-            TIPLHIP_CHECK(hipEventCreate(&c->k1));
             TIPLHIP_CHECK(hipEventCreateWithFlags(&c->e_read, hipEventDisableTiming));
         } catch (...) {
             if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -628,8 +640,10 @@ int tiplhip_ctx_destroy(tiplhip_ctx* ctx)
         {
             DeviceGuard dg(ctx->device);
             (void)hipStreamSynchronize(ctx->stream);
-            for (hipEvent_t e : {ctx->t0, ctx->t1, ctx->k0, ctx->k1, ctx->e_read})
+            for (hipEvent_t e : {ctx->t0, ctx->t1, ctx->e_read})
                 if (e) (void)hipEventDestroy(e);
+            for (hipEvent_t e : ctx->kt_events) (void)hipEventDestroy(e);
+            ctx->kt_events.clear();
             clear_graphs(ctx);
             ctx->d_T.release();
             ctx->d_hist.release();
@@ -677,6 +691,7 @@ int tiplhip_ctx_set_kernel_timing(tiplhip_ctx* ctx, int enable)
         TIPLHIP_REQUIRE(ctx != nullptr, "null context");
         std::lock_guard<std::recursive_mutex> lk(ctx->mu);
         ctx->kernel_timing = enable != 0;
+        ctx->kt_pending = 0;  // pairs not yet read are dropped with the totals
         ctx->kernel_ms = 0.0;
         ctx->kernel_launches = 0;
     });
@@ -687,6 +702,8 @@ int tiplhip_ctx_kernel_time(tiplhip_ctx* ctx, double* total_ms, int64_t* launche
     return guarded([&] {
         TIPLHIP_REQUIRE(ctx != nullptr, "null context");
         std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+        DeviceGuard dg(ctx->device);
+        timing_flush(ctx);
         if (total_ms) *total_ms = ctx->kernel_ms;
         if (launches) *launches = ctx->kernel_launches;
     });

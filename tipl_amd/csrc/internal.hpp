@@ -12,6 +12,7 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "../../include/tiplhip.h"
 #include "errors.hpp"
@@ -147,7 +148,7 @@ struct tiplhip_ctx {
     // autotuned launch shapes of single-pair cost launches
     std::map<tiplhip::TuneKey, tiplhip::LaunchShape> tuned;
     // timing
-    hipEvent_t t0 = nullptr, t1 = nullptr, k0 = nullptr, k1 = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
     hipEvent_t e_read = nullptr;     // marks a device-to-host copy the host waits for
     bool kernel_timing = false;
     bool use_graphs = false;         // see tiplhip_ctx_set_graphs
@@ -158,6 +159,10 @@ struct tiplhip_ctx {
     std::shared_ptr<tiplhip::CdmBatcher> cdm_batch;
     double kernel_ms = 0.0;
     int64_t kernel_launches = 0;
+    // kernel timing without host synchronisation: event pairs recorded
+    // around each timed launch, resolved into kernel_ms by timing_flush
+    std::vector<hipEvent_t> kt_events;
+    size_t kt_pending = 0;
 };
 
 struct tiplhip_volume {
@@ -201,6 +206,10 @@ void volume_compute_stats(tiplhip_ctx* ctx, tiplhip_volume* vol);
 // After vol->d was rewritten on ctx's stream: drops the packed copy and
 // recomputes the statistics.
 void volume_invalidate(tiplhip_ctx* ctx, tiplhip_volume* vol);
+
+// Resolves the context's recorded kernel-timing event pairs into kernel_ms
+// and kernel_launches (waits for the last one).
+void timing_flush(tiplhip_ctx* ctx);
 
 // cdm() / the fixed-schedule extension into the device field d_field
 // (vox(It[0]->dim) x 3 floats, zeroed here) on ctx's stream.  Returns
